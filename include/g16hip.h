@@ -1,0 +1,197 @@
+/*
+ * g16hip — the C ABI of the MI355X (gfx950) Groth16 prover back end.
+ *
+ * This is the boundary a thin Rust crate `crates/hip` would bind to implement the
+ * reference's back-end contract, `snarkrs_groth16::{Backend, PreparedCircuit}`
+ * (crates/groth16/src/lib.rs:371-455), and its standalone MSM plugin API,
+ * `snarkrs_msm::MsmBackend` (crates/msm/src/lib.rs:52-82). Every entry point below names
+ * the reference interface it replaces.
+ *
+ * Wire format (crates/gpu-layout/src/lib.rs:75-185, crates/formats/src/binfile.rs:10-25):
+ *   - a field element is 32 bytes, 8 little-endian u32 limbs. Fr values (witness, r, s,
+ *     key coefficients, H) are MONTGOMERY form with R = 2^256 (arkworks' own limbs, so a
+ *     Rust caller passes `Fr.0.0` unchanged);
+ *   - a G1 affine point is x then y (64 bytes), a G2 affine point x then y with each Fq2
+ *     stored c0 then c1 (128 bytes); coordinates are Montgomery; the all-zero encoding is
+ *     the point at infinity;
+ *   - every output point is affine in the same encoding.
+ *
+ * Errors: every call returns a g16hip_status that maps 1:1 onto ProveError
+ * (crates/groth16/src/lib.rs:79-154); g16hip_last_error() returns the message of the last
+ * failure on the calling thread.
+ *
+ * Threading: a prepared circuit may be used from many threads at once
+ * (crates/groth16/src/lib.rs:380-381); calls on one circuit are serialised internally on
+ * that circuit's own HIP stream and scratch space. Different circuits run concurrently.
+ */
+#ifndef G16HIP_H
+#define G16HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ProveError, crates/groth16/src/lib.rs:79-154. */
+typedef enum {
+  G16HIP_OK = 0,
+  G16HIP_WITNESS_LENGTH = 1, /* ProveError::WitnessLength { got, want } */
+  G16HIP_BACKEND = 2,        /* ProveError::Backend: malformed key/arguments, deterministic */
+  G16HIP_DEVICE = 3,         /* ProveError::Device: a device fault, may succeed on retry */
+  G16HIP_DEVICE_MEMORY = 4,  /* ProveError::DeviceMemory: refused before allocating */
+  G16HIP_CONSTANT_WIRE = 5,  /* ProveError::ConstantWire: witness[0] != 1 */
+  G16HIP_ZERO_BLINDER = 6    /* ProveError::ZeroBlinder: r or s is zero */
+} g16hip_status;
+
+typedef struct g16hip_device g16hip_device;
+typedef struct g16hip_circuit g16hip_circuit;
+/* HPoly::Device (crates/groth16/src/lib.rs:194-218): H left on the device by compute_h. */
+typedef struct g16hip_h g16hip_h;
+
+/*
+ * A parsed proving key, host memory, borrowed for the duration of g16hip_prepare.
+ * Field meanings follow `ProvingKey` (crates/formats/src/lib.rs:35-75) with section 4 in
+ * CSR form (`Coefficients`, crates/formats/src/lib.rs:118-150): for matrix m (0 = A,
+ * 1 = B), row c's nonzeros are k in [row_ptr[m][c], row_ptr[m][c+1]); each is
+ * values[value_id[m][k]] * witness[signal[m][k]]; values are Fr Montgomery (the section 4
+ * double-Montgomery bytes already divided by R, crates/formats/src/binfile.rs:488-503).
+ */
+typedef struct {
+  uint32_t n_vars;
+  uint32_t n_public;
+  uint32_t domain_size; /* power of two, <= 2^27 */
+  uint32_t n_values;
+  const uint8_t* alpha_g1; /* G1 */
+  const uint8_t* beta_g1;  /* G1 */
+  const uint8_t* beta_g2;  /* G2 */
+  const uint8_t* gamma_g2; /* G2 */
+  const uint8_t* delta_g1; /* G1 */
+  const uint8_t* delta_g2; /* G2 */
+  const uint8_t* ic;       /* G1[n_public + 1] */
+  const uint32_t* row_ptr[2];  /* [domain_size + 1] each */
+  const uint32_t* signal[2];   /* [row_ptr[m][domain_size]] */
+  const uint32_t* value_id[2]; /* [row_ptr[m][domain_size]] */
+  const uint8_t* values;       /* Fr[n_values] */
+  const uint8_t* a_query;    /* section 5, G1[n_vars] */
+  const uint8_t* b_g1_query; /* section 6, G1[n_vars] */
+  const uint8_t* b_g2_query; /* section 7, G2[n_vars] */
+  const uint8_t* l_query;    /* section 8, G1[n_vars - n_public - 1] */
+  const uint8_t* h_query;    /* section 9, G1[domain_size] */
+} g16hip_key;
+
+/* StageTimings, crates/groth16/src/lib.rs:322-368 (microseconds; calls add to them). */
+typedef struct {
+  uint64_t gather_us;
+  uint64_t ntt_us;
+  uint64_t pointwise_us;
+  uint64_t msm_us;
+  uint64_t assemble_us;
+  uint64_t verify_us;
+} g16hip_timings;
+
+/* MsmOutputs, crates/groth16/src/lib.rs:311-318, each point affine. */
+typedef struct {
+  uint8_t a_g1[64];
+  uint8_t b_g2[128];
+  uint8_t b_g1[64];
+  uint8_t l_g1[64];
+  uint8_t h_g1[64];
+} g16hip_msm_out;
+
+/* Proof, crates/groth16/src/lib.rs:175-180. */
+typedef struct {
+  uint8_t a[64];
+  uint8_t b[128];
+  uint8_t c[64];
+} g16hip_proof;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* g16hip_last_error(void);
+
+/* Opens GPU `ordinal`; replaces CudaBackend::new (crates/cuda/src/backend.rs:152-184). */
+g16hip_status g16hip_open(int ordinal, g16hip_device** out);
+void g16hip_close(g16hip_device* dev);
+/* Free device memory in bytes, the admission budget (crates/cuda/src/backend.rs:626-643). */
+g16hip_status g16hip_mem_info(g16hip_device* dev, size_t* free_bytes, size_t* total_bytes);
+
+/* Backend::prepare (crates/groth16/src/lib.rs:371-378): validates the key
+ * (crates/groth16/src/cpu.rs:73-122), uploads CSR, twiddles, coset table and the five base
+ * vectors, and keeps them resident. The key is only read. */
+g16hip_status g16hip_prepare(g16hip_device* dev, const g16hip_key* key, g16hip_circuit** out);
+void g16hip_circuit_release(g16hip_circuit* c);
+/* PreparedCircuit::{n_vars, n_public, domain_size} (crates/groth16/src/lib.rs:382-387). */
+g16hip_status g16hip_circuit_dims(const g16hip_circuit* c, uint32_t* n_vars, uint32_t* n_public,
+                                  uint32_t* domain_size);
+/* Device bytes this circuit holds resident (key + per-proof scratch). */
+size_t g16hip_circuit_resident_bytes(const g16hip_circuit* c);
+
+/* PreparedCircuit::compute_h (crates/groth16/src/lib.rs:398-399): stages 0-4. `witness` is
+ * Fr[n_witness] Montgomery in host memory; H stays on the device in *h_out. */
+g16hip_status g16hip_compute_h(g16hip_circuit* c, const uint8_t* witness, size_t n_witness,
+                               g16hip_h** h_out, g16hip_timings* t);
+/* PreparedCircuit::h_to_host (crates/groth16/src/lib.rs:446-454): H (Fr[domain_size],
+ * Montgomery) copied down. Debug/test only. */
+g16hip_status g16hip_h_to_host(g16hip_circuit* c, const g16hip_h* h, uint8_t* out);
+void g16hip_h_release(g16hip_h* h);
+
+/* PreparedCircuit::msms (crates/groth16/src/lib.rs:401-410): stages 5-9. `h` must come from
+ * g16hip_compute_h on this circuit; another circuit's handle is refused (require_handle,
+ * crates/groth16/src/lib.rs:285-307). */
+g16hip_status g16hip_msms(g16hip_circuit* c, const uint8_t* witness, size_t n_witness,
+                          const g16hip_h* h, g16hip_msm_out* out, g16hip_timings* t);
+
+/* PreparedCircuit::h_and_msms (crates/groth16/src/lib.rs:412-423): stages 0-9. */
+g16hip_status g16hip_h_and_msms(g16hip_circuit* c, const uint8_t* witness, size_t n_witness,
+                                g16hip_msm_out* out, g16hip_timings* t);
+/* The same with the witness already in device memory (Fr[n_witness] Montgomery). */
+g16hip_status g16hip_h_and_msms_device(g16hip_circuit* c, const void* d_witness,
+                                       size_t n_witness, g16hip_msm_out* out, g16hip_timings* t);
+
+/* prove_with_blinders (crates/groth16/src/prove.rs:175-229): stages 0-11 with caller r, s
+ * (Fr Montgomery), checking the witness shape first (prove.rs:272-283). Assembly (stage 11)
+ * runs on the host like the reference's, its blinder terms overlapped with the GPU work. */
+g16hip_status g16hip_prove_with_blinders(g16hip_circuit* c, const uint8_t* witness,
+                                         size_t n_witness, const uint8_t* r, const uint8_t* s,
+                                         g16hip_proof* proof, g16hip_timings* t);
+g16hip_status g16hip_prove_with_blinders_device(g16hip_circuit* c, const void* d_witness,
+                                                size_t n_witness, const uint8_t* r,
+                                                const uint8_t* s, g16hip_proof* proof,
+                                                g16hip_timings* t);
+
+/* MsmBackend::try_msm_g1 / try_msm_g2 (crates/msm/src/lib.rs:61-81): bases affine, scalars
+ * Fr Montgomery (as `&[Fr]` is), host memory; result affine. Empty input is infinity. */
+g16hip_status g16hip_msm_g1(g16hip_device* dev, const uint8_t* bases, const uint8_t* scalars,
+                            size_t n, uint8_t* out);
+g16hip_status g16hip_msm_g2(g16hip_device* dev, const uint8_t* bases, const uint8_t* scalars,
+                            size_t n, uint8_t* out);
+/* The same with bases and scalars already in device memory; window width c (0 = the
+ * library's choice). Used for the resident-input MSM throughput figure. */
+g16hip_status g16hip_msm_g1_device(g16hip_device* dev, const void* d_bases, const void* d_scalars,
+                                   size_t n, int c, uint8_t* out);
+g16hip_status g16hip_msm_g2_device(g16hip_device* dev, const void* d_bases, const void* d_scalars,
+                                   size_t n, int c, uint8_t* out);
+
+/* Standalone NTT over Fr, in place on host memory (Fr[2^log_n] Montgomery, natural order):
+ * kind 0 = forward, 1 = inverse (with 1/n), 2 = the prover's coset pipeline
+ * iNTT -> x shift^j -> NTT (CpuNtt::{ntt, intt_coset_ntt}, crates/ntt/src/lib.rs:415, :508). */
+g16hip_status g16hip_ntt(g16hip_device* dev, uint8_t* data, int log_n, int kind);
+g16hip_status g16hip_ntt_device(g16hip_device* dev, void* d_data, int log_n, int kind);
+
+/* Field arithmetic probe (the role of crates/cuda/src/kernels/field_probe.cu:16-112):
+ * field 0 = Fr, 1 = Fq, 2 = Fq2; op 0 add, 1 sub, 2 mul, 3 sqr, 4 inv, 5 neg,
+ * 6 to Montgomery, 7 from Montgomery. n elements, host memory. */
+g16hip_status g16hip_field_probe(g16hip_device* dev, int field, int op, const uint8_t* a,
+                                 const uint8_t* b, uint8_t* out, size_t n);
+
+/* Batched fixed-base multiplication k_i * G (G the standard generator of `group`, 1 or 2),
+ * k Fr Montgomery, output affine: the device half of the synthetic setup that builds the
+ * proving keys the benchmarks use (the role of crates/ceremony's setup MSM seam). */
+g16hip_status g16hip_fixed_base(g16hip_device* dev, int group, const uint8_t* k, size_t n,
+                                uint8_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* G16HIP_H */
