@@ -1,0 +1,480 @@
+// BN254 field and curve arithmetic for gfx950, usable from host and device code.
+//
+// Representation (the packed wire format of crates/gpu-layout/src/lib.rs:75-185):
+//   * Fr, Fq: 8 little-endian u32 limbs, Montgomery form with R = 2^256, fully reduced;
+//   * Fq2 = Fq[u]/(u^2 + 1), c0 then c1;
+//   * affine points x, y with the all-zero encoding for infinity;
+//   * accumulators in XYZZ coordinates (x = X/ZZ, y = Y/ZZZ, ZZ^3 = ZZZ^2), infinity = ZZ 0.
+//
+// The multiply is CIOS over 32-bit limbs: every partial product is a 32x32->64 VALU
+// multiply-add (v_mad_u64_u32), carries ride 64-bit adds (v_add_co/v_addc); there is no
+// MFMA here because a Montgomery product is not a dense contraction. Constants are pinned by
+// tests/test_gpu_field.py against Python big integers and the CPU oracle.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define HD __host__ __device__ __forceinline__
+
+namespace g16 {
+
+struct FrCfg {
+  // r = 0x30644e72e131a029b85045b68181585d2833e84879b9709143e1f593f0000001
+  static HD uint32_t p(int i) {
+    constexpr uint32_t P[8] = {0xf0000001u, 0x43e1f593u, 0x79b97091u, 0x2833e848u,
+                               0x8181585du, 0xb85045b6u, 0xe131a029u, 0x30644e72u};
+    return P[i];
+  }
+  static HD uint32_t one(int i) {  // R mod r
+    constexpr uint32_t R1[8] = {0x4ffffffbu, 0xac96341cu, 0x9f60cd29u, 0x36fc7695u,
+                                0x7879462eu, 0x666ea36fu, 0x9a07df2fu, 0x0e0a77c1u};
+    return R1[i];
+  }
+  static HD uint32_t r2(int i) {  // R^2 mod r
+    constexpr uint32_t R2[8] = {0xae216da7u, 0x1bb8e645u, 0xe35c59e3u, 0x53fe3ab1u,
+                                0x53bb8085u, 0x8c49833du, 0x7f4e44a5u, 0x0216d0b1u};
+    return R2[i];
+  }
+  static constexpr uint32_t N0 = 0xefffffffu;  // -r^-1 mod 2^32
+};
+
+struct FqCfg {
+  // q = 0x30644e72e131a029b85045b68181585d97816a916871ca8d3c208c16d87cfd47
+  static HD uint32_t p(int i) {
+    constexpr uint32_t P[8] = {0xd87cfd47u, 0x3c208c16u, 0x6871ca8du, 0x97816a91u,
+                               0x8181585du, 0xb85045b6u, 0xe131a029u, 0x30644e72u};
+    return P[i];
+  }
+  static HD uint32_t one(int i) {
+    constexpr uint32_t R1[8] = {0xc58f0d9du, 0xd35d438du, 0xf5c70b3du, 0x0a78eb28u,
+                                0x7879462cu, 0x666ea36fu, 0x9a07df2fu, 0x0e0a77c1u};
+    return R1[i];
+  }
+  static HD uint32_t r2(int i) {
+    constexpr uint32_t R2[8] = {0x538afa89u, 0xf32cfc5bu, 0xd44501fbu, 0xb5e71911u,
+                                0x0a417ff6u, 0x47ab1effu, 0xcab8351fu, 0x06d89f71u};
+    return R2[i];
+  }
+  static constexpr uint32_t N0 = 0xe4866389u;
+};
+
+template <class C>
+struct Fp {
+  uint32_t v[8];
+};
+
+typedef Fp<FrCfg> Fr;
+typedef Fp<FqCfg> Fq;
+
+template <class C>
+HD Fp<C> fp_zero() {
+  Fp<C> r;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) r.v[i] = 0;
+  return r;
+}
+
+template <class C>
+HD Fp<C> fp_one() {
+  Fp<C> r;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) r.v[i] = C::one(i);
+  return r;
+}
+
+template <class C>
+HD bool is_zero(const Fp<C>& a) {
+  uint32_t x = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) x |= a.v[i];
+  return x == 0;
+}
+
+template <class C>
+HD bool eq(const Fp<C>& a, const Fp<C>& b) {
+  uint32_t x = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) x |= a.v[i] ^ b.v[i];
+  return x == 0;
+}
+
+template <class C>
+HD bool is_one(const Fp<C>& a) {
+  uint32_t x = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) x |= a.v[i] ^ C::one(i);
+  return x == 0;
+}
+
+// a - p if a >= p, computed as a subtraction with borrow and a select.
+template <class C>
+HD void reduce_once(uint32_t a[8]) {
+  uint32_t d[8];
+  uint32_t borrow = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    uint64_t t = (uint64_t)a[i] - C::p(i) - borrow;
+    d[i] = (uint32_t)t;
+    borrow = (uint32_t)(t >> 63);
+  }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) a[i] = borrow ? a[i] : d[i];
+}
+
+template <class C>
+HD Fp<C> add(const Fp<C>& a, const Fp<C>& b) {
+  Fp<C> r;
+  uint32_t carry = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    uint64_t t = (uint64_t)a.v[i] + b.v[i] + carry;
+    r.v[i] = (uint32_t)t;
+    carry = (uint32_t)(t >> 32);
+  }
+  reduce_once<C>(r.v);  // p < 2^254: the sum never carries out of 256 bits
+  return r;
+}
+
+template <class C>
+HD Fp<C> sub(const Fp<C>& a, const Fp<C>& b) {
+  Fp<C> r;
+  uint32_t borrow = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    uint64_t t = (uint64_t)a.v[i] - b.v[i] - borrow;
+    r.v[i] = (uint32_t)t;
+    borrow = (uint32_t)(t >> 63);
+  }
+  // Add p back when the difference went negative: mask the modulus with the borrow.
+  uint32_t mask = 0u - borrow;
+  uint32_t carry = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    uint64_t t = (uint64_t)r.v[i] + (C::p(i) & mask) + carry;
+    r.v[i] = (uint32_t)t;
+    carry = (uint32_t)(t >> 32);
+  }
+  return r;
+}
+
+template <class C>
+HD Fp<C> neg(const Fp<C>& a) {
+  return is_zero(a) ? a : sub(fp_zero<C>(), a);
+}
+
+template <class C>
+HD Fp<C> dbl(const Fp<C>& a) {
+  return add(a, a);
+}
+
+// Montgomery product a*b*2^-256 mod p, CIOS (Koc, Acar, Kaliski 1996) over 32-bit limbs.
+template <class C>
+HD Fp<C> mul(const Fp<C>& a, const Fp<C>& b) {
+  uint32_t t[10];
+#pragma unroll
+  for (int i = 0; i < 10; ++i) t[i] = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    uint64_t c = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      uint64_t x = (uint64_t)a.v[j] * b.v[i] + t[j] + c;
+      t[j] = (uint32_t)x;
+      c = x >> 32;
+    }
+    uint64_t x = (uint64_t)t[8] + c;
+    t[8] = (uint32_t)x;
+    t[9] = (uint32_t)(x >> 32);
+    uint32_t m = t[0] * C::N0;
+    x = (uint64_t)m * C::p(0) + t[0];
+    c = x >> 32;
+#pragma unroll
+    for (int j = 1; j < 8; ++j) {
+      x = (uint64_t)m * C::p(j) + t[j] + c;
+      t[j - 1] = (uint32_t)x;
+      c = x >> 32;
+    }
+    x = (uint64_t)t[8] + c;
+    t[7] = (uint32_t)x;
+    t[8] = t[9] + (uint32_t)(x >> 32);
+  }
+  Fp<C> r;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) r.v[i] = t[i];
+  reduce_once<C>(r.v);  // the CIOS result is below 2p < 2^256
+  return r;
+}
+
+template <class C>
+HD Fp<C> sqr(const Fp<C>& a) {
+  return mul(a, a);
+}
+
+template <class C>
+HD Fp<C> from_std(const Fp<C>& plain) {
+  Fp<C> r2;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) r2.v[i] = C::r2(i);
+  return mul(plain, r2);
+}
+
+template <class C>
+HD Fp<C> to_std(const Fp<C>& a) {
+  Fp<C> one_plain = fp_zero<C>();
+  one_plain.v[0] = 1;
+  return mul(a, one_plain);
+}
+
+// a^(p-2) (Fermat), square-and-multiply over the exponent's bits; zero maps to zero.
+template <class C>
+HD Fp<C> inv(const Fp<C>& a) {
+  Fp<C> acc = fp_one<C>();
+  for (int i = 255; i >= 0; --i) {
+    acc = sqr(acc);
+    uint32_t e = C::p(i >> 5) - ((i >> 5) == 0 ? 2u : 0u);
+    if ((e >> (i & 31)) & 1) acc = mul(acc, a);
+  }
+  return acc;
+}
+
+// ---------------------------------------------------------------------------------------
+// Fq2 = Fq[u]/(u^2 + 1)
+// ---------------------------------------------------------------------------------------
+struct Fq2 {
+  Fq c0, c1;
+};
+
+HD Fq2 fq2_zero() { return {fp_zero<FqCfg>(), fp_zero<FqCfg>()}; }
+HD Fq2 fq2_one() { return {fp_one<FqCfg>(), fp_zero<FqCfg>()}; }
+HD bool is_zero(const Fq2& a) { return is_zero(a.c0) && is_zero(a.c1); }
+HD bool eq(const Fq2& a, const Fq2& b) { return eq(a.c0, b.c0) && eq(a.c1, b.c1); }
+HD Fq2 add(const Fq2& a, const Fq2& b) { return {add(a.c0, b.c0), add(a.c1, b.c1)}; }
+HD Fq2 sub(const Fq2& a, const Fq2& b) { return {sub(a.c0, b.c0), sub(a.c1, b.c1)}; }
+HD Fq2 neg(const Fq2& a) { return {neg(a.c0), neg(a.c1)}; }
+HD Fq2 dbl(const Fq2& a) { return add(a, a); }
+
+// Karatsuba: 3 Fq products.
+HD Fq2 mul(const Fq2& a, const Fq2& b) {
+  Fq v0 = mul(a.c0, b.c0);
+  Fq v1 = mul(a.c1, b.c1);
+  Fq s = mul(add(a.c0, a.c1), add(b.c0, b.c1));
+  return {sub(v0, v1), sub(sub(s, v0), v1)};
+}
+
+// (a0 + a1 u)^2 = (a0 + a1)(a0 - a1) + 2 a0 a1 u: 2 Fq products.
+HD Fq2 sqr(const Fq2& a) {
+  Fq m = mul(add(a.c0, a.c1), sub(a.c0, a.c1));
+  Fq t = mul(a.c0, a.c1);
+  return {m, add(t, t)};
+}
+
+HD Fq2 inv(const Fq2& a) {
+  Fq t = inv(add(sqr(a.c0), sqr(a.c1)));
+  return {mul(a.c0, t), neg(mul(a.c1, t))};
+}
+
+// ---------------------------------------------------------------------------------------
+// Points
+// ---------------------------------------------------------------------------------------
+template <class F>
+struct Aff {
+  F x, y;
+};
+
+template <class F>
+struct Xyzz {
+  F X, Y, ZZ, ZZZ;
+};
+
+typedef Aff<Fq> G1Aff;
+typedef Aff<Fq2> G2Aff;
+typedef Xyzz<Fq> G1X;
+typedef Xyzz<Fq2> G2X;
+
+template <class F> HD F f_zero();
+template <> HD Fq f_zero<Fq>() { return fp_zero<FqCfg>(); }
+template <> HD Fq2 f_zero<Fq2>() { return fq2_zero(); }
+template <class F> HD F f_one();
+template <> HD Fq f_one<Fq>() { return fp_one<FqCfg>(); }
+template <> HD Fq2 f_one<Fq2>() { return fq2_one(); }
+
+template <class F>
+HD bool aff_is_inf(const Aff<F>& p) {
+  return is_zero(p.x) && is_zero(p.y);
+}
+
+template <class F>
+HD Xyzz<F> xyzz_inf() {
+  return {f_zero<F>(), f_zero<F>(), f_zero<F>(), f_zero<F>()};
+}
+
+template <class F>
+HD bool xyzz_is_inf(const Xyzz<F>& p) {
+  return is_zero(p.ZZ);
+}
+
+template <class F>
+HD Xyzz<F> xyzz_from_aff(const Aff<F>& a) {
+  if (aff_is_inf(a)) return xyzz_inf<F>();
+  return {a.x, a.y, f_one<F>(), f_one<F>()};
+}
+
+// dbl-2008-s-1 (a = 0).
+template <class F>
+HD Xyzz<F> xyzz_dbl(const Xyzz<F>& p) {
+  if (xyzz_is_inf(p)) return p;
+  F U = dbl(p.Y);
+  F V = sqr(U);
+  F W = mul(U, V);
+  F S = mul(p.X, V);
+  F X2 = sqr(p.X);
+  F M = add(dbl(X2), X2);
+  Xyzz<F> r;
+  r.X = sub(sqr(M), dbl(S));
+  r.Y = sub(mul(M, sub(S, r.X)), mul(W, p.Y));
+  r.ZZ = mul(V, p.ZZ);
+  r.ZZZ = mul(W, p.ZZZ);
+  return r;
+}
+
+// mdbl-2008-s-1: doubling of an affine point.
+template <class F>
+HD Xyzz<F> xyzz_dbl_aff(const Aff<F>& a) {
+  F U = dbl(a.y);
+  F V = sqr(U);
+  F W = mul(U, V);
+  F S = mul(a.x, V);
+  F X2 = sqr(a.x);
+  F M = add(dbl(X2), X2);
+  Xyzz<F> r;
+  r.X = sub(sqr(M), dbl(S));
+  r.Y = sub(mul(M, sub(S, r.X)), mul(W, a.y));
+  r.ZZ = V;
+  r.ZZZ = W;
+  return r;
+}
+
+// madd-2008-s: p + affine a, with the doubling and cancellation cases resolved.
+template <class F>
+HD Xyzz<F> xyzz_add_aff(const Xyzz<F>& p, const Aff<F>& a) {
+  if (aff_is_inf(a)) return p;
+  if (xyzz_is_inf(p)) return {a.x, a.y, f_one<F>(), f_one<F>()};
+  F U2 = mul(a.x, p.ZZ);
+  F S2 = mul(a.y, p.ZZZ);
+  F P = sub(U2, p.X);
+  F R = sub(S2, p.Y);
+  if (is_zero(P)) {
+    if (is_zero(R)) return xyzz_dbl_aff(a);
+    return xyzz_inf<F>();
+  }
+  F PP = sqr(P);
+  F PPP = mul(P, PP);
+  F Q = mul(p.X, PP);
+  Xyzz<F> r;
+  r.X = sub(sub(sqr(R), PPP), dbl(Q));
+  r.Y = sub(mul(R, sub(Q, r.X)), mul(p.Y, PPP));
+  r.ZZ = mul(p.ZZ, PP);
+  r.ZZZ = mul(p.ZZZ, PPP);
+  return r;
+}
+
+// add-2008-s: p + q, general.
+template <class F>
+HD Xyzz<F> xyzz_add(const Xyzz<F>& p, const Xyzz<F>& q) {
+  if (xyzz_is_inf(p)) return q;
+  if (xyzz_is_inf(q)) return p;
+  F U1 = mul(p.X, q.ZZ);
+  F U2 = mul(q.X, p.ZZ);
+  F S1 = mul(p.Y, q.ZZZ);
+  F S2 = mul(q.Y, p.ZZZ);
+  F P = sub(U2, U1);
+  F R = sub(S2, S1);
+  if (is_zero(P)) {
+    if (is_zero(R)) return xyzz_dbl(p);
+    return xyzz_inf<F>();
+  }
+  F PP = sqr(P);
+  F PPP = mul(P, PP);
+  F Q = mul(U1, PP);
+  Xyzz<F> r;
+  r.X = sub(sub(sqr(R), PPP), dbl(Q));
+  r.Y = sub(mul(R, sub(Q, r.X)), mul(S1, PPP));
+  r.ZZ = mul(mul(p.ZZ, q.ZZ), PP);
+  r.ZZZ = mul(mul(p.ZZZ, q.ZZZ), PPP);
+  return r;
+}
+
+template <class F>
+HD Xyzz<F> xyzz_neg(const Xyzz<F>& p) {
+  Xyzz<F> r = p;
+  r.Y = neg(p.Y);
+  return r;
+}
+
+template <class F>
+HD Aff<F> aff_neg(const Aff<F>& a) {
+  if (aff_is_inf(a)) return a;
+  return {a.x, neg(a.y)};
+}
+
+template <class F>
+HD Aff<F> xyzz_to_aff(const Xyzz<F>& p) {
+  if (xyzz_is_inf(p)) return {f_zero<F>(), f_zero<F>()};
+  // x = X / ZZ, y = Y / ZZZ: one inversion of ZZZ gives both (ZZ^-1 = ZZZ^-1 * ZZZ / ZZ, and
+  // ZZZ / ZZ = ZZZ^2 / (ZZ * ZZZ) ... kept simple: two inversions through one).
+  F izzz = inv(p.ZZZ);
+  F t = mul(izzz, p.ZZ);       // ZZ / ZZZ = 1 / Z
+  F iz2 = sqr(t);              // 1 / ZZ
+  return {mul(p.X, iz2), mul(p.Y, izzz)};
+}
+
+// k * a for a plain (standard-form) 256-bit scalar, double-and-add from the top bit.
+template <class F>
+HD Xyzz<F> xyzz_mul_plain(const Aff<F>& a, const uint32_t k[8]) {
+  Xyzz<F> acc = xyzz_inf<F>();
+  for (int i = 255; i >= 0; --i) {
+    acc = xyzz_dbl(acc);
+    if ((k[i >> 5] >> (i & 31)) & 1) acc = xyzz_add_aff(acc, a);
+  }
+  return acc;
+}
+
+// Scalar multiplication by a small non-negative integer.
+template <class F>
+HD Xyzz<F> xyzz_mul_small(const Xyzz<F>& p, uint32_t k) {
+  Xyzz<F> acc = xyzz_inf<F>();
+  for (int i = 31; i >= 0; --i) {
+    acc = xyzz_dbl(acc);
+    if ((k >> i) & 1) acc = xyzz_add(acc, p);
+  }
+  return acc;
+}
+
+// Standard generators.
+HD G1Aff g1_generator() {
+  Fq x = fp_zero<FqCfg>(), y = fp_zero<FqCfg>();
+  x.v[0] = 1;
+  y.v[0] = 2;
+  return {from_std(x), from_std(y)};
+}
+
+HD G2Aff g2_generator() {
+  Fq x0, x1, y0, y1;
+  const uint32_t X0[8] = {0xd992f6edu, 0x46debd5cu, 0xf75edaddu, 0x674322d4u,
+                          0x5e5c4479u, 0x426a0066u, 0x121f1e76u, 0x1800deefu};
+  const uint32_t X1[8] = {0xaef312c2u, 0x97e485b7u, 0x35a9e712u, 0xf1aa4933u,
+                          0x31fb5d25u, 0x7260bfb7u, 0x920d483au, 0x198e9393u};
+  const uint32_t Y0[8] = {0x66fa7daau, 0x4ce6cc01u, 0x0c43d37bu, 0xe3d1e769u,
+                          0x8dcb408fu, 0x4aab7180u, 0xdb8c6debu, 0x12c85ea5u};
+  const uint32_t Y1[8] = {0xd122975bu, 0x55acdadcu, 0x70b38ef3u, 0xbc4b3133u,
+                          0x690c3395u, 0xec9e99adu, 0x585ff075u, 0x090689d0u};
+  for (int i = 0; i < 8; ++i) {
+    x0.v[i] = X0[i];
+    x1.v[i] = X1[i];
+    y0.v[i] = Y0[i];
+    y1.v[i] = Y1[i];
+  }
+  return {{from_std(x0), from_std(x1)}, {from_std(y0), from_std(y1)}};
+}
+
+}  // namespace g16

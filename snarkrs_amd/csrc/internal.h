@@ -1,0 +1,80 @@
+// Internal declarations shared by the kernel translation units and the runtime.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace g16 {
+
+// Throws on a HIP error; the C ABI layer turns it into G16HIP_DEVICE + message.
+struct HipError {
+  std::string msg;
+};
+struct ArgError {
+  std::string msg;
+};
+
+void hip_check(hipError_t e, const char* what);
+#define G16_CHECK(call) ::g16::hip_check((call), #call)
+#define G16_LAUNCH_CHECK(what) ::g16::hip_check(hipGetLastError(), what)
+
+// ---- field / setup kernels (kernels_field.hip) ----
+void launch_field_probe(int field, int op, const void* a, const void* b, void* out, size_t n,
+                        hipStream_t s);
+void launch_fixed_base(int group, const void* k, size_t n, void* out, hipStream_t s);
+
+// ---- NTT kernels (kernels_ntt.hip) ----
+// Twiddle tables: fwd[i] = w^i, inv[i] = w^-i for i < n/2 (Montgomery Fr).
+void launch_powers(void* out, const void* base, size_t n, hipStream_t s);
+void launch_coset_table(void* out, const void* shift, const void* scale, int log_n,
+                        hipStream_t s);
+// Transforms of `nvec` vectors stored back to back (stride n elements), in place.
+// dif_inverse: natural order in -> bit-reversed out, unscaled (w^-1 twiddles).
+// dit_forward: bit-reversed in -> natural out; if `prescale` is given, element p is first
+// multiplied by prescale[p] (the coset table in bit-reversed order).
+void ntt_dif_inverse(void* data, int nvec, int log_n, const void* tw_inv, hipStream_t s);
+void ntt_dit_forward(void* data, int nvec, int log_n, const void* tw_fwd, const void* prescale,
+                     hipStream_t s);
+void launch_bitrev_permute(void* dst, const void* src, int log_n, hipStream_t s);
+void launch_scale(void* data, size_t n, const void* factor, hipStream_t s);
+
+// ---- prove stages (kernels_prove.hip) ----
+void launch_gather(const uint32_t* row_ptr, const uint32_t* signal, const uint32_t* value_id,
+                   const void* values, const void* witness, void* out, uint32_t n_rows,
+                   hipStream_t s);
+void launch_mul_into(void* out, const void* a, const void* b, size_t n, hipStream_t s);
+void launch_h(void* h, const void* a, const void* b, const void* c, size_t n, hipStream_t s);
+
+// ---- MSM (kernels_msm.hip) ----
+struct MsmPlan {
+  int c = 0;              // window width
+  int windows = 0;        // ceil(255 / c)
+  uint32_t n = 0;         // scalars
+  uint32_t nb = 0;        // buckets per window, 2^(c-1)
+  uint32_t seg = 0;       // entries per accumulation segment
+  size_t max_entries = 0; // n * windows
+  // device scratch (owned by the caller)
+  uint32_t* count = nullptr;    // [windows*nb + 1]
+  uint32_t* offset = nullptr;   // [windows*nb + 1]
+  uint32_t* cursor = nullptr;   // [windows*nb]
+  uint32_t* block_sums = nullptr;
+  uint32_t* keys = nullptr;     // [max_entries]
+  uint32_t* vals = nullptr;     // [max_entries]  (index << 1 | sign)
+  uint32_t* scalars_std = nullptr;  // [n * 8]
+  uint32_t* total = nullptr;    // [4] entry count, heavy count
+  uint32_t* heavy = nullptr;    // [windows*nb] heavy bucket list
+};
+size_t msm_sort_scratch_bytes(uint32_t n, int c);
+void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c);
+// Digits of `n` Montgomery scalars into (window, bucket)-sorted entries.
+void msm_sort(MsmPlan& p, const void* scalars_mont, hipStream_t s);
+size_t msm_accum_scratch_bytes(const MsmPlan& p, int group);
+// Bucket accumulation + window reduction for one base vector over the sorted entries;
+// writes `windows` XYZZ window sums to `window_out` (device memory).
+void msm_accumulate(const MsmPlan& p, int group, const void* bases, void* scratch,
+                    void* window_out, hipStream_t s);
+int msm_default_c(uint32_t n);
+
+}  // namespace g16

@@ -1,0 +1,349 @@
+"""ctypes binding of libg16hip.so (include/g16hip.h).
+
+This is the only way the package reaches the GPU. There is no fallback: if the library is
+missing or cannot be loaded, every entry point raises :class:`NativeUnavailable`.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+from .key import KeyStruct, ProvingKey
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libg16hip.so")
+
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_vp = ctypes.c_void_p
+
+# g16hip_status
+OK, WITNESS_LENGTH, BACKEND, DEVICE, DEVICE_MEMORY, CONSTANT_WIRE, ZERO_BLINDER = range(7)
+
+
+class NativeUnavailable(RuntimeError):
+    """libg16hip.so is not built or cannot be loaded; there is no CPU fallback."""
+
+
+class ProveError(RuntimeError):
+    """``ProveError`` (crates/groth16/src/lib.rs:79-154); ``status`` is the g16hip_status."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(message)
+        self.status = status
+
+    def is_device_fault(self) -> bool:
+        return self.status == DEVICE
+
+
+class WitnessLength(ProveError):
+    pass
+
+
+class BackendError(ProveError):
+    pass
+
+
+class DeviceFault(ProveError):
+    pass
+
+
+class DeviceMemory(ProveError):
+    pass
+
+
+class ConstantWire(ProveError):
+    pass
+
+
+class ZeroBlinder(ProveError):
+    pass
+
+
+_ERRORS = {WITNESS_LENGTH: WitnessLength, BACKEND: BackendError, DEVICE: DeviceFault,
+           DEVICE_MEMORY: DeviceMemory, CONSTANT_WIRE: ConstantWire, ZERO_BLINDER: ZeroBlinder}
+
+
+class Timings(ctypes.Structure):
+    """``g16hip_timings`` = StageTimings (crates/groth16/src/lib.rs:322-368), microseconds."""
+
+    _fields_ = [(n, ctypes.c_uint64) for n in
+                ("gather_us", "ntt_us", "pointwise_us", "msm_us", "assemble_us", "verify_us")]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class MsmOut(ctypes.Structure):
+    _fields_ = [("a_g1", ctypes.c_uint8 * 64), ("b_g2", ctypes.c_uint8 * 128),
+                ("b_g1", ctypes.c_uint8 * 64), ("l_g1", ctypes.c_uint8 * 64),
+                ("h_g1", ctypes.c_uint8 * 64)]
+
+    def as_array(self) -> np.ndarray:
+        return np.frombuffer(bytes(self), dtype=np.uint8).copy()
+
+
+class ProofOut(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_uint8 * 64), ("b", ctypes.c_uint8 * 128), ("c", ctypes.c_uint8 * 64)]
+
+    def as_array(self) -> np.ndarray:
+        return np.frombuffer(bytes(self), dtype=np.uint8).copy()
+
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def load() -> ctypes.CDLL:
+    """Loads libg16hip.so once; raises NativeUnavailable if it is not there."""
+    global _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise NativeUnavailable(
+                f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C snarkrs_amd/csrc)")
+        try:
+            lib = ctypes.CDLL(LIB_PATH)
+        except OSError as e:
+            raise NativeUnavailable(f"cannot load {LIB_PATH}: {e}") from e
+        _declare(lib)
+        _lib = lib
+        return lib
+
+
+EXPORTS = {
+    # name: (restype, argtypes)
+    "g16hip_last_error": (ctypes.c_char_p, []),
+    "g16hip_open": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+    "g16hip_close": (None, [_vp]),
+    "g16hip_mem_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t),
+                                       ctypes.POINTER(ctypes.c_size_t)]),
+    "g16hip_prepare": (ctypes.c_int, [_vp, ctypes.POINTER(KeyStruct), ctypes.POINTER(_vp)]),
+    "g16hip_circuit_release": (None, [_vp]),
+    "g16hip_circuit_dims": (ctypes.c_int, [_vp] + [ctypes.POINTER(ctypes.c_uint32)] * 3),
+    "g16hip_circuit_resident_bytes": (ctypes.c_size_t, [_vp]),
+    "g16hip_compute_h": (ctypes.c_int, [_vp, _u8p, ctypes.c_size_t, ctypes.POINTER(_vp),
+                                        ctypes.POINTER(Timings)]),
+    "g16hip_h_to_host": (ctypes.c_int, [_vp, _vp, _u8p]),
+    "g16hip_h_release": (None, [_vp]),
+    "g16hip_msms": (ctypes.c_int, [_vp, _u8p, ctypes.c_size_t, _vp, ctypes.POINTER(MsmOut),
+                                   ctypes.POINTER(Timings)]),
+    "g16hip_h_and_msms": (ctypes.c_int, [_vp, _u8p, ctypes.c_size_t, ctypes.POINTER(MsmOut),
+                                         ctypes.POINTER(Timings)]),
+    "g16hip_h_and_msms_device": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t,
+                                                ctypes.POINTER(MsmOut), ctypes.POINTER(Timings)]),
+    "g16hip_prove_with_blinders": (ctypes.c_int, [_vp, _u8p, ctypes.c_size_t, _u8p, _u8p,
+                                                  ctypes.POINTER(ProofOut), ctypes.POINTER(Timings)]),
+    "g16hip_prove_with_blinders_device": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _u8p, _u8p,
+                                                         ctypes.POINTER(ProofOut),
+                                                         ctypes.POINTER(Timings)]),
+    "g16hip_msm_g1": (ctypes.c_int, [_vp, _u8p, _u8p, ctypes.c_size_t, _u8p]),
+    "g16hip_msm_g2": (ctypes.c_int, [_vp, _u8p, _u8p, ctypes.c_size_t, _u8p]),
+    "g16hip_msm_g1_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t, ctypes.c_int, _u8p]),
+    "g16hip_msm_g2_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t, ctypes.c_int, _u8p]),
+    "g16hip_ntt": (ctypes.c_int, [_vp, _u8p, ctypes.c_int, ctypes.c_int]),
+    "g16hip_ntt_device": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int]),
+    "g16hip_field_probe": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _u8p, _u8p, _u8p,
+                                          ctypes.c_size_t]),
+    "g16hip_fixed_base": (ctypes.c_int, [_vp, ctypes.c_int, _u8p, ctypes.c_size_t, _u8p]),
+}
+
+
+def _declare(lib) -> None:
+    for name, (res, args) in EXPORTS.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue  # reported by exported_symbols()
+        fn.restype = res
+        fn.argtypes = args
+
+
+def exported_symbols() -> dict:
+    """Which of the declared entry points the loaded library exports."""
+    lib = load()
+    return {name: hasattr(lib, name) for name in EXPORTS}
+
+
+def _check(rc: int) -> None:
+    if rc != OK:
+        msg = load().g16hip_last_error().decode(errors="replace")
+        raise _ERRORS.get(rc, ProveError)(rc, msg)
+
+
+def _p8(a: np.ndarray):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "arrays crossing the ABI must be C-contiguous"
+    return a.ctypes.data_as(_u8p)
+
+
+def _c8(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint8)
+
+
+class Device:
+    """An open GPU (``g16hip_device``)."""
+
+    def __init__(self, ordinal: int = 0):
+        self.lib = load()
+        h = _vp()
+        _check(self.lib.g16hip_open(ordinal, ctypes.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.g16hip_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def mem_info(self) -> tuple[int, int]:
+        f, t = ctypes.c_size_t(), ctypes.c_size_t()
+        _check(self.lib.g16hip_mem_info(self.h, ctypes.byref(f), ctypes.byref(t)))
+        return f.value, t.value
+
+    def field_probe(self, field: int, op: int, a: np.ndarray, b: np.ndarray | None = None):
+        w = 64 if field == 2 else 32
+        a = _c8(a).reshape(-1, w)
+        b = _c8(b).reshape(-1, w) if b is not None else None
+        out = np.zeros_like(a)
+        _check(self.lib.g16hip_field_probe(self.h, field, op, _p8(a), _p8(b), _p8(out), len(a)))
+        return out
+
+    def fixed_base(self, group: int, k: np.ndarray) -> np.ndarray:
+        k = _c8(k).reshape(-1, 32)
+        out = np.zeros((len(k), 64 if group == 1 else 128), dtype=np.uint8)
+        _check(self.lib.g16hip_fixed_base(self.h, group, _p8(k), len(k), _p8(out)))
+        return out
+
+    def msm(self, group: int, bases: np.ndarray, scalars: np.ndarray) -> np.ndarray:
+        """MsmBackend::try_msm_g1/g2 (crates/msm/src/lib.rs:61-81)."""
+        w = 64 if group == 1 else 128
+        bases = _c8(bases).reshape(-1, w)
+        scalars = _c8(scalars).reshape(-1, 32)
+        if len(bases) != len(scalars):
+            raise BackendError(BACKEND, f"{len(bases)} bases but {len(scalars)} scalars")
+        out = np.zeros(w, dtype=np.uint8)
+        fn = self.lib.g16hip_msm_g1 if group == 1 else self.lib.g16hip_msm_g2
+        _check(fn(self.h, _p8(bases), _p8(scalars), len(bases), _p8(out)))
+        return out
+
+    def msm_device(self, group: int, d_bases: int, d_scalars: int, n: int, c: int = 0):
+        w = 64 if group == 1 else 128
+        out = np.zeros(w, dtype=np.uint8)
+        fn = self.lib.g16hip_msm_g1_device if group == 1 else self.lib.g16hip_msm_g2_device
+        _check(fn(self.h, _vp(d_bases), _vp(d_scalars), n, c, _p8(out)))
+        return out
+
+    def ntt(self, data: np.ndarray, kind: int = 0) -> np.ndarray:
+        a = _c8(data).reshape(-1, 32).copy()
+        log_n = len(a).bit_length() - 1
+        if len(a) != 1 << log_n:
+            raise BackendError(BACKEND, "NTT length must be a power of two")
+        _check(self.lib.g16hip_ntt(self.h, _p8(a), log_n, kind))
+        return a
+
+    def ntt_device(self, d_data: int, log_n: int, kind: int = 0) -> None:
+        _check(self.lib.g16hip_ntt_device(self.h, _vp(d_data), log_n, kind))
+
+    def prepare(self, key: ProvingKey) -> "Circuit":
+        return Circuit(self, key)
+
+
+class HandleH:
+    """``HPoly::Device``: H left on the GPU by compute_h, owned by its circuit."""
+
+    def __init__(self, circuit: "Circuit", h):
+        self.circuit = circuit
+        self.h = h
+
+    def release(self) -> None:
+        if self.h:
+            self.circuit.lib.g16hip_h_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.release()
+
+
+class Circuit:
+    """A prepared circuit (``g16hip_circuit``), the PreparedCircuit of the reference
+    (crates/groth16/src/lib.rs:382-455)."""
+
+    def __init__(self, dev: Device, key: ProvingKey):
+        self.dev = dev
+        self.lib = dev.lib
+        self.key = key
+        st = key.struct()
+        h = _vp()
+        _check(self.lib.g16hip_prepare(dev.h, ctypes.byref(st), ctypes.byref(h)))
+        self.h = h
+        nv, npub, dom = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        _check(self.lib.g16hip_circuit_dims(h, ctypes.byref(nv), ctypes.byref(npub),
+                                            ctypes.byref(dom)))
+        self.n_vars, self.n_public, self.domain_size = nv.value, npub.value, dom.value
+
+    def release(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.g16hip_circuit_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.release()
+
+    def resident_bytes(self) -> int:
+        return int(self.lib.g16hip_circuit_resident_bytes(self.h))
+
+    def compute_h(self, witness: np.ndarray, t: Timings | None = None) -> HandleH:
+        w = _c8(witness).reshape(-1, 32)
+        h = _vp()
+        _check(self.lib.g16hip_compute_h(self.h, _p8(w), len(w), ctypes.byref(h),
+                                         ctypes.byref(t or Timings())))
+        return HandleH(self, h)
+
+    def h_to_host(self, hh: HandleH) -> np.ndarray:
+        out = np.zeros((self.domain_size, 32), dtype=np.uint8)
+        _check(self.lib.g16hip_h_to_host(self.h, hh.h, _p8(out)))
+        return out
+
+    def msms(self, witness: np.ndarray, hh: HandleH, t: Timings | None = None) -> np.ndarray:
+        w = _c8(witness).reshape(-1, 32)
+        out = MsmOut()
+        _check(self.lib.g16hip_msms(self.h, _p8(w), len(w), hh.h, ctypes.byref(out),
+                                    ctypes.byref(t or Timings())))
+        return out.as_array()
+
+    def h_and_msms(self, witness: np.ndarray, t: Timings | None = None) -> np.ndarray:
+        w = _c8(witness).reshape(-1, 32)
+        out = MsmOut()
+        _check(self.lib.g16hip_h_and_msms(self.h, _p8(w), len(w), ctypes.byref(out),
+                                          ctypes.byref(t or Timings())))
+        return out.as_array()
+
+    def h_and_msms_device(self, d_witness: int, n: int, t: Timings | None = None) -> np.ndarray:
+        out = MsmOut()
+        _check(self.lib.g16hip_h_and_msms_device(self.h, _vp(d_witness), n, ctypes.byref(out),
+                                                 ctypes.byref(t or Timings())))
+        return out.as_array()
+
+    def prove_with_blinders(self, witness: np.ndarray, r: np.ndarray, s: np.ndarray,
+                            t: Timings | None = None) -> np.ndarray:
+        w = _c8(witness).reshape(-1, 32)
+        out = ProofOut()
+        _check(self.lib.g16hip_prove_with_blinders(self.h, _p8(w), len(w), _p8(_c8(r)),
+                                                   _p8(_c8(s)), ctypes.byref(out),
+                                                   ctypes.byref(t or Timings())))
+        return out.as_array()
+
+    def prove_with_blinders_device(self, d_witness: int, n: int, r: np.ndarray, s: np.ndarray,
+                                   t: Timings | None = None) -> np.ndarray:
+        out = ProofOut()
+        _check(self.lib.g16hip_prove_with_blinders_device(self.h, _vp(d_witness), n,
+                                                          _p8(_c8(r)), _p8(_c8(s)),
+                                                          ctypes.byref(out),
+                                                          ctypes.byref(t or Timings())))
+        return out.as_array()

@@ -19,3 +19,12 @@ def orc():
 
     build()
     return Oracle()
+
+
+@pytest.fixture(scope="session")
+def dev():
+    from snarkrs_amd.native import Device
+
+    d = Device(0)
+    yield d
+    d.close()
