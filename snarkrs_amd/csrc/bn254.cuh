@@ -439,6 +439,17 @@ HD Xyzz<F> xyzz_mul_plain(const Aff<F>& a, const uint32_t k[8]) {
   return acc;
 }
 
+// k * p for an XYZZ point and a plain 256-bit scalar.
+template <class F>
+HD Xyzz<F> xyzz_mul(const Xyzz<F>& p, const uint32_t k[8]) {
+  Xyzz<F> acc = xyzz_inf<F>();
+  for (int i = 255; i >= 0; --i) {
+    acc = xyzz_dbl(acc);
+    if ((k[i >> 5] >> (i & 31)) & 1) acc = xyzz_add(acc, p);
+  }
+  return acc;
+}
+
 // Scalar multiplication by a small non-negative integer.
 template <class F>
 HD Xyzz<F> xyzz_mul_small(const Xyzz<F>& p, uint32_t k) {

@@ -1,0 +1,77 @@
+// Stage 0 (CSR gather, crates/groth16/src/cpu.rs:140-174) and the pointwise stages
+// (C = A o B, crates/groth16/src/cpu.rs:331-337; H = A*B - C, :230-245).
+#include "bn254.cuh"
+#include "internal.h"
+
+namespace g16 {
+
+__device__ __forceinline__ Fr ldfr4(const uint4* p) {
+  uint4 a = p[0], b = p[1];
+  return {{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w}};
+}
+
+__device__ __forceinline__ void stfr4(uint4* p, const Fr& x) {
+  p[0] = make_uint4(x.v[0], x.v[1], x.v[2], x.v[3]);
+  p[1] = make_uint4(x.v[4], x.v[5], x.v[6], x.v[7]);
+}
+
+// One thread per row: out[c] = sum_k values[value_id[k]] * w[signal[k]], zero witness values
+// skipped and ones added without a multiply (both exact shortcuts, as the reference's).
+__global__ void __launch_bounds__(256) k_gather(const uint32_t* __restrict__ row_ptr,
+                                                const uint32_t* __restrict__ signal,
+                                                const uint32_t* __restrict__ value_id,
+                                                const uint4* __restrict__ values,
+                                                const uint4* __restrict__ w, uint4* out,
+                                                uint32_t n_rows) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_rows) return;
+  uint32_t lo = row_ptr[c], hi = row_ptr[c + 1];
+  Fr acc = fp_zero<FrCfg>();
+  for (uint32_t k = lo; k < hi; ++k) {
+    Fr x = ldfr4(w + 2 * (size_t)signal[k]);
+    if (is_zero(x)) continue;
+    Fr v = ldfr4(values + 2 * (size_t)value_id[k]);
+    acc = add(acc, is_one(x) ? v : mul(v, x));
+  }
+  stfr4(out + 2 * (size_t)c, acc);
+}
+
+void launch_gather(const uint32_t* row_ptr, const uint32_t* signal, const uint32_t* value_id,
+                   const void* values, const void* witness, void* out, uint32_t n_rows,
+                   hipStream_t s) {
+  hipLaunchKernelGGL(k_gather, dim3((n_rows + 255) / 256), dim3(256), 0, s, row_ptr, signal,
+                     value_id, static_cast<const uint4*>(values),
+                     static_cast<const uint4*>(witness), static_cast<uint4*>(out), n_rows);
+  G16_LAUNCH_CHECK("gather");
+}
+
+__global__ void k_mul_into(uint4* out, const uint4* a, const uint4* b, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  stfr4(out + 2 * i, mul(ldfr4(a + 2 * i), ldfr4(b + 2 * i)));
+}
+
+void launch_mul_into(void* out, const void* a, const void* b, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_mul_into, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                     static_cast<uint4*>(out), static_cast<const uint4*>(a),
+                     static_cast<const uint4*>(b), n);
+  G16_LAUNCH_CHECK("pointwise product");
+}
+
+// H = A*B - C on the odd coset; no division by Z (folded into the H bases at setup).
+__global__ void k_h(uint4* h, const uint4* a, const uint4* b, const uint4* c, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  stfr4(h + 2 * i, sub(mul(ldfr4(a + 2 * i), ldfr4(b + 2 * i)), ldfr4(c + 2 * i)));
+}
+
+void launch_h(void* h, const void* a, const void* b, const void* c, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_h, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                     static_cast<uint4*>(h), static_cast<const uint4*>(a),
+                     static_cast<const uint4*>(b), static_cast<const uint4*>(c), n);
+  G16_LAUNCH_CHECK("H = A*B - C");
+}
+
+}  // namespace g16

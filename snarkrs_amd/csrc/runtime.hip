@@ -4,9 +4,11 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
-#include <future>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/g16hip.h"
@@ -35,6 +37,11 @@ static g16hip_status fail(g16hip_status st, const std::string& msg) {
   return st;
 }
 
+struct StatusError {
+  g16hip_status st;
+  std::string msg;
+};
+
 template <class F>
 static g16hip_status guarded(F&& f) {
   try {
@@ -45,6 +52,8 @@ static g16hip_status guarded(F&& f) {
     return fail(oom ? G16HIP_DEVICE_MEMORY : G16HIP_DEVICE, e.msg);
   } catch (const ArgError& e) {
     return fail(G16HIP_BACKEND, e.msg);
+  } catch (const StatusError& e) {
+    return fail(e.st, e.msg);
   } catch (const std::bad_alloc&) {
     return fail(G16HIP_BACKEND, "host allocation failed");
   }
@@ -80,17 +89,427 @@ static void upload(DevBuf& b, const void* src, size_t n, hipStream_t s) {
   if (n) G16_CHECK(hipMemcpyAsync(b.p, src, n, hipMemcpyHostToDevice, s));
 }
 
+// ---------------------------------------------------------------------------------------
+// host-side helpers (the same bn254.cuh arithmetic, compiled for the CPU)
+// ---------------------------------------------------------------------------------------
+static Fr fr_load(const uint8_t* p) {
+  Fr x;
+  std::memcpy(x.v, p, 32);
+  return x;
+}
+static G1Aff g1_load(const uint8_t* p) {
+  G1Aff a;
+  std::memcpy(a.x.v, p, 32);
+  std::memcpy(a.y.v, p + 32, 32);
+  return a;
+}
+static G2Aff g2_load(const uint8_t* p) {
+  G2Aff a;
+  std::memcpy(a.x.c0.v, p, 32);
+  std::memcpy(a.x.c1.v, p + 32, 32);
+  std::memcpy(a.y.c0.v, p + 64, 32);
+  std::memcpy(a.y.c1.v, p + 96, 32);
+  return a;
+}
+static void g1_store(uint8_t* p, const G1Aff& a) {
+  std::memcpy(p, a.x.v, 32);
+  std::memcpy(p + 32, a.y.v, 32);
+}
+static void g2_store(uint8_t* p, const G2Aff& a) {
+  std::memcpy(p, a.x.c0.v, 32);
+  std::memcpy(p + 32, a.x.c1.v, 32);
+  std::memcpy(p + 64, a.y.c0.v, 32);
+  std::memcpy(p + 96, a.y.c1.v, 32);
+}
+
+static bool fq_canonical(const Fq& x) {
+  for (int i = 7; i >= 0; --i) {
+    if (x.v[i] != FqCfg::p(i)) return x.v[i] < FqCfg::p(i);
+  }
+  return false;
+}
+
+// Domain root of order 2^L: 5^((r-1)/2^28) squared 28 - L times (crates/field/src/lib.rs:48-82).
+static Fr domain_root(int L) {
+  Fr w;
+  const uint32_t W[8] = {0x725b19f0u, 0x9bd61b6eu, 0x41112ed4u, 0x402d111eu,
+                         0x8ef62abcu, 0x00e0a7ebu, 0xa58a7e85u, 0x2a3c09f0u};
+  std::memcpy(w.v, W, 32);
+  w = from_std(w);
+  for (int i = L; i < 28; ++i) w = sqr(w);
+  return w;
+}
+
+static Fr fr_from_u64(uint64_t x) {
+  Fr r = fp_zero<FrCfg>();
+  r.v[0] = (uint32_t)x;
+  r.v[1] = (uint32_t)(x >> 32);
+  return from_std(r);
+}
+
+// sum_w 2^(c*w) S_w on the host (the reference finishes the same way, crates/cuda/src/msm.rs:3976-4010).
+template <class F>
+static Xyzz<F> horner(const std::vector<Xyzz<F>>& S, int c) {
+  Xyzz<F> acc = xyzz_inf<F>();
+  for (int w = (int)S.size() - 1; w >= 0; --w) {
+    for (int i = 0; i < c && w != (int)S.size() - 1; ++i) acc = xyzz_dbl(acc);
+    acc = xyzz_add(acc, S[w]);
+  }
+  return acc;
+}
+
+struct Timer {
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  uint64_t us() const {
+    return (uint64_t)std::chrono::duration_cast<std::chrono::microseconds>(
+               std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+// ---------------------------------------------------------------------------------------
+// device
+// ---------------------------------------------------------------------------------------
+struct NttTables {
+  DevBuf fwd, inv, coset, scalars;  // scalars: [w, w^-1, shift, 1/n]
+};
+
 struct g16hip_device {
   int ordinal = 0;
   hipStream_t stream = nullptr;
   std::mutex mu;  // serialises the standalone entry points' scratch
-  DevBuf scratch_a, scratch_b, scratch_c, scratch_d;
+  DevBuf scratch_a, scratch_b, scratch_c, scratch_d, scratch_e;
+  std::map<int, std::unique_ptr<NttTables>> ntt;
 };
 
 static void set_device(const g16hip_device* d) { G16_CHECK(hipSetDevice(d->ordinal)); }
 
+static void build_ntt_tables(NttTables& t, int L, hipStream_t s) {
+  size_t n = (size_t)1 << L;
+  size_t half = n / 2 ? n / 2 : 1;
+  Fr host[6];
+  host[0] = domain_root(L);
+  host[1] = inv(host[0]);
+  host[2] = domain_root(L + 1);  // coset shift: primitive 2n-th root (snarkjs' odd coset)
+  host[3] = inv(fr_from_u64(n));
+  host[4] = fp_one<FrCfg>();
+  host[5] = fp_zero<FrCfg>();
+  upload(t.scalars, host, sizeof host, s);
+  t.fwd.alloc(32 * half);
+  t.inv.alloc(32 * half);
+  t.coset.alloc(32 * n);
+  Fr* sc = t.scalars.as<Fr>();
+  launch_powers(t.fwd.p, sc + 0, half, s);
+  launch_powers(t.inv.p, sc + 1, half, s);
+  launch_coset_table(t.coset.p, sc + 2, sc + 3, L, s);
+}
+
+static NttTables& ntt_tables(g16hip_device* d, int L) {
+  auto it = d->ntt.find(L);
+  if (it != d->ntt.end()) return *it->second;
+  auto t = std::make_unique<NttTables>();
+  build_ntt_tables(*t, L, d->stream);
+  NttTables& ref = *t;
+  d->ntt[L] = std::move(t);
+  return ref;
+}
+
+// One MSM on device-resident bases and scalars with scratch from `d`.
+template <class F>
+static Xyzz<F> run_msm(g16hip_device* d, int group, const void* bases, const void* scalars,
+                       uint32_t n, int c) {
+  if (c <= 0) c = msm_default_c(n);
+  if (c < 2 || c > 20) throw ArgError{"window width out of range"};
+  MsmPlan plan;
+  d->scratch_d.alloc(msm_sort_scratch_bytes(n, c));
+  msm_plan_bind(plan, d->scratch_d.p, n, c);
+  d->scratch_e.alloc(msm_accum_scratch_bytes(plan, group) + sizeof(Xyzz<F>) * plan.windows);
+  msm_sort(plan, scalars, d->stream);
+  char* wout = static_cast<char*>(d->scratch_e.p) + msm_accum_scratch_bytes(plan, group);
+  msm_accumulate(plan, group, bases, d->scratch_e.p, wout, d->stream);
+  std::vector<Xyzz<F>> S(plan.windows);
+  G16_CHECK(hipMemcpyAsync(S.data(), wout, sizeof(Xyzz<F>) * plan.windows,
+                           hipMemcpyDeviceToHost, d->stream));
+  G16_CHECK(hipStreamSynchronize(d->stream));
+  return horner(S, c);
+}
+
 // ---------------------------------------------------------------------------------------
-// C ABI: device, field probe, fixed base
+// prepared circuit
+// ---------------------------------------------------------------------------------------
+struct g16hip_circuit {
+  g16hip_device* dev = nullptr;
+  std::mutex mu;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[8];
+  uint32_t n_vars = 0, n_public = 0, domain = 0;
+  int log_n = 0;
+  G1Aff alpha_g1, beta_g1, delta_g1;
+  G2Aff beta_g2, delta_g2;
+  DevBuf row_ptr[2], signal[2], value_id[2], values;
+  DevBuf a_q, b1_q, b2_q, l_q, h_q;
+  NttTables tables;
+  DevBuf witness, abc, hbuf;
+  DevBuf sort_w, sort_h, accum, wout;
+  MsmPlan plan_w, plan_h;
+  size_t resident = 0;
+};
+
+struct g16hip_h {
+  g16hip_circuit* owner = nullptr;
+  DevBuf buf;
+};
+
+static void validate_key(const g16hip_key* k) {
+  if (!k) throw ArgError{"null key"};
+  uint32_t n = k->domain_size;
+  if (n == 0 || (n & (n - 1))) throw ArgError{"domain size " + std::to_string(n) + " is not a power of two"};
+  if (n > (1u << 27)) throw ArgError{"domain size exceeds 2^27"};
+  if ((uint64_t)k->n_public + 1 > k->n_vars)
+    throw ArgError{"n_public " + std::to_string(k->n_public) + " exceeds n_vars " + std::to_string(k->n_vars)};
+  for (int m = 0; m < 2; ++m) {
+    const uint32_t* rp = k->row_ptr[m];
+    if (!rp) throw ArgError{"missing row_ptr"};
+    if (rp[0] != 0) throw ArgError{"matrix " + std::to_string(m) + " row_ptr starts at " + std::to_string(rp[0])};
+    for (uint32_t c = 0; c < n; ++c)
+      if (rp[c + 1] < rp[c]) throw ArgError{"matrix " + std::to_string(m) + " row_ptr is not monotone"};
+    uint32_t nnz = rp[n];
+    for (uint32_t e = 0; e < nnz; ++e) {
+      if (k->signal[m][e] >= k->n_vars) throw ArgError{"signal index out of range"};
+      if (k->value_id[m][e] >= k->n_values) throw ArgError{"value index out of range"};
+    }
+  }
+  const uint8_t* pts[] = {k->alpha_g1, k->beta_g1, k->delta_g1};
+  for (auto p : pts) {
+    G1Aff a = g1_load(p);
+    if (!fq_canonical(a.x) || !fq_canonical(a.y)) throw ArgError{"non-canonical key point"};
+    if (aff_is_inf(a)) throw ArgError{"a key point is the point at infinity, which no honest setup produces"};
+  }
+  if (aff_is_inf(g2_load(k->beta_g2)) || aff_is_inf(g2_load(k->delta_g2)))
+    throw ArgError{"a key point is the point at infinity, which no honest setup produces"};
+}
+
+static void prepare_circuit(g16hip_circuit* C, const g16hip_key* k) {
+  hipStream_t s = C->stream;
+  uint32_t n = k->domain_size;
+  C->n_vars = k->n_vars;
+  C->n_public = k->n_public;
+  C->domain = n;
+  C->log_n = 0;
+  while ((1u << C->log_n) < n) ++C->log_n;
+  C->alpha_g1 = g1_load(k->alpha_g1);
+  C->beta_g1 = g1_load(k->beta_g1);
+  C->delta_g1 = g1_load(k->delta_g1);
+  C->beta_g2 = g2_load(k->beta_g2);
+  C->delta_g2 = g2_load(k->delta_g2);
+  for (int m = 0; m < 2; ++m) {
+    uint32_t nnz = k->row_ptr[m][n];
+    upload(C->row_ptr[m], k->row_ptr[m], 4 * (size_t)(n + 1), s);
+    upload(C->signal[m], k->signal[m], 4 * (size_t)nnz, s);
+    upload(C->value_id[m], k->value_id[m], 4 * (size_t)nnz, s);
+  }
+  upload(C->values, k->values, 32 * (size_t)k->n_values, s);
+  size_t nv = k->n_vars;
+  upload(C->a_q, k->a_query, 64 * nv, s);
+  upload(C->b1_q, k->b_g1_query, 64 * nv, s);
+  upload(C->b2_q, k->b_g2_query, 128 * nv, s);
+  // L covers the private wires only; it is stored behind n_public + 1 points at infinity so
+  // that it multiplies the full witness and shares the witness sort with A, B1 and B2.
+  size_t pub = (size_t)k->n_public + 1;
+  C->l_q.alloc(64 * nv);
+  G16_CHECK(hipMemsetAsync(C->l_q.p, 0, 64 * pub, s));
+  if (nv > pub)
+    G16_CHECK(hipMemcpyAsync(static_cast<char*>(C->l_q.p) + 64 * pub, k->l_query, 64 * (nv - pub),
+                             hipMemcpyHostToDevice, s));
+  upload(C->h_q, k->h_query, 64 * (size_t)n, s);
+  build_ntt_tables(C->tables, C->log_n, s);
+  // Per-proof scratch, sized once here so a proof allocates nothing.
+  C->witness.alloc(32 * nv);
+  C->abc.alloc(32 * 3 * (size_t)n);
+  C->hbuf.alloc(32 * (size_t)n);
+  int cw = msm_default_c((uint32_t)nv), ch = msm_default_c(n);
+  C->sort_w.alloc(msm_sort_scratch_bytes((uint32_t)nv, cw));
+  C->sort_h.alloc(msm_sort_scratch_bytes(n, ch));
+  msm_plan_bind(C->plan_w, C->sort_w.p, (uint32_t)nv, cw);
+  msm_plan_bind(C->plan_h, C->sort_h.p, n, ch);
+  size_t acc = std::max({msm_accum_scratch_bytes(C->plan_w, 1), msm_accum_scratch_bytes(C->plan_w, 2),
+                         msm_accum_scratch_bytes(C->plan_h, 1)});
+  C->accum.alloc(acc);
+  C->wout.alloc(256 * (size_t)(4 * C->plan_w.windows + C->plan_h.windows));
+  G16_CHECK(hipStreamSynchronize(s));
+  size_t tot = 0;
+  DevBuf* bufs[] = {&C->row_ptr[0], &C->row_ptr[1], &C->signal[0], &C->signal[1], &C->value_id[0],
+                    &C->value_id[1], &C->values, &C->a_q, &C->b1_q, &C->b2_q, &C->l_q, &C->h_q,
+                    &C->tables.fwd, &C->tables.inv, &C->tables.coset, &C->witness, &C->abc,
+                    &C->hbuf, &C->sort_w, &C->sort_h, &C->accum, &C->wout};
+  for (auto b : bufs) tot += b->bytes;
+  C->resident = tot;
+}
+
+static void check_witness_len(g16hip_circuit* C, size_t n) {
+  if (n != C->n_vars)
+    throw StatusError{G16HIP_WITNESS_LENGTH, "witness has " + std::to_string(n) +
+                                                 " entries, proving key expects " +
+                                                 std::to_string(C->n_vars)};
+}
+
+static float ev_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  G16_CHECK(hipEventElapsedTime(&ms, a, b));
+  return ms;
+}
+
+// Stages 0-4 from the circuit's device witness buffer into `h_out` (device).
+static void stages_0_4(g16hip_circuit* C, void* h_out) {
+  hipStream_t s = C->stream;
+  size_t n = C->domain;
+  char* abc = C->abc.as<char>();
+  void* A = abc;
+  void* B = abc + 32 * n;
+  void* Cv = abc + 64 * n;
+  G16_CHECK(hipEventRecord(C->ev[0], s));
+  launch_gather(C->row_ptr[0].as<uint32_t>(), C->signal[0].as<uint32_t>(),
+                C->value_id[0].as<uint32_t>(), C->values.p, C->witness.p, A, C->domain, s);
+  launch_gather(C->row_ptr[1].as<uint32_t>(), C->signal[1].as<uint32_t>(),
+                C->value_id[1].as<uint32_t>(), C->values.p, C->witness.p, B, C->domain, s);
+  launch_mul_into(Cv, A, B, n, s);
+  G16_CHECK(hipEventRecord(C->ev[1], s));
+  ntt_dif_inverse(abc, 3, C->log_n, C->tables.inv.p, s);
+  ntt_dit_forward(abc, 3, C->log_n, C->tables.fwd.p, C->tables.coset.p, s);
+  G16_CHECK(hipEventRecord(C->ev[2], s));
+  launch_h(h_out, A, B, Cv, n, s);
+  G16_CHECK(hipEventRecord(C->ev[3], s));
+}
+
+// Stages 5-9: five MSMs, witness from the circuit's device buffer, H from `h` (device).
+static void stages_5_9(g16hip_circuit* C, const void* h, g16hip_msm_out* out,
+                       uint64_t* msm_us) {
+  hipStream_t s = C->stream;
+  G16_CHECK(hipEventRecord(C->ev[4], s));
+  MsmPlan& pw = C->plan_w;
+  MsmPlan& ph = C->plan_h;
+  char* wout = C->wout.as<char>();
+  size_t Ww = pw.windows, Wh = ph.windows;
+  msm_sort(pw, C->witness.p, s);
+  msm_accumulate(pw, 1, C->a_q.p, C->accum.p, wout, s);
+  msm_accumulate(pw, 1, C->b1_q.p, C->accum.p, wout + 128 * Ww, s);
+  msm_accumulate(pw, 1, C->l_q.p, C->accum.p, wout + 256 * Ww, s);
+  msm_accumulate(pw, 2, C->b2_q.p, C->accum.p, wout + 384 * Ww, s);
+  msm_sort(ph, h, s);
+  msm_accumulate(ph, 1, C->h_q.p, C->accum.p, wout + 640 * Ww, s);
+  G16_CHECK(hipEventRecord(C->ev[5], s));
+  std::vector<uint8_t> host(640 * Ww + 128 * Wh);
+  G16_CHECK(hipMemcpyAsync(host.data(), wout, host.size(), hipMemcpyDeviceToHost, s));
+  G16_CHECK(hipStreamSynchronize(s));
+  auto g1w = [&](size_t off, size_t W) {
+    std::vector<G1X> v(W);
+    std::memcpy(v.data(), host.data() + off, 128 * W);
+    return v;
+  };
+  std::vector<G2X> b2(Ww);
+  std::memcpy(b2.data(), host.data() + 384 * Ww, 256 * Ww);
+  g1_store(out->a_g1, xyzz_to_aff(horner(g1w(0, Ww), pw.c)));
+  g1_store(out->b_g1, xyzz_to_aff(horner(g1w(128 * Ww, Ww), pw.c)));
+  g1_store(out->l_g1, xyzz_to_aff(horner(g1w(256 * Ww, Ww), pw.c)));
+  g2_store(out->b_g2, xyzz_to_aff(horner(b2, pw.c)));
+  g1_store(out->h_g1, xyzz_to_aff(horner(g1w(640 * Ww, Wh), ph.c)));
+  if (msm_us) *msm_us += (uint64_t)(ev_ms(C->ev[4], C->ev[5]) * 1000.0f);
+}
+
+static void add_stage_timings(g16hip_circuit* C, g16hip_timings* t) {
+  if (!t) return;
+  t->gather_us += (uint64_t)(ev_ms(C->ev[0], C->ev[1]) * 1000.0f);
+  t->ntt_us += (uint64_t)(ev_ms(C->ev[1], C->ev[2]) * 1000.0f);
+  t->pointwise_us += (uint64_t)(ev_ms(C->ev[2], C->ev[3]) * 1000.0f);
+}
+
+static void load_witness(g16hip_circuit* C, const void* w, size_t n, bool on_device) {
+  check_witness_len(C, n);
+  G16_CHECK(hipMemcpyAsync(C->witness.p, w, 32 * n,
+                           on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, C->stream));
+}
+
+static void h_and_msms_locked(g16hip_circuit* C, const void* w, size_t n, bool on_device,
+                              g16hip_msm_out* out, g16hip_timings* t) {
+  load_witness(C, w, n, on_device);
+  stages_0_4(C, C->hbuf.p);
+  uint64_t msm_us = 0;
+  stages_5_9(C, C->hbuf.p, out, &msm_us);
+  add_stage_timings(C, t);
+  if (t) t->msm_us += msm_us;
+}
+
+// Stage 11 (crates/groth16/src/prove.rs:391-430), on the host.
+struct BlinderTerms {
+  G1X d1r, d1s, d1rs;
+  G2X d2s;
+  Fr r_std, s_std;
+};
+
+static BlinderTerms blinder_terms(const g16hip_circuit* C, const Fr& r, const Fr& s) {
+  BlinderTerms b;
+  b.r_std = to_std(r);
+  b.s_std = to_std(s);
+  Fr rs = to_std(mul(r, s));
+  b.d1r = xyzz_mul_plain(C->delta_g1, b.r_std.v);
+  b.d1s = xyzz_mul_plain(C->delta_g1, b.s_std.v);
+  b.d1rs = xyzz_mul_plain(C->delta_g1, rs.v);
+  b.d2s = xyzz_mul_plain(C->delta_g2, b.s_std.v);
+  return b;
+}
+
+static void assemble(const g16hip_circuit* C, const g16hip_msm_out& m, const BlinderTerms& b,
+                     g16hip_proof* proof) {
+  G1X pi_a = xyzz_add(xyzz_add_aff(xyzz_from_aff(g1_load(m.a_g1)), C->alpha_g1), b.d1r);
+  G2X pi_b = xyzz_add(xyzz_add_aff(xyzz_from_aff(g2_load(m.b_g2)), C->beta_g2), b.d2s);
+  G1X pib1 = xyzz_add(xyzz_add_aff(xyzz_from_aff(g1_load(m.b_g1)), C->beta_g1), b.d1s);
+  G1X pi_c = xyzz_add_aff(xyzz_from_aff(g1_load(m.l_g1)), g1_load(m.h_g1));
+  pi_c = xyzz_add(pi_c, xyzz_mul(pi_a, b.s_std.v));
+  pi_c = xyzz_add(pi_c, xyzz_mul(pib1, b.r_std.v));
+  pi_c = xyzz_add(pi_c, xyzz_neg(b.d1rs));
+  g1_store(proof->a, xyzz_to_aff(pi_a));
+  g2_store(proof->b, xyzz_to_aff(pi_b));
+  g1_store(proof->c, xyzz_to_aff(pi_c));
+}
+
+static g16hip_status prove_impl(g16hip_circuit* C, const void* w, size_t n, bool on_device,
+                                const uint8_t* r8, const uint8_t* s8, g16hip_proof* proof,
+                                g16hip_timings* t) {
+  return guarded([&] {
+    if (!C || !r8 || !s8 || !proof) throw ArgError{"null argument"};
+    check_witness_len(C, n);
+    // w[0] is the constant-one wire (crates/groth16/src/prove.rs:272-283).
+    Fr w0;
+    if (on_device) {
+      G16_CHECK(hipSetDevice(C->dev->ordinal));
+      G16_CHECK(hipMemcpy(w0.v, w, 32, hipMemcpyDeviceToHost));
+    } else {
+      std::memcpy(w0.v, w, 32);
+    }
+    if (!is_one(w0)) throw StatusError{G16HIP_CONSTANT_WIRE, "w[0] is the constant-one wire and must be 1"};
+    Fr r = fr_load(r8), s = fr_load(s8);
+    if (is_zero(r) || is_zero(s)) throw StatusError{G16HIP_ZERO_BLINDER, "a blinder is zero"};
+    BlinderTerms terms;
+    Timer tt;
+    std::thread th([&] { terms = blinder_terms(C, r, s); });
+    g16hip_msm_out m;
+    try {
+      std::lock_guard<std::mutex> lk(C->mu);
+      G16_CHECK(hipSetDevice(C->dev->ordinal));
+      h_and_msms_locked(C, w, n, on_device, &m, t);
+    } catch (...) {
+      th.join();
+      throw;
+    }
+    th.join();
+    Timer ta;
+    assemble(C, m, terms, proof);
+    if (t) t->assemble_us += ta.us();
+    return G16HIP_OK;
+  });
+}
+
+// ---------------------------------------------------------------------------------------
+// C ABI
 // ---------------------------------------------------------------------------------------
 extern "C" {
 
@@ -117,10 +536,12 @@ void g16hip_close(g16hip_device* d) {
   if (!d) return;
   (void)hipSetDevice(d->ordinal);
   (void)hipStreamSynchronize(d->stream);
+  d->ntt.clear();
   d->scratch_a.release();
   d->scratch_b.release();
   d->scratch_c.release();
   d->scratch_d.release();
+  d->scratch_e.release();
   (void)hipStreamDestroy(d->stream);
   delete d;
 }
@@ -167,6 +588,249 @@ g16hip_status g16hip_fixed_base(g16hip_device* d, int group, const uint8_t* k, s
     G16_CHECK(hipStreamSynchronize(d->stream));
     return G16HIP_OK;
   });
+}
+
+static g16hip_status msm_host(g16hip_device* d, int group, const uint8_t* bases,
+                              const uint8_t* scalars, size_t n, uint8_t* out) {
+  return guarded([&] {
+    if (!d || !out) throw ArgError{"null argument"};
+    size_t pw = group == 1 ? 64 : 128;
+    if (n == 0) {
+      std::memset(out, 0, pw);
+      return G16HIP_OK;
+    }
+    if (n >= (1u << 31)) throw ArgError{"MSM larger than 2^31 points"};
+    std::lock_guard<std::mutex> lk(d->mu);
+    set_device(d);
+    upload(d->scratch_a, bases, n * pw, d->stream);
+    upload(d->scratch_b, scalars, n * 32, d->stream);
+    if (group == 1)
+      g1_store(out, xyzz_to_aff(run_msm<Fq>(d, 1, d->scratch_a.p, d->scratch_b.p, (uint32_t)n, 0)));
+    else
+      g2_store(out, xyzz_to_aff(run_msm<Fq2>(d, 2, d->scratch_a.p, d->scratch_b.p, (uint32_t)n, 0)));
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_msm_g1(g16hip_device* d, const uint8_t* b, const uint8_t* s, size_t n,
+                            uint8_t* out) {
+  return msm_host(d, 1, b, s, n, out);
+}
+
+g16hip_status g16hip_msm_g2(g16hip_device* d, const uint8_t* b, const uint8_t* s, size_t n,
+                            uint8_t* out) {
+  return msm_host(d, 2, b, s, n, out);
+}
+
+static g16hip_status msm_dev(g16hip_device* d, int group, const void* bases, const void* scalars,
+                             size_t n, int c, uint8_t* out) {
+  return guarded([&] {
+    if (!d || !out) throw ArgError{"null argument"};
+    size_t pw = group == 1 ? 64 : 128;
+    if (n == 0) {
+      std::memset(out, 0, pw);
+      return G16HIP_OK;
+    }
+    if (n >= (1u << 31)) throw ArgError{"MSM larger than 2^31 points"};
+    std::lock_guard<std::mutex> lk(d->mu);
+    set_device(d);
+    if (group == 1)
+      g1_store(out, xyzz_to_aff(run_msm<Fq>(d, 1, bases, scalars, (uint32_t)n, c)));
+    else
+      g2_store(out, xyzz_to_aff(run_msm<Fq2>(d, 2, bases, scalars, (uint32_t)n, c)));
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_msm_g1_device(g16hip_device* d, const void* b, const void* s, size_t n, int c,
+                                   uint8_t* out) {
+  return msm_dev(d, 1, b, s, n, c, out);
+}
+
+g16hip_status g16hip_msm_g2_device(g16hip_device* d, const void* b, const void* s, size_t n, int c,
+                                   uint8_t* out) {
+  return msm_dev(d, 2, b, s, n, c, out);
+}
+
+static void ntt_on_device(g16hip_device* d, void* data, int L, int kind) {
+  NttTables& t = ntt_tables(d, L);
+  size_t n = (size_t)1 << L;
+  hipStream_t s = d->stream;
+  Fr* sc = t.scalars.as<Fr>();
+  if (kind == 0) {
+    // natural -> natural: permute, then the bit-reversed-input DIT.
+    d->scratch_c.alloc(32 * n);
+    launch_bitrev_permute(d->scratch_c.p, data, L, s);
+    G16_CHECK(hipMemcpyAsync(data, d->scratch_c.p, 32 * n, hipMemcpyDeviceToDevice, s));
+    ntt_dit_forward(data, 1, L, t.fwd.p, nullptr, s);
+  } else if (kind == 1) {
+    ntt_dif_inverse(data, 1, L, t.inv.p, s);
+    d->scratch_c.alloc(32 * n);
+    launch_bitrev_permute(d->scratch_c.p, data, L, s);
+    G16_CHECK(hipMemcpyAsync(data, d->scratch_c.p, 32 * n, hipMemcpyDeviceToDevice, s));
+    launch_scale(data, n, sc + 3, s);
+  } else {
+    ntt_dif_inverse(data, 1, L, t.inv.p, s);
+    ntt_dit_forward(data, 1, L, t.fwd.p, t.coset.p, s);
+  }
+}
+
+g16hip_status g16hip_ntt(g16hip_device* d, uint8_t* data, int log_n, int kind) {
+  return guarded([&] {
+    if (!d || !data) throw ArgError{"null argument"};
+    if (log_n < 0 || log_n > 27 || kind < 0 || kind > 2) throw ArgError{"bad NTT size or kind"};
+    std::lock_guard<std::mutex> lk(d->mu);
+    set_device(d);
+    size_t n = (size_t)1 << log_n;
+    upload(d->scratch_a, data, 32 * n, d->stream);
+    ntt_on_device(d, d->scratch_a.p, log_n, kind);
+    G16_CHECK(hipMemcpyAsync(data, d->scratch_a.p, 32 * n, hipMemcpyDeviceToHost, d->stream));
+    G16_CHECK(hipStreamSynchronize(d->stream));
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_ntt_device(g16hip_device* d, void* data, int log_n, int kind) {
+  return guarded([&] {
+    if (!d || !data) throw ArgError{"null argument"};
+    if (log_n < 0 || log_n > 27 || kind < 0 || kind > 2) throw ArgError{"bad NTT size or kind"};
+    std::lock_guard<std::mutex> lk(d->mu);
+    set_device(d);
+    ntt_on_device(d, data, log_n, kind);
+    G16_CHECK(hipStreamSynchronize(d->stream));
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_prepare(g16hip_device* d, const g16hip_key* key, g16hip_circuit** out) {
+  return guarded([&] {
+    if (!d || !out) throw ArgError{"null argument"};
+    validate_key(key);
+    set_device(d);
+    auto C = std::make_unique<g16hip_circuit>();
+    C->dev = d;
+    G16_CHECK(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
+    for (auto& e : C->ev) G16_CHECK(hipEventCreate(&e));
+    try {
+      prepare_circuit(C.get(), key);
+    } catch (...) {
+      (void)hipStreamSynchronize(C->stream);
+      for (auto& e : C->ev) (void)hipEventDestroy(e);
+      (void)hipStreamDestroy(C->stream);
+      throw;
+    }
+    *out = C.release();
+    return G16HIP_OK;
+  });
+}
+
+void g16hip_circuit_release(g16hip_circuit* C) {
+  if (!C) return;
+  (void)hipSetDevice(C->dev->ordinal);
+  (void)hipStreamSynchronize(C->stream);
+  for (auto& e : C->ev) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(C->stream);
+  delete C;
+}
+
+g16hip_status g16hip_circuit_dims(const g16hip_circuit* C, uint32_t* nv, uint32_t* np,
+                                  uint32_t* dom) {
+  return guarded([&] {
+    if (!C) throw ArgError{"null circuit"};
+    if (nv) *nv = C->n_vars;
+    if (np) *np = C->n_public;
+    if (dom) *dom = C->domain;
+    return G16HIP_OK;
+  });
+}
+
+size_t g16hip_circuit_resident_bytes(const g16hip_circuit* C) { return C ? C->resident : 0; }
+
+g16hip_status g16hip_compute_h(g16hip_circuit* C, const uint8_t* w, size_t n, g16hip_h** h_out,
+                               g16hip_timings* t) {
+  return guarded([&] {
+    if (!C || !w || !h_out) throw ArgError{"null argument"};
+    std::lock_guard<std::mutex> lk(C->mu);
+    G16_CHECK(hipSetDevice(C->dev->ordinal));
+    load_witness(C, w, n, false);
+    auto H = std::make_unique<g16hip_h>();
+    H->owner = C;
+    H->buf.alloc(32 * (size_t)C->domain);
+    stages_0_4(C, H->buf.p);
+    G16_CHECK(hipStreamSynchronize(C->stream));
+    add_stage_timings(C, t);
+    *h_out = H.release();
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_h_to_host(g16hip_circuit* C, const g16hip_h* h, uint8_t* out) {
+  return guarded([&] {
+    if (!C || !h || !out) throw ArgError{"null argument"};
+    if (h->owner != C) throw ArgError{"H handle came from another circuit's compute_h"};
+    std::lock_guard<std::mutex> lk(C->mu);
+    G16_CHECK(hipSetDevice(C->dev->ordinal));
+    G16_CHECK(hipMemcpy(out, h->buf.p, 32 * (size_t)C->domain, hipMemcpyDeviceToHost));
+    return G16HIP_OK;
+  });
+}
+
+void g16hip_h_release(g16hip_h* h) {
+  if (!h) return;
+  (void)hipSetDevice(h->owner->dev->ordinal);
+  delete h;
+}
+
+g16hip_status g16hip_msms(g16hip_circuit* C, const uint8_t* w, size_t n, const g16hip_h* h,
+                          g16hip_msm_out* out, g16hip_timings* t) {
+  return guarded([&] {
+    if (!C || !w || !h || !out) throw ArgError{"null argument"};
+    if (h->owner != C)
+      throw ArgError{"H is a device handle from another circuit's compute_h, which this circuit cannot read"};
+    std::lock_guard<std::mutex> lk(C->mu);
+    G16_CHECK(hipSetDevice(C->dev->ordinal));
+    load_witness(C, w, n, false);
+    uint64_t us = 0;
+    stages_5_9(C, h->buf.p, out, &us);
+    if (t) t->msm_us += us;
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_h_and_msms(g16hip_circuit* C, const uint8_t* w, size_t n,
+                                g16hip_msm_out* out, g16hip_timings* t) {
+  return guarded([&] {
+    if (!C || !w || !out) throw ArgError{"null argument"};
+    std::lock_guard<std::mutex> lk(C->mu);
+    G16_CHECK(hipSetDevice(C->dev->ordinal));
+    h_and_msms_locked(C, w, n, false, out, t);
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_h_and_msms_device(g16hip_circuit* C, const void* w, size_t n,
+                                       g16hip_msm_out* out, g16hip_timings* t) {
+  return guarded([&] {
+    if (!C || !w || !out) throw ArgError{"null argument"};
+    std::lock_guard<std::mutex> lk(C->mu);
+    G16_CHECK(hipSetDevice(C->dev->ordinal));
+    h_and_msms_locked(C, w, n, true, out, t);
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_prove_with_blinders(g16hip_circuit* C, const uint8_t* w, size_t n,
+                                         const uint8_t* r, const uint8_t* s, g16hip_proof* proof,
+                                         g16hip_timings* t) {
+  if (!w) return fail(G16HIP_BACKEND, "null witness");
+  return prove_impl(C, w, n, false, r, s, proof, t);
+}
+
+g16hip_status g16hip_prove_with_blinders_device(g16hip_circuit* C, const void* w, size_t n,
+                                                const uint8_t* r, const uint8_t* s,
+                                                g16hip_proof* proof, g16hip_timings* t) {
+  if (!w) return fail(G16HIP_BACKEND, "null witness");
+  return prove_impl(C, w, n, true, r, s, proof, t);
 }
 
 }  // extern "C"

@@ -1,0 +1,47 @@
+"""Device buffers for tests through the HIP runtime's own C API (hipMalloc / hipMemcpy), so the
+device-pointer entry points can be exercised without importing torch."""
+import ctypes
+
+import numpy as np
+
+_hip = None
+
+
+def _lib():
+    global _hip
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        _hip.hipFree.argtypes = [ctypes.c_void_p]
+        _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        _hip.hipDeviceSynchronize.argtypes = []
+    return _hip
+
+
+class DeviceArray:
+    H2D, D2H = 1, 2
+
+    def __init__(self, host: np.ndarray):
+        host = np.ascontiguousarray(host)
+        self.nbytes = host.nbytes
+        self.ptr = ctypes.c_void_p()
+        assert _lib().hipMalloc(ctypes.byref(self.ptr), max(self.nbytes, 1)) == 0
+        if self.nbytes:
+            assert _lib().hipMemcpy(self.ptr, host.ctypes.data, self.nbytes, self.H2D) == 0
+
+    @property
+    def addr(self) -> int:
+        return self.ptr.value
+
+    def to_host(self, like: np.ndarray) -> np.ndarray:
+        out = np.empty_like(like)
+        assert _lib().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, self.D2H) == 0
+        return out
+
+    def free(self):
+        if self.ptr:
+            _lib().hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        self.free()
