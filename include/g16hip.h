@@ -173,6 +173,32 @@ g16hip_status g16hip_msm_g1_device(g16hip_device* dev, const void* d_bases, cons
 g16hip_status g16hip_msm_g2_device(g16hip_device* dev, const void* d_bases, const void* d_scalars,
                                    size_t n, int c, uint8_t* out);
 
+/* Phase timings of the last standalone MSM on `dev` (HIP events on the stream its kernels
+ * ran on), the per-stage split the reference's StageTimings gives for a proof
+ * (crates/groth16/src/lib.rs:322-368) at kernel granularity. */
+typedef struct {
+  float sort_ms;       /* digits + histogram + scan + scatter */
+  float accumulate_ms; /* the bucket-accumulation kernel alone */
+  float merge_ms;      /* spanning and heavy buckets */
+  float reduce_ms;     /* window reduction */
+  float total_ms;      /* first kernel start to window sums on the host */
+  uint64_t entries;    /* sorted (window, bucket) entries = bucket additions */
+  uint32_t c;          /* window width used */
+  uint32_t windows;
+} g16hip_msm_stats;
+g16hip_status g16hip_msm_last_stats(g16hip_device* dev, g16hip_msm_stats* out);
+
+/* Running totals over every bucket-accumulation kernel launched on `dev` (standalone MSMs and
+ * proofs alike) since the last reset, index 0 = G1, 1 = G2: the per-launch figures a roofline
+ * needs (work = entries, time = HIP events on the launch stream). */
+typedef struct {
+  uint64_t launches[2];
+  uint64_t entries[2];
+  double accumulate_ms[2];
+} g16hip_kernel_stats;
+g16hip_status g16hip_kernel_stats_get(g16hip_device* dev, g16hip_kernel_stats* out);
+g16hip_status g16hip_kernel_stats_reset(g16hip_device* dev);
+
 /* Standalone NTT over Fr, in place on host memory (Fr[2^log_n] Montgomery, natural order):
  * kind 0 = forward, 1 = inverse (with 1/n), 2 = the prover's coset pipeline
  * iNTT -> x shift^j -> NTT (CpuNtt::{ntt, intt_coset_ntt}, crates/ntt/src/lib.rs:415, :508). */

@@ -73,8 +73,10 @@ void msm_sort(MsmPlan& p, const void* scalars_mont, hipStream_t s);
 size_t msm_accum_scratch_bytes(const MsmPlan& p, int group);
 // Bucket accumulation + window reduction for one base vector over the sorted entries;
 // writes `windows` XYZZ window sums to `window_out` (device memory).
+// If `ev` is given, ev[0..3] are recorded before the accumulation kernel, after it, after the
+// merges and after the window reduction.
 void msm_accumulate(const MsmPlan& p, int group, const void* bases, void* scratch,
-                    void* window_out, hipStream_t s);
+                    void* window_out, hipStream_t s, hipEvent_t* ev = nullptr);
 int msm_default_c(uint32_t n);
 
 }  // namespace g16

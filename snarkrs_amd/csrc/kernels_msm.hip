@@ -235,9 +235,8 @@ size_t msm_sort_scratch_bytes(uint32_t n, int c) {
   int W = (RECODE_BITS + c - 1) / c;
   size_t K = (size_t)W << (c - 1);
   size_t E = (size_t)n * W;
-  size_t nblk = (K + 1023) / 1024;
   return align_up(4 * (K + 1)) * 3 + align_up(4 * 1025) + align_up(4 * E) * 2 +
-         align_up((size_t)n * 32) + align_up(16) + align_up(4 * K) + 0 * nblk;
+         align_up((size_t)n * 32) + align_up(16) + align_up(4 * K);
 }
 
 void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c) {
@@ -449,7 +448,7 @@ size_t msm_accum_scratch_bytes(const MsmPlan& p, int group) {
 
 template <class F>
 static void accumulate_impl(const MsmPlan& p, const void* bases, void* scratch, void* window_out,
-                            hipStream_t s) {
+                            hipStream_t s, hipEvent_t* ev) {
   constexpr size_t PT = sizeof(Xyzz<F>);
   size_t K = (size_t)p.windows * p.nb;
   size_t nseg = (p.max_entries + SEG - 1) / SEG;
@@ -465,11 +464,13 @@ static void accumulate_impl(const MsmPlan& p, const void* bases, void* scratch, 
   uint4* groups = reinterpret_cast<uint4*>(q);
   auto bp = static_cast<const uint4*>(bases);
 
+  if (ev) G16_CHECK(hipEventRecord(ev[0], s));
   if (nseg) {
     hipLaunchKernelGGL(k_accumulate<F>, dim3((unsigned)((nseg + 127) / 128)), dim3(128), 0, s,
                        p.keys, p.vals, p.total, nseg, bp, buckets, pfirst, plast);
     G16_LAUNCH_CHECK("msm accumulate");
   }
+  if (ev) G16_CHECK(hipEventRecord(ev[1], s));
   hipLaunchKernelGGL(k_merge<F>, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, s, p.offset,
                      (uint32_t)K, buckets, pfirst, plast, p.total, p.heavy);
   G16_LAUNCH_CHECK("msm merge");
@@ -477,6 +478,7 @@ static void accumulate_impl(const MsmPlan& p, const void* bases, void* scratch, 
   hipLaunchKernelGGL((k_heavy<F, HB>), dim3(256), dim3(HB), 0, s, p.offset, p.total, p.heavy,
                      buckets, pfirst, plast);
   G16_LAUNCH_CHECK("msm heavy merge");
+  if (ev) G16_CHECK(hipEventRecord(ev[2], s));
   uint32_t nthreads = (uint32_t)p.windows * G;
   hipLaunchKernelGGL(k_reduce_groups<F>, dim3((nthreads + 127) / 128), dim3(128), 0, s, buckets,
                      p.nb, p.windows, B, groups);
@@ -485,12 +487,13 @@ static void accumulate_impl(const MsmPlan& p, const void* bases, void* scratch, 
   hipLaunchKernelGGL((k_reduce_windows<F, RB>), dim3(p.windows), dim3(RB), 0, s, groups, G,
                      static_cast<uint4*>(window_out));
   G16_LAUNCH_CHECK("msm reduce windows");
+  if (ev) G16_CHECK(hipEventRecord(ev[3], s));
 }
 
 void msm_accumulate(const MsmPlan& p, int group, const void* bases, void* scratch,
-                    void* window_out, hipStream_t s) {
-  if (group == 1) accumulate_impl<Fq>(p, bases, scratch, window_out, s);
-  else accumulate_impl<Fq2>(p, bases, scratch, window_out, s);
+                    void* window_out, hipStream_t s, hipEvent_t* ev) {
+  if (group == 1) accumulate_impl<Fq>(p, bases, scratch, window_out, s, ev);
+  else accumulate_impl<Fq2>(p, bases, scratch, window_out, s, ev);
 }
 
 }  // namespace g16

@@ -158,6 +158,12 @@ static Xyzz<F> horner(const std::vector<Xyzz<F>>& S, int c) {
   return acc;
 }
 
+static float ev_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  G16_CHECK(hipEventElapsedTime(&ms, a, b));
+  return ms;
+}
+
 struct Timer {
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   uint64_t us() const {
@@ -179,7 +185,18 @@ struct g16hip_device {
   std::mutex mu;  // serialises the standalone entry points' scratch
   DevBuf scratch_a, scratch_b, scratch_c, scratch_d, scratch_e;
   std::map<int, std::unique_ptr<NttTables>> ntt;
+  hipEvent_t ev[6] = {};
+  g16hip_msm_stats stats = {};
+  std::mutex stats_mu;
+  g16hip_kernel_stats kstats = {};
 };
+
+static void note_accumulate(g16hip_device* d, int group, uint64_t entries, double ms) {
+  std::lock_guard<std::mutex> lk(d->stats_mu);
+  d->kstats.launches[group - 1] += 1;
+  d->kstats.entries[group - 1] += entries;
+  d->kstats.accumulate_ms[group - 1] += ms;
+}
 
 static void set_device(const g16hip_device* d) { G16_CHECK(hipSetDevice(d->ordinal)); }
 
@@ -223,13 +240,27 @@ static Xyzz<F> run_msm(g16hip_device* d, int group, const void* bases, const voi
   d->scratch_d.alloc(msm_sort_scratch_bytes(n, c));
   msm_plan_bind(plan, d->scratch_d.p, n, c);
   d->scratch_e.alloc(msm_accum_scratch_bytes(plan, group) + sizeof(Xyzz<F>) * plan.windows);
+  G16_CHECK(hipEventRecord(d->ev[0], d->stream));
   msm_sort(plan, scalars, d->stream);
   char* wout = static_cast<char*>(d->scratch_e.p) + msm_accum_scratch_bytes(plan, group);
-  msm_accumulate(plan, group, bases, d->scratch_e.p, wout, d->stream);
+  msm_accumulate(plan, group, bases, d->scratch_e.p, wout, d->stream, d->ev + 1);
   std::vector<Xyzz<F>> S(plan.windows);
+  uint32_t entries = 0;
   G16_CHECK(hipMemcpyAsync(S.data(), wout, sizeof(Xyzz<F>) * plan.windows,
                            hipMemcpyDeviceToHost, d->stream));
+  G16_CHECK(hipMemcpyAsync(&entries, plan.total, 4, hipMemcpyDeviceToHost, d->stream));
+  G16_CHECK(hipEventRecord(d->ev[5], d->stream));
   G16_CHECK(hipStreamSynchronize(d->stream));
+  g16hip_msm_stats& st = d->stats;
+  st.sort_ms = ev_ms(d->ev[0], d->ev[1]);
+  st.accumulate_ms = ev_ms(d->ev[1], d->ev[2]);
+  st.merge_ms = ev_ms(d->ev[2], d->ev[3]);
+  st.reduce_ms = ev_ms(d->ev[3], d->ev[4]);
+  st.total_ms = ev_ms(d->ev[0], d->ev[5]);
+  st.entries = entries;
+  st.c = (uint32_t)c;
+  st.windows = (uint32_t)plan.windows;
+  note_accumulate(d, group, entries, st.accumulate_ms);
   return horner(S, c);
 }
 
@@ -241,6 +272,7 @@ struct g16hip_circuit {
   std::mutex mu;
   hipStream_t stream = nullptr;
   hipEvent_t ev[8];
+  hipEvent_t mev[5][4];  // per MSM of a proof: around accumulation, merge, reduction
   uint32_t n_vars = 0, n_public = 0, domain = 0;
   int log_n = 0;
   G1Aff alpha_g1, beta_g1, delta_g1;
@@ -352,12 +384,6 @@ static void check_witness_len(g16hip_circuit* C, size_t n) {
                                                  std::to_string(C->n_vars)};
 }
 
-static float ev_ms(hipEvent_t a, hipEvent_t b) {
-  float ms = 0;
-  G16_CHECK(hipEventElapsedTime(&ms, a, b));
-  return ms;
-}
-
 // Stages 0-4 from the circuit's device witness buffer into `h_out` (device).
 static void stages_0_4(g16hip_circuit* C, void* h_out) {
   hipStream_t s = C->stream;
@@ -390,16 +416,22 @@ static void stages_5_9(g16hip_circuit* C, const void* h, g16hip_msm_out* out,
   char* wout = C->wout.as<char>();
   size_t Ww = pw.windows, Wh = ph.windows;
   msm_sort(pw, C->witness.p, s);
-  msm_accumulate(pw, 1, C->a_q.p, C->accum.p, wout, s);
-  msm_accumulate(pw, 1, C->b1_q.p, C->accum.p, wout + 128 * Ww, s);
-  msm_accumulate(pw, 1, C->l_q.p, C->accum.p, wout + 256 * Ww, s);
-  msm_accumulate(pw, 2, C->b2_q.p, C->accum.p, wout + 384 * Ww, s);
+  msm_accumulate(pw, 1, C->a_q.p, C->accum.p, wout, s, C->mev[0]);
+  msm_accumulate(pw, 1, C->b1_q.p, C->accum.p, wout + 128 * Ww, s, C->mev[1]);
+  msm_accumulate(pw, 1, C->l_q.p, C->accum.p, wout + 256 * Ww, s, C->mev[2]);
+  msm_accumulate(pw, 2, C->b2_q.p, C->accum.p, wout + 384 * Ww, s, C->mev[3]);
   msm_sort(ph, h, s);
-  msm_accumulate(ph, 1, C->h_q.p, C->accum.p, wout + 640 * Ww, s);
+  msm_accumulate(ph, 1, C->h_q.p, C->accum.p, wout + 640 * Ww, s, C->mev[4]);
   G16_CHECK(hipEventRecord(C->ev[5], s));
   std::vector<uint8_t> host(640 * Ww + 128 * Wh);
+  uint32_t totals[2] = {0, 0};
   G16_CHECK(hipMemcpyAsync(host.data(), wout, host.size(), hipMemcpyDeviceToHost, s));
+  G16_CHECK(hipMemcpyAsync(&totals[0], pw.total, 4, hipMemcpyDeviceToHost, s));
+  G16_CHECK(hipMemcpyAsync(&totals[1], ph.total, 4, hipMemcpyDeviceToHost, s));
   G16_CHECK(hipStreamSynchronize(s));
+  for (int k = 0; k < 5; ++k)
+    note_accumulate(C->dev, k == 3 ? 2 : 1, k == 4 ? totals[1] : totals[0],
+                    ev_ms(C->mev[k][0], C->mev[k][1]));
   auto g1w = [&](size_t off, size_t W) {
     std::vector<G1X> v(W);
     std::memcpy(v.data(), host.data() + off, 128 * W);
@@ -527,6 +559,7 @@ g16hip_status g16hip_open(int ordinal, g16hip_device** out) {
     d->ordinal = ordinal;
     G16_CHECK(hipSetDevice(ordinal));
     G16_CHECK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+    for (auto& e : d->ev) G16_CHECK(hipEventCreate(&e));
     *out = d;
     return G16HIP_OK;
   });
@@ -542,6 +575,8 @@ void g16hip_close(g16hip_device* d) {
   d->scratch_c.release();
   d->scratch_d.release();
   d->scratch_e.release();
+  for (auto& e : d->ev)
+    if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(d->stream);
   delete d;
 }
@@ -652,6 +687,33 @@ g16hip_status g16hip_msm_g2_device(g16hip_device* d, const void* b, const void* 
   return msm_dev(d, 2, b, s, n, c, out);
 }
 
+g16hip_status g16hip_msm_last_stats(g16hip_device* d, g16hip_msm_stats* out) {
+  return guarded([&] {
+    if (!d || !out) throw ArgError{"null argument"};
+    std::lock_guard<std::mutex> lk(d->mu);
+    *out = d->stats;
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_kernel_stats_get(g16hip_device* d, g16hip_kernel_stats* out) {
+  return guarded([&] {
+    if (!d || !out) throw ArgError{"null argument"};
+    std::lock_guard<std::mutex> lk(d->stats_mu);
+    *out = d->kstats;
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_kernel_stats_reset(g16hip_device* d) {
+  return guarded([&] {
+    if (!d) throw ArgError{"null argument"};
+    std::lock_guard<std::mutex> lk(d->stats_mu);
+    d->kstats = {};
+    return G16HIP_OK;
+  });
+}
+
 static void ntt_on_device(g16hip_device* d, void* data, int L, int kind) {
   NttTables& t = ntt_tables(d, L);
   size_t n = (size_t)1 << L;
@@ -711,11 +773,15 @@ g16hip_status g16hip_prepare(g16hip_device* d, const g16hip_key* key, g16hip_cir
     C->dev = d;
     G16_CHECK(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
     for (auto& e : C->ev) G16_CHECK(hipEventCreate(&e));
+    for (auto& row : C->mev)
+      for (auto& e : row) G16_CHECK(hipEventCreate(&e));
     try {
       prepare_circuit(C.get(), key);
     } catch (...) {
       (void)hipStreamSynchronize(C->stream);
       for (auto& e : C->ev) (void)hipEventDestroy(e);
+      for (auto& row : C->mev)
+        for (auto& e : row) (void)hipEventDestroy(e);
       (void)hipStreamDestroy(C->stream);
       throw;
     }
@@ -729,6 +795,8 @@ void g16hip_circuit_release(g16hip_circuit* C) {
   (void)hipSetDevice(C->dev->ordinal);
   (void)hipStreamSynchronize(C->stream);
   for (auto& e : C->ev) (void)hipEventDestroy(e);
+  for (auto& row : C->mev)
+    for (auto& e : row) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(C->stream);
   delete C;
 }

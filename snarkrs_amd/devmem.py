@@ -1,5 +1,6 @@
-"""Device buffers for tests through the HIP runtime's own C API (hipMalloc / hipMemcpy), so the
-device-pointer entry points can be exercised without importing torch."""
+"""Device buffers through the HIP runtime's own C API (hipMalloc / hipMemcpy), for callers
+that hand device pointers to the ``*_device`` entry points (resident witnesses, resident MSM
+inputs) without a framework's allocator."""
 import ctypes
 
 import numpy as np
@@ -18,6 +19,12 @@ def _lib():
     return _hip
 
 
+def synchronize() -> None:
+    rc = _lib().hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceSynchronize failed ({rc})")
+
+
 class DeviceArray:
     H2D, D2H = 1, 2
 
@@ -25,9 +32,10 @@ class DeviceArray:
         host = np.ascontiguousarray(host)
         self.nbytes = host.nbytes
         self.ptr = ctypes.c_void_p()
-        assert _lib().hipMalloc(ctypes.byref(self.ptr), max(self.nbytes, 1)) == 0
-        if self.nbytes:
-            assert _lib().hipMemcpy(self.ptr, host.ctypes.data, self.nbytes, self.H2D) == 0
+        if _lib().hipMalloc(ctypes.byref(self.ptr), max(self.nbytes, 1)) != 0:
+            raise MemoryError(f"hipMalloc of {self.nbytes} bytes failed")
+        if self.nbytes and _lib().hipMemcpy(self.ptr, host.ctypes.data, self.nbytes, self.H2D) != 0:
+            raise RuntimeError("hipMemcpy to device failed")
 
     @property
     def addr(self) -> int:
@@ -35,7 +43,8 @@ class DeviceArray:
 
     def to_host(self, like: np.ndarray) -> np.ndarray:
         out = np.empty_like(like)
-        assert _lib().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, self.D2H) == 0
+        if _lib().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, self.D2H) != 0:
+            raise RuntimeError("hipMemcpy to host failed")
         return out
 
     def free(self):

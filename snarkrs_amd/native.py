@@ -76,6 +76,25 @@ class Timings(ctypes.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class MsmStats(ctypes.Structure):
+    """``g16hip_msm_stats``: phase split of the last standalone MSM (milliseconds)."""
+
+    _fields_ = [("sort_ms", ctypes.c_float), ("accumulate_ms", ctypes.c_float),
+                ("merge_ms", ctypes.c_float), ("reduce_ms", ctypes.c_float),
+                ("total_ms", ctypes.c_float), ("entries", ctypes.c_uint64),
+                ("c", ctypes.c_uint32), ("windows", ctypes.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class KernelStats(ctypes.Structure):
+    """``g16hip_kernel_stats``: running totals over bucket-accumulation launches."""
+
+    _fields_ = [("launches", ctypes.c_uint64 * 2), ("entries", ctypes.c_uint64 * 2),
+                ("accumulate_ms", ctypes.c_double * 2)]
+
+
 class MsmOut(ctypes.Structure):
     _fields_ = [("a_g1", ctypes.c_uint8 * 64), ("b_g2", ctypes.c_uint8 * 128),
                 ("b_g1", ctypes.c_uint8 * 64), ("l_g1", ctypes.c_uint8 * 64),
@@ -144,6 +163,9 @@ EXPORTS = {
     "g16hip_msm_g2": (ctypes.c_int, [_vp, _u8p, _u8p, ctypes.c_size_t, _u8p]),
     "g16hip_msm_g1_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t, ctypes.c_int, _u8p]),
     "g16hip_msm_g2_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t, ctypes.c_int, _u8p]),
+    "g16hip_msm_last_stats": (ctypes.c_int, [_vp, ctypes.POINTER(MsmStats)]),
+    "g16hip_kernel_stats_get": (ctypes.c_int, [_vp, ctypes.POINTER(KernelStats)]),
+    "g16hip_kernel_stats_reset": (ctypes.c_int, [_vp]),
     "g16hip_ntt": (ctypes.c_int, [_vp, _u8p, ctypes.c_int, ctypes.c_int]),
     "g16hip_ntt_device": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int]),
     "g16hip_field_probe": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _u8p, _u8p, _u8p,
@@ -238,6 +260,20 @@ class Device:
         fn = self.lib.g16hip_msm_g1_device if group == 1 else self.lib.g16hip_msm_g2_device
         _check(fn(self.h, _vp(d_bases), _vp(d_scalars), n, c, _p8(out)))
         return out
+
+    def msm_stats(self) -> dict:
+        st = MsmStats()
+        _check(self.lib.g16hip_msm_last_stats(self.h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def kernel_stats(self) -> dict:
+        st = KernelStats()
+        _check(self.lib.g16hip_kernel_stats_get(self.h, ctypes.byref(st)))
+        return {"launches": list(st.launches), "entries": list(st.entries),
+                "accumulate_ms": list(st.accumulate_ms)}
+
+    def kernel_stats_reset(self) -> None:
+        _check(self.lib.g16hip_kernel_stats_reset(self.h))
 
     def ntt(self, data: np.ndarray, kind: int = 0) -> np.ndarray:
         a = _c8(data).reshape(-1, 32).copy()

@@ -54,7 +54,7 @@ def test_adversarial_classes(dev, orc, group):
 
 
 def test_every_window_width(dev, orc):
-    from tests.hipmem import DeviceArray
+    from snarkrs_amd.devmem import DeviceArray
 
     rng = SplitMix64(77)
     n = 3000
