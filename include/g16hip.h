@@ -211,6 +211,11 @@ g16hip_status g16hip_ntt_device(g16hip_device* dev, void* d_data, int log_n, int
 g16hip_status g16hip_field_probe(g16hip_device* dev, int field, int op, const uint8_t* a,
                                  const uint8_t* b, uint8_t* out, size_t n);
 
+/* VALU ceilings for the roofline, measured the way the reference's device microbenchmark is
+ * (bench/results/device-microbench.md:7-10): kind 0 = 32x32+64 multiply-adds per second
+ * (independent chains), kind 1 = Fq Montgomery products per second. */
+g16hip_status g16hip_valu_probe(g16hip_device* dev, int kind, double* per_second);
+
 /* Batched fixed-base multiplication k_i * G (G the standard generator of `group`, 1 or 2),
  * k Fr Montgomery, output affine: the device half of the synthetic setup that builds the
  * proving keys the benchmarks use (the role of crates/ceremony's setup MSM seam). */

@@ -240,35 +240,59 @@ HD Fp<C> inv(const Fp<C>& a) {
 // ---------------------------------------------------------------------------------------
 // Fq2 = Fq[u]/(u^2 + 1)
 // ---------------------------------------------------------------------------------------
-struct Fq2 {
+// Fq2T<true> expands its three Fq products in place; Fq2T<false> (= Fq2) calls one
+// out-of-line copy of the Fq multiply. A G2 point formula with every product expanded is
+// ~17k instructions, past the 64 KB instruction cache a CU pair shares, and costs the
+// compiler minutes per kernel, so only the G2 bucket-accumulation loop uses the expanded form.
+template <bool INL>
+struct Fq2T {
   Fq c0, c1;
 };
+typedef Fq2T<false> Fq2;
+typedef Fq2T<true> Fq2i;
 
-HD Fq2 fq2_zero() { return {fp_zero<FqCfg>(), fp_zero<FqCfg>()}; }
-HD Fq2 fq2_one() { return {fp_one<FqCfg>(), fp_zero<FqCfg>()}; }
-HD bool is_zero(const Fq2& a) { return is_zero(a.c0) && is_zero(a.c1); }
-HD bool eq(const Fq2& a, const Fq2& b) { return eq(a.c0, b.c0) && eq(a.c1, b.c1); }
-HD Fq2 add(const Fq2& a, const Fq2& b) { return {add(a.c0, b.c0), add(a.c1, b.c1)}; }
-HD Fq2 sub(const Fq2& a, const Fq2& b) { return {sub(a.c0, b.c0), sub(a.c1, b.c1)}; }
-HD Fq2 neg(const Fq2& a) { return {neg(a.c0), neg(a.c1)}; }
-HD Fq2 dbl(const Fq2& a) { return add(a, a); }
+template <int = 0>
+__host__ __device__ __attribute__((noinline)) Fq fq_mul_ool(const Fq& a, const Fq& b) {
+  return mul(a, b);
+}
+
+template <bool INL>
+HD Fq fq2_part_mul(const Fq& a, const Fq& b) {
+  if constexpr (INL) return mul(a, b);
+  else return fq_mul_ool(a, b);
+}
+
+template <bool INL = false> HD Fq2T<INL> fq2_zero() { return {fp_zero<FqCfg>(), fp_zero<FqCfg>()}; }
+template <bool INL = false> HD Fq2T<INL> fq2_one() { return {fp_one<FqCfg>(), fp_zero<FqCfg>()}; }
+template <bool INL> HD bool is_zero(const Fq2T<INL>& a) { return is_zero(a.c0) && is_zero(a.c1); }
+template <bool INL> HD bool eq(const Fq2T<INL>& a, const Fq2T<INL>& b) {
+  return eq(a.c0, b.c0) && eq(a.c1, b.c1);
+}
+template <bool INL> HD Fq2T<INL> add(const Fq2T<INL>& a, const Fq2T<INL>& b) {
+  return {add(a.c0, b.c0), add(a.c1, b.c1)};
+}
+template <bool INL> HD Fq2T<INL> sub(const Fq2T<INL>& a, const Fq2T<INL>& b) {
+  return {sub(a.c0, b.c0), sub(a.c1, b.c1)};
+}
+template <bool INL> HD Fq2T<INL> neg(const Fq2T<INL>& a) { return {neg(a.c0), neg(a.c1)}; }
+template <bool INL> HD Fq2T<INL> dbl(const Fq2T<INL>& a) { return add(a, a); }
 
 // Karatsuba: 3 Fq products.
-HD Fq2 mul(const Fq2& a, const Fq2& b) {
-  Fq v0 = mul(a.c0, b.c0);
-  Fq v1 = mul(a.c1, b.c1);
-  Fq s = mul(add(a.c0, a.c1), add(b.c0, b.c1));
+template <bool INL> HD Fq2T<INL> mul(const Fq2T<INL>& a, const Fq2T<INL>& b) {
+  Fq v0 = fq2_part_mul<INL>(a.c0, b.c0);
+  Fq v1 = fq2_part_mul<INL>(a.c1, b.c1);
+  Fq s = fq2_part_mul<INL>(add(a.c0, a.c1), add(b.c0, b.c1));
   return {sub(v0, v1), sub(sub(s, v0), v1)};
 }
 
 // (a0 + a1 u)^2 = (a0 + a1)(a0 - a1) + 2 a0 a1 u: 2 Fq products.
-HD Fq2 sqr(const Fq2& a) {
-  Fq m = mul(add(a.c0, a.c1), sub(a.c0, a.c1));
-  Fq t = mul(a.c0, a.c1);
+template <bool INL> HD Fq2T<INL> sqr(const Fq2T<INL>& a) {
+  Fq m = fq2_part_mul<INL>(add(a.c0, a.c1), sub(a.c0, a.c1));
+  Fq t = fq2_part_mul<INL>(a.c0, a.c1);
   return {m, add(t, t)};
 }
 
-HD Fq2 inv(const Fq2& a) {
+template <bool INL> HD Fq2T<INL> inv(const Fq2T<INL>& a) {
   Fq t = inv(add(sqr(a.c0), sqr(a.c1)));
   return {mul(a.c0, t), neg(mul(a.c1, t))};
 }
@@ -291,12 +315,17 @@ typedef Aff<Fq2> G2Aff;
 typedef Xyzz<Fq> G1X;
 typedef Xyzz<Fq2> G2X;
 
-template <class F> HD F f_zero();
-template <> HD Fq f_zero<Fq>() { return fp_zero<FqCfg>(); }
-template <> HD Fq2 f_zero<Fq2>() { return fq2_zero(); }
-template <class F> HD F f_one();
-template <> HD Fq f_one<Fq>() { return fp_one<FqCfg>(); }
-template <> HD Fq2 f_one<Fq2>() { return fq2_one(); }
+template <class F> struct FieldOps;
+template <> struct FieldOps<Fq> {
+  static HD Fq zero() { return fp_zero<FqCfg>(); }
+  static HD Fq one() { return fp_one<FqCfg>(); }
+};
+template <bool INL> struct FieldOps<Fq2T<INL>> {
+  static HD Fq2T<INL> zero() { return fq2_zero<INL>(); }
+  static HD Fq2T<INL> one() { return fq2_one<INL>(); }
+};
+template <class F> HD F f_zero() { return FieldOps<F>::zero(); }
+template <class F> HD F f_one() { return FieldOps<F>::one(); }
 
 template <class F>
 HD bool aff_is_inf(const Aff<F>& p) {

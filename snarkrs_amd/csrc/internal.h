@@ -24,6 +24,9 @@ void hip_check(hipError_t e, const char* what);
 void launch_field_probe(int field, int op, const void* a, const void* b, void* out, size_t n,
                         hipStream_t s);
 void launch_fixed_base(int group, const void* k, size_t n, void* out, hipStream_t s);
+// kind 0: 8 chains of v_mad_u64_u32 per lane, kind 1: 4 chains of Fq products per lane.
+void launch_valu_peak(int kind, const void* in, void* out, unsigned blocks, int iters,
+                      hipStream_t s);
 
 // ---- NTT kernels (kernels_ntt.hip) ----
 // Twiddle tables: fwd[i] = w^i, inv[i] = w^-i for i < n/2 (Montgomery Fr).

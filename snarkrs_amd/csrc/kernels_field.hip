@@ -112,3 +112,65 @@ void launch_fixed_base(int group, const void* k, size_t n, void* out, hipStream_
 }
 
 }  // namespace g16
+
+// ---- VALU ceilings (the method of the reference's bench/results/device-microbench.md:7-10:
+// dependent-free chains, long enough that launch cost vanishes) ---------------------------
+namespace g16 {
+
+// 8 independent 32x32+64 multiply-add chains per lane.
+__global__ void __launch_bounds__(256) k_mad_peak(const uint32_t* in, uint64_t* out, int iters) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t a[8];
+  uint64_t acc[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    a[k] = in[(t + k) & 1023];
+    acc[k] = a[k];
+  }
+  uint32_t b = in[t & 1023] | 1u;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = (uint64_t)a[k] * b + acc[k];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = (uint32_t)acc[k];
+  }
+  uint64_t x = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) x ^= acc[k];
+  out[t] = x;
+}
+
+// 4 independent chains of Fq Montgomery products per lane.
+__global__ void __launch_bounds__(256) k_fqmul_peak(const uint32_t* in, uint32_t* out, int iters) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  Fq x[4], y;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[k].v[j] = in[(t * 8 + k * 3 + j) & 1023] & 0x0fffffffu;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) y.v[j] = in[(t + j) & 1023] & 0x0fffffffu;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) x[k] = mul(x[k], y);
+  }
+  uint32_t h = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) h ^= x[k].v[j];
+  out[t] = h;
+}
+
+void launch_valu_peak(int kind, const void* in, void* out, unsigned blocks, int iters,
+                      hipStream_t s) {
+  if (kind == 0)
+    hipLaunchKernelGGL(k_mad_peak, dim3(blocks), dim3(256), 0, s,
+                       static_cast<const uint32_t*>(in), static_cast<uint64_t*>(out), iters);
+  else
+    hipLaunchKernelGGL(k_fqmul_peak, dim3(blocks), dim3(256), 0, s,
+                       static_cast<const uint32_t*>(in), static_cast<uint32_t*>(out), iters);
+  G16_LAUNCH_CHECK("valu peak");
+}
+
+}  // namespace g16

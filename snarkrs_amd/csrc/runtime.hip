@@ -625,6 +625,29 @@ g16hip_status g16hip_fixed_base(g16hip_device* d, int group, const uint8_t* k, s
   });
 }
 
+g16hip_status g16hip_valu_probe(g16hip_device* d, int kind, double* per_second) {
+  return guarded([&] {
+    if (!d || !per_second || kind < 0 || kind > 1) throw ArgError{"bad argument"};
+    std::lock_guard<std::mutex> lk(d->mu);
+    set_device(d);
+    std::vector<uint32_t> seed(1024);
+    for (uint32_t i = 0; i < 1024; ++i) seed[i] = 0x9E3779B9u * (i + 1);
+    upload(d->scratch_a, seed.data(), 4096, d->stream);
+    const unsigned blocks = 256 * 16;  // 16 workgroups of 256 per CU
+    const int iters = kind == 0 ? 4096 : 256;
+    d->scratch_b.alloc((size_t)blocks * 256 * 8);
+    launch_valu_peak(kind, d->scratch_a.p, d->scratch_b.p, blocks, 8, d->stream);  // warm
+    G16_CHECK(hipEventRecord(d->ev[0], d->stream));
+    launch_valu_peak(kind, d->scratch_a.p, d->scratch_b.p, blocks, iters, d->stream);
+    G16_CHECK(hipEventRecord(d->ev[1], d->stream));
+    G16_CHECK(hipStreamSynchronize(d->stream));
+    double sec = ev_ms(d->ev[0], d->ev[1]) / 1e3;
+    double ops = (double)blocks * 256 * iters * (kind == 0 ? 8 : 4);
+    *per_second = ops / sec;
+    return G16HIP_OK;
+  });
+}
+
 static g16hip_status msm_host(g16hip_device* d, int group, const uint8_t* bases,
                               const uint8_t* scalars, size_t n, uint8_t* out) {
   return guarded([&] {

@@ -170,6 +170,7 @@ EXPORTS = {
     "g16hip_ntt_device": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int]),
     "g16hip_field_probe": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _u8p, _u8p, _u8p,
                                           ctypes.c_size_t]),
+    "g16hip_valu_probe": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "g16hip_fixed_base": (ctypes.c_int, [_vp, ctypes.c_int, _u8p, ctypes.c_size_t, _u8p]),
 }
 
@@ -235,6 +236,12 @@ class Device:
         out = np.zeros_like(a)
         _check(self.lib.g16hip_field_probe(self.h, field, op, _p8(a), _p8(b), _p8(out), len(a)))
         return out
+
+    def valu_probe(self, kind: int) -> float:
+        """kind 0: v_mad_u64_u32 per second; kind 1: Fq products per second."""
+        x = ctypes.c_double()
+        _check(self.lib.g16hip_valu_probe(self.h, kind, ctypes.byref(x)))
+        return x.value
 
     def fixed_base(self, group: int, k: np.ndarray) -> np.ndarray:
         k = _c8(k).reshape(-1, 32)
