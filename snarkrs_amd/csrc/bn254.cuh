@@ -168,6 +168,9 @@ HD Fp<C> dbl(const Fp<C>& a) {
 }
 
 // Montgomery product a*b*2^-256 mod p, CIOS (Koc, Acar, Kaliski 1996) over 32-bit limbs.
+// Each step is one 32x32+32 -> 64 multiply-add (the partial product plus the running word)
+// and a 32-bit add-with-carry of the previous step's high word: the carry chain stays in
+// 32-bit registers and the carry flag instead of 64-bit register pairs.
 template <class C>
 HD Fp<C> mul(const Fp<C>& a, const Fp<C>& b) {
   uint32_t t[10];
@@ -175,28 +178,26 @@ HD Fp<C> mul(const Fp<C>& a, const Fp<C>& b) {
   for (int i = 0; i < 10; ++i) t[i] = 0;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
-    uint64_t c = 0;
+    uint32_t c = 0, co;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      uint64_t x = (uint64_t)a.v[j] * b.v[i] + t[j] + c;
-      t[j] = (uint32_t)x;
-      c = x >> 32;
+      uint64_t x = (uint64_t)a.v[j] * b.v[i] + (uint64_t)t[j];
+      t[j] = __builtin_addc((uint32_t)x, c, 0u, &co);
+      c = (uint32_t)(x >> 32) + co;  // < 2^32: x's high word is at most 2^32 - 2
     }
-    uint64_t x = (uint64_t)t[8] + c;
-    t[8] = (uint32_t)x;
-    t[9] = (uint32_t)(x >> 32);
+    t[8] = __builtin_addc(t[8], c, 0u, &co);
+    t[9] = co;
     uint32_t m = t[0] * C::N0;
-    x = (uint64_t)m * C::p(0) + t[0];
-    c = x >> 32;
+    uint64_t x = (uint64_t)m * C::p(0) + (uint64_t)t[0];
+    c = (uint32_t)(x >> 32);
 #pragma unroll
     for (int j = 1; j < 8; ++j) {
-      x = (uint64_t)m * C::p(j) + t[j] + c;
-      t[j - 1] = (uint32_t)x;
-      c = x >> 32;
+      x = (uint64_t)m * C::p(j) + (uint64_t)t[j];
+      t[j - 1] = __builtin_addc((uint32_t)x, c, 0u, &co);
+      c = (uint32_t)(x >> 32) + co;
     }
-    x = (uint64_t)t[8] + c;
-    t[7] = (uint32_t)x;
-    t[8] = t[9] + (uint32_t)(x >> 32);
+    t[7] = __builtin_addc(t[8], c, 0u, &co);
+    t[8] = t[9] + co;
   }
   Fp<C> r;
 #pragma unroll

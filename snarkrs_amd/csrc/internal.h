@@ -66,8 +66,7 @@ struct MsmPlan {
   uint32_t* keys = nullptr;     // [max_entries]
   uint32_t* vals = nullptr;     // [max_entries]  (index << 1 | sign)
   uint32_t* scalars_std = nullptr;  // [n * 8]
-  uint32_t* total = nullptr;    // [4] entry count, heavy count
-  uint32_t* heavy = nullptr;    // [windows*nb] heavy bucket list
+  uint32_t* total = nullptr;    // [4] entry count
 };
 size_t msm_sort_scratch_bytes(uint32_t n, int c);
 void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c);

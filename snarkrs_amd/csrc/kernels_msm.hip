@@ -62,13 +62,29 @@ static void fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
   G16_LAUNCH_CHECK("fill");
 }
 
+__device__ __forceinline__ bool scalar_is_one(const uint32_t s[8]) {
+  return (s[0] == 1u) && ((s[1] | s[2] | s[3] | s[4] | s[5] | s[6] | s[7]) == 0u);
+}
+
+// A scalar equal to 1 has one nonzero digit, window 0 digit 1: key 0. Witnesses are full of
+// them (every set bit), so their counter is taken once per wavefront (ballot + popcount)
+// rather than once per lane, which would serialise a quarter of a million atomics on one
+// address.
 __global__ void __launch_bounds__(256) k_digits_count(const uint4* scalars, uint32_t n, int c,
                                                       int W, uint32_t nb, uint4* std_out,
                                                       uint32_t* count) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  Fr s = to_std(ld_fr(scalars + 2 * (size_t)i));
-  st_fr(std_out + 2 * (size_t)i, s);
+  bool live = i < n;
+  Fr s = fp_zero<FrCfg>();
+  if (live) {
+    s = to_std(ld_fr(scalars + 2 * (size_t)i));
+    st_fr(std_out + 2 * (size_t)i, s);
+  }
+  bool one = live && scalar_is_one(s.v);
+  uint64_t ones = __ballot(one);
+  if (ones && __lane_id() == (uint32_t)(__ffsll((unsigned long long)ones) - 1))
+    atomicAdd(&count[0], (uint32_t)__popcll(ones));
+  if (!live || one) return;
   for (int w = 0; w < W; ++w) {
     int d = signed_digit(s.v, w, c);
     if (d == 0) continue;
@@ -81,8 +97,24 @@ __global__ void __launch_bounds__(256) k_scatter(const uint4* std_in, uint32_t n
                                                  uint32_t nb, uint32_t* cursor, uint32_t* keys,
                                                  uint32_t* vals) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  Fr s = ld_fr(std_in + 2 * (size_t)i);
+  bool live = i < n;
+  Fr s = fp_zero<FrCfg>();
+  if (live) s = ld_fr(std_in + 2 * (size_t)i);
+  bool one = live && scalar_is_one(s.v);
+  uint64_t ones = __ballot(one);
+  if (ones) {
+    uint32_t leader = (uint32_t)(__ffsll((unsigned long long)ones) - 1);
+    uint32_t base = 0;
+    if (__lane_id() == leader) base = atomicAdd(&cursor[0], (uint32_t)__popcll(ones));
+    base = __shfl(base, (int)leader);
+    if (one) {
+      uint64_t below = ones & ((1ull << __lane_id()) - 1ull);
+      uint32_t pos = base + (uint32_t)__popcll(below);
+      keys[pos] = 0;
+      vals[pos] = i << 1;
+    }
+  }
+  if (!live || one) return;
   for (int w = 0; w < W; ++w) {
     int d = signed_digit(s.v, w, c);
     if (d == 0) continue;
@@ -149,7 +181,6 @@ __global__ void k_scan_fix(uint32_t* offset, uint32_t* cursor, const uint32_t* b
   if (i == 0) {
     offset[K] = bsum[1024];
     total[0] = bsum[1024];
-    total[1] = 0;  // heavy bucket count
   }
 }
 
@@ -177,7 +208,7 @@ size_t msm_sort_scratch_bytes(uint32_t n, int c) {
   size_t K = (size_t)W << (c - 1);
   size_t E = (size_t)n * W;
   return align_up(4 * (K + 1)) * 3 + align_up(4 * 1025) + align_up(4 * E) * 2 +
-         align_up((size_t)n * 32) + align_up(16) + align_up(4 * K);
+         align_up((size_t)n * 32) + align_up(16);
 }
 
 void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c) {
@@ -204,7 +235,6 @@ void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c) {
   p.vals = reinterpret_cast<uint32_t*>(take(4 * p.max_entries));
   p.scalars_std = reinterpret_cast<uint32_t*>(take((size_t)n * 32));
   p.total = reinterpret_cast<uint32_t*>(take(16));
-  p.heavy = reinterpret_cast<uint32_t*>(take(4 * K));
 }
 
 void msm_sort(MsmPlan& p, const void* scalars, hipStream_t s) {
@@ -237,7 +267,18 @@ size_t msm_accum_scratch_bytes(const MsmPlan& p, int group) {
   return msm_accum_scratch_bytes_impl(p, group);
 }
 
-// G2 lives in kernels_msm_g2.hip, compiled on its own (it is the slow one to compile).
+template <>
+void Tail<Fq>::merge(const MsmPlan& p, const AccumPtrs& a, hipStream_t s) {
+  tail_merge_impl<Fq>(p, a, s);
+}
+
+template <>
+void Tail<Fq>::reduce(const MsmPlan& p, const AccumPtrs& a, uint32_t B, void* window_out,
+                      hipStream_t s) {
+  tail_reduce_impl<Fq>(p, a, B, window_out, s);
+}
+
+// G2 lives in kernels_msm_g2*.hip, compiled on their own (the slow ones to compile).
 void msm_accumulate_g2(const MsmPlan& p, const void* bases, void* scratch, void* window_out,
                        hipStream_t s, hipEvent_t* ev);
 

@@ -9,7 +9,12 @@ namespace g16 {
 
 inline size_t accum_align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
-static constexpr int SEG = 32;          // entries per accumulation segment
+static constexpr int SEG = 32;
+
+template <int = 0>
+__global__ void k_clear_cnt(uint32_t* cnt) {
+  if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
+}          // entries per accumulation segment
 static constexpr int HEAVY_SPAN = 32;   // segments a merge thread folds itself
 
 __device__ __forceinline__ Fr ld_fr(const uint4* p) {
@@ -97,6 +102,11 @@ __global__ void __launch_bounds__(128) k_accumulate(const uint32_t* __restrict__
   bool starts = (b == 0) || keys[b - 1] != cur;
   size_t rs = b;
   Xyzz<F> acc = xyzz_inf<F>();
+  // G1: the next entry's point is loaded before this one is added, so the load latency of a
+  // random base hides behind a mixed addition (the G2 loop has no registers to spare).
+  constexpr bool PREFETCH = sizeof(F) == sizeof(Fq);
+  uint32_t v = vals[b];
+  Aff<F> a = ld_aff<F>(bases, v >> 1);
   for (size_t k = b; k < e; ++k) {
     uint32_t key = keys[k];
     if (key != cur) {
@@ -108,10 +118,21 @@ __global__ void __launch_bounds__(128) k_accumulate(const uint32_t* __restrict__
       starts = true;
       rs = k;
     }
-    uint32_t v = vals[k];
-    Aff<F> a = ld_aff<F>(bases, v >> 1);
-    if (v & 1) a = aff_neg(a);
-    acc = xyzz_add_aff(acc, a);
+    if (!PREFETCH && k != b) {
+      v = vals[k];
+      a = ld_aff<F>(bases, v >> 1);
+    }
+    uint32_t v_next = 0;
+    Aff<F> a_next;
+    if (PREFETCH && k + 1 < e) {
+      v_next = vals[k + 1];
+      a_next = ld_aff<F>(bases, v_next >> 1);
+    }
+    acc = xyzz_add_aff(acc, (v & 1) ? aff_neg(a) : a);
+    if (PREFETCH) {
+      v = v_next;
+      a = a_next;
+    }
   }
   bool ends = (e == T) || keys[e] != cur;
   if (starts && ends) {
@@ -122,10 +143,17 @@ __global__ void __launch_bounds__(128) k_accumulate(const uint32_t* __restrict__
   }
 }
 
-// Buckets that span segments: one thread folds a short span, a heavy one is queued.
+// Buckets that span segments: one thread folds a short span; a heavy one (e.g. every scalar
+// equal to 1, in bucket 0 of window 0) is cut into tasks of HEAVY_TASK interior segments
+// that a two-level tree sums (k_heavy_tasks, k_heavy_finish): depth log2 instead of a chain.
+// cnt[0] = heavy buckets, cnt[1] = tasks; heavy[h] = {bucket, first task, task count};
+// tasks[t] = {first segment, end segment}.
+static constexpr int HEAVY_TASK = 256;
+
 template <class F>
-__global__ void __launch_bounds__(256) k_merge(const uint32_t* offset, uint32_t K, uint4* buckets, const uint4* pfirst,
-                        const uint4* plast, uint32_t* total, uint32_t* heavy) {
+__global__ void __launch_bounds__(256) k_merge(const uint32_t* offset, uint32_t K, uint4* buckets,
+                                               const uint4* pfirst, const uint4* plast,
+                                               uint32_t* cnt, uint32_t* heavy, uint32_t* tasks) {
   uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= K) return;
   uint32_t s = offset[k], e = offset[k + 1];
@@ -136,8 +164,18 @@ __global__ void __launch_bounds__(256) k_merge(const uint32_t* offset, uint32_t 
   uint32_t sa = s / SEG, sb = (e - 1) / SEG;
   if (sa == sb) return;  // stored whole by the accumulation
   if (sb - sa > HEAVY_SPAN) {
-    uint32_t slot = atomicAdd(&total[1], 1u);
-    heavy[slot] = k;
+    uint32_t interior = sb - sa - 1;
+    uint32_t ntask = (interior + HEAVY_TASK - 1) / HEAVY_TASK;
+    uint32_t slot = atomicAdd(&cnt[0], 1u);
+    uint32_t tb = atomicAdd(&cnt[1], ntask);
+    heavy[3 * slot] = k;
+    heavy[3 * slot + 1] = tb;
+    heavy[3 * slot + 2] = ntask;
+    for (uint32_t t = 0; t < ntask; ++t) {
+      uint32_t lo = sa + 1 + t * HEAVY_TASK;
+      tasks[2 * (tb + t)] = lo;
+      tasks[2 * (tb + t) + 1] = lo + HEAVY_TASK < sb ? lo + HEAVY_TASK : sb;
+    }
     return;
   }
   Xyzz<F> acc = ld_xyzz<F>(plast, sa);
@@ -146,28 +184,52 @@ __global__ void __launch_bounds__(256) k_merge(const uint32_t* offset, uint32_t 
   st_xyzz(buckets, k, acc);
 }
 
-// One workgroup per heavy bucket: threads stride over the interior segments, then an LDS
-// tree, then the two end pieces.
+// Sum of BLOCK per-thread values in LDS, left in sh[0].
 template <class F, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_heavy(const uint32_t* offset, const uint32_t* total,
-                                                 const uint32_t* heavy, uint4* buckets,
-                                                 const uint4* pfirst, const uint4* plast) {
-  __shared__ Xyzz<F> sh[BLOCK];
-  uint32_t nh = total[1];
-  for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
-    uint32_t k = heavy[h];
-    uint32_t s = offset[k], e = offset[k + 1];
-    uint32_t sa = s / SEG, sb = (e - 1) / SEG;
-    Xyzz<F> acc = xyzz_inf<F>();
-    for (uint32_t t = sa + 1 + threadIdx.x; t < sb; t += BLOCK)
-      acc = xyzz_add(acc, ld_xyzz<F>(pfirst, t));
-    sh[threadIdx.x] = acc;
+__device__ __forceinline__ void block_tree(Xyzz<F>* sh) {
+  for (int off = BLOCK / 2; off > 0; off >>= 1) {
     __syncthreads();
-    for (int off = BLOCK / 2; off > 0; off >>= 1) {
-      if ((int)threadIdx.x < off) sh[threadIdx.x] = xyzz_add(sh[threadIdx.x], sh[threadIdx.x + off]);
-      __syncthreads();
-    }
+    if ((int)threadIdx.x < off) sh[threadIdx.x] = xyzz_add(sh[threadIdx.x], sh[threadIdx.x + off]);
+  }
+  __syncthreads();
+}
+
+// Level 1: each task's interior segment partials (one per thread) summed by an LDS tree.
+template <class F, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_heavy_tasks(const uint32_t* cnt, const uint32_t* tasks,
+                                                       const uint4* pfirst, uint4* task_sum) {
+  static_assert(BLOCK == HEAVY_TASK, "one segment partial per thread");
+  __shared__ Xyzz<F> sh[BLOCK];
+  uint32_t nt = cnt[1];
+  for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+    uint32_t lo = tasks[2 * t], hi = tasks[2 * t + 1];
+    uint32_t seg = lo + threadIdx.x;
+    sh[threadIdx.x] = seg < hi ? ld_xyzz<F>(pfirst, seg) : xyzz_inf<F>();
+    block_tree<F, BLOCK>(sh);
+    if (threadIdx.x == 0) st_xyzz(task_sum, t, sh[0]);
+    __syncthreads();
+  }
+}
+
+// Level 2: a heavy bucket's task sums (strided over the block, then a tree) plus its two end
+// pieces.
+template <class F, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_heavy_finish(const uint32_t* offset,
+                                                        const uint32_t* cnt,
+                                                        const uint32_t* heavy,
+                                                        const uint4* task_sum, uint4* buckets,
+                                                        const uint4* pfirst, const uint4* plast) {
+  __shared__ Xyzz<F> sh[BLOCK];
+  uint32_t nh = cnt[0];
+  for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
+    uint32_t k = heavy[3 * h], tb = heavy[3 * h + 1], nt = heavy[3 * h + 2];
+    Xyzz<F> acc = xyzz_inf<F>();
+    for (uint32_t t = threadIdx.x; t < nt; t += BLOCK) acc = xyzz_add(acc, ld_xyzz<F>(task_sum, tb + t));
+    sh[threadIdx.x] = acc;
+    block_tree<F, BLOCK>(sh);
     if (threadIdx.x == 0) {
+      uint32_t s = offset[k], e = offset[k + 1];
+      uint32_t sa = s / SEG, sb = (e - 1) / SEG;
       Xyzz<F> r = xyzz_add(ld_xyzz<F>(plast, sa), sh[0]);
       r = xyzz_add(r, ld_xyzz<F>(pfirst, sb));
       st_xyzz(buckets, k, r);
@@ -222,61 +284,120 @@ inline uint32_t reduce_group_size(const MsmPlan& p) {
   return B > p.nb ? p.nb : B;
 }
 
-inline size_t msm_accum_scratch_bytes_impl(const MsmPlan& p, int group) {
-  size_t pt = group == 1 ? 128 : 256;
+// Scratch of one accumulation (one base vector over one sorted entry list): buckets, the
+// segment end partials, heavy-bucket bookkeeping, task sums, reduction groups, counters.
+struct AccumLayout {
+  size_t buckets, pfirst, plast, heavy, tasks, task_sum, groups, cnt, total;
+};
+
+inline AccumLayout accum_layout(const MsmPlan& p, size_t pt) {
   size_t K = (size_t)p.windows * p.nb;
   size_t nseg = (p.max_entries + SEG - 1) / SEG;
+  size_t max_heavy = nseg / HEAVY_SPAN + 2;
+  size_t max_tasks = nseg / HEAVY_TASK + max_heavy + 2;
   uint32_t B = reduce_group_size(p);
   size_t G = p.nb / B;
-  return accum_align_up(pt * K) + 2 * accum_align_up(pt * (nseg + 1)) + accum_align_up(pt * G * p.windows);
+  AccumLayout L;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t r = off;
+    off += accum_align_up(bytes);
+    return r;
+  };
+  L.buckets = take(pt * K);
+  L.pfirst = take(pt * (nseg + 1));
+  L.plast = take(pt * (nseg + 1));
+  L.heavy = take(12 * max_heavy);
+  L.tasks = take(8 * max_tasks);
+  L.task_sum = take(pt * max_tasks);
+  L.groups = take(pt * G * p.windows);
+  L.cnt = take(16);
+  L.total = off;
+  return L;
 }
 
-// F: the coordinate field of the merge and reduction kernels; FHot: the same field as the
-// bucket-accumulation loop computes in (Fq2T<true> for G2: products expanded in place).
-template <class F, class FHot = F>
+inline size_t msm_accum_scratch_bytes_impl(const MsmPlan& p, int group) {
+  return accum_layout(p, group == 1 ? 128 : 256).total;
+}
+
+// Pointers into one accumulation's scratch.
+struct AccumPtrs {
+  uint4 *buckets, *pfirst, *plast, *task_sum, *groups;
+  uint32_t *heavy, *tasks, *cnt;
+};
+
+// Launchers of the merge and reduction kernels over field F. The G1 ones are defined here;
+// the G2 ones (F = Fq2i, products expanded in place so the latency-bound tail does not spill)
+// live in kernels_msm_g2_tail.hip and kernels_msm_g2_reduce.hip, so the slow G2 kernels
+// compile in parallel translation units.
+template <class F>
+struct Tail {
+  static void merge(const MsmPlan& p, const AccumPtrs& a, hipStream_t s);
+  static void reduce(const MsmPlan& p, const AccumPtrs& a, uint32_t B, void* window_out,
+                     hipStream_t s);
+};
+
+template <class F>
+inline void tail_merge_impl(const MsmPlan& p, const AccumPtrs& a, hipStream_t s) {
+  size_t K = (size_t)p.windows * p.nb;
+  hipLaunchKernelGGL(k_clear_cnt<0>, dim3(1), dim3(64), 0, s, a.cnt);
+  G16_LAUNCH_CHECK("msm clear counters");
+  hipLaunchKernelGGL(k_merge<F>, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, s, p.offset,
+                     (uint32_t)K, a.buckets, a.pfirst, a.plast, a.cnt, a.heavy, a.tasks);
+  G16_LAUNCH_CHECK("msm merge");
+  hipLaunchKernelGGL((k_heavy_tasks<F, HEAVY_TASK>), dim3(1024), dim3(HEAVY_TASK), 0, s, a.cnt,
+                     a.tasks, a.pfirst, a.task_sum);
+  G16_LAUNCH_CHECK("msm heavy tasks");
+  hipLaunchKernelGGL((k_heavy_finish<F, 64>), dim3(256), dim3(64), 0, s, p.offset, a.cnt, a.heavy,
+                     a.task_sum, a.buckets, a.pfirst, a.plast);
+  G16_LAUNCH_CHECK("msm heavy finish");
+}
+
+template <class F>
+inline void tail_reduce_impl(const MsmPlan& p, const AccumPtrs& a, uint32_t B, void* window_out,
+                             hipStream_t s) {
+  uint32_t G = p.nb / B;
+  uint32_t nthreads = (uint32_t)p.windows * G;
+  hipLaunchKernelGGL(k_reduce_groups<F>, dim3((nthreads + 127) / 128), dim3(128), 0, s,
+                     a.buckets, p.nb, p.windows, B, a.groups);
+  G16_LAUNCH_CHECK("msm reduce groups");
+  constexpr int RB = sizeof(F) == sizeof(Fq) ? 128 : 64;
+  hipLaunchKernelGGL((k_reduce_windows<F, RB>), dim3(p.windows), dim3(RB), 0, s, a.groups, G,
+                     static_cast<uint4*>(window_out));
+  G16_LAUNCH_CHECK("msm reduce windows");
+}
+
+// F: the coordinate field (Fq, or Fq2i for G2).
+template <class F>
 inline void accumulate_impl(const MsmPlan& p, const void* bases, void* scratch, void* window_out,
                             hipStream_t s, hipEvent_t* ev) {
-  static_assert(sizeof(F) == sizeof(FHot), "one layout");
   constexpr size_t PT = sizeof(Xyzz<F>);
-  size_t K = (size_t)p.windows * p.nb;
   size_t nseg = (p.max_entries + SEG - 1) / SEG;
   uint32_t B = reduce_group_size(p);
-  uint32_t G = p.nb / B;
+  AccumLayout L = accum_layout(p, PT);
   char* q = static_cast<char*>(scratch);
-  uint4* buckets = reinterpret_cast<uint4*>(q);
-  q += accum_align_up(PT * K);
-  uint4* pfirst = reinterpret_cast<uint4*>(q);
-  q += accum_align_up(PT * (nseg + 1));
-  uint4* plast = reinterpret_cast<uint4*>(q);
-  q += accum_align_up(PT * (nseg + 1));
-  uint4* groups = reinterpret_cast<uint4*>(q);
+  AccumPtrs a;
+  a.buckets = reinterpret_cast<uint4*>(q + L.buckets);
+  a.pfirst = reinterpret_cast<uint4*>(q + L.pfirst);
+  a.plast = reinterpret_cast<uint4*>(q + L.plast);
+  a.heavy = reinterpret_cast<uint32_t*>(q + L.heavy);
+  a.tasks = reinterpret_cast<uint32_t*>(q + L.tasks);
+  a.task_sum = reinterpret_cast<uint4*>(q + L.task_sum);
+  a.groups = reinterpret_cast<uint4*>(q + L.groups);
+  a.cnt = reinterpret_cast<uint32_t*>(q + L.cnt);
   auto bp = static_cast<const uint4*>(bases);
 
   if (ev) G16_CHECK(hipEventRecord(ev[0], s));
   if (nseg) {
-    hipLaunchKernelGGL(k_accumulate<FHot>, dim3((unsigned)((nseg + 127) / 128)), dim3(128), 0, s,
-                       p.keys, p.vals, p.total, nseg, bp, buckets, pfirst, plast);
+    hipLaunchKernelGGL(k_accumulate<F>, dim3((unsigned)((nseg + 127) / 128)), dim3(128), 0, s,
+                       p.keys, p.vals, p.total, nseg, bp, a.buckets, a.pfirst, a.plast);
     G16_LAUNCH_CHECK("msm accumulate");
   }
   if (ev) G16_CHECK(hipEventRecord(ev[1], s));
-  hipLaunchKernelGGL(k_merge<F>, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, s, p.offset,
-                     (uint32_t)K, buckets, pfirst, plast, p.total, p.heavy);
-  G16_LAUNCH_CHECK("msm merge");
-  constexpr int HB = sizeof(F) == sizeof(Fq) ? 128 : 64;
-  hipLaunchKernelGGL((k_heavy<F, HB>), dim3(256), dim3(HB), 0, s, p.offset, p.total, p.heavy,
-                     buckets, pfirst, plast);
-  G16_LAUNCH_CHECK("msm heavy merge");
+  Tail<F>::merge(p, a, s);
   if (ev) G16_CHECK(hipEventRecord(ev[2], s));
-  uint32_t nthreads = (uint32_t)p.windows * G;
-  hipLaunchKernelGGL(k_reduce_groups<F>, dim3((nthreads + 127) / 128), dim3(128), 0, s, buckets,
-                     p.nb, p.windows, B, groups);
-  G16_LAUNCH_CHECK("msm reduce groups");
-  constexpr int RB = sizeof(F) == sizeof(Fq) ? 128 : 64;
-  hipLaunchKernelGGL((k_reduce_windows<F, RB>), dim3(p.windows), dim3(RB), 0, s, groups, G,
-                     static_cast<uint4*>(window_out));
-  G16_LAUNCH_CHECK("msm reduce windows");
+  Tail<F>::reduce(p, a, B, window_out, s);
   if (ev) G16_CHECK(hipEventRecord(ev[3], s));
 }
-
 
 }  // namespace g16

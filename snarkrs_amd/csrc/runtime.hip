@@ -2,7 +2,9 @@
 // (stages 0-9 on the GPU, stage 11 on the host) and the standalone entry points.
 #include <algorithm>
 #include <chrono>
+#include <exception>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -267,11 +269,20 @@ static Xyzz<F> run_msm(g16hip_device* d, int group, const void* bases, const voi
 // ---------------------------------------------------------------------------------------
 // prepared circuit
 // ---------------------------------------------------------------------------------------
+// A proof runs on three streams of its circuit: stream (0) stages 0-4, the H sort and the H
+// MSM; side[0] the witness sort and the A, B1 and L MSMs; side[1] the B2 (G2) MSM. Stages 5-8
+// read only the witness, so they start as soon as it is on the device and overlap the
+// transforms, and the latency-bound tail of each MSM (merges, window reduction) overlaps the
+// throughput-bound accumulation of another.
 struct g16hip_circuit {
   g16hip_device* dev = nullptr;
   std::mutex mu;
   hipStream_t stream = nullptr;
+  hipStream_t side[2] = {nullptr, nullptr};
   hipEvent_t ev[8];
+  hipEvent_t sync_ev[4];  // witness ready, witness sorted, side[0] done, side[1] done
+  hipEvent_t done_ev[5];  // window sums of A, B1, L, B2, H copied to host_wout
+  uint8_t* host_wout = nullptr;  // pinned: window sums + the two entry totals
   hipEvent_t mev[5][4];  // per MSM of a proof: around accumulation, merge, reduction
   uint32_t n_vars = 0, n_public = 0, domain = 0;
   int log_n = 0;
@@ -281,7 +292,7 @@ struct g16hip_circuit {
   DevBuf a_q, b1_q, b2_q, l_q, h_q;
   NttTables tables;
   DevBuf witness, abc, hbuf;
-  DevBuf sort_w, sort_h, accum, wout;
+  DevBuf sort_w, sort_h, accum_w1, accum_w2, accum_h, wout;
   MsmPlan plan_w, plan_h;
   size_t resident = 0;
 };
@@ -363,16 +374,20 @@ static void prepare_circuit(g16hip_circuit* C, const g16hip_key* k) {
   C->sort_h.alloc(msm_sort_scratch_bytes(n, ch));
   msm_plan_bind(C->plan_w, C->sort_w.p, (uint32_t)nv, cw);
   msm_plan_bind(C->plan_h, C->sort_h.p, n, ch);
-  size_t acc = std::max({msm_accum_scratch_bytes(C->plan_w, 1), msm_accum_scratch_bytes(C->plan_w, 2),
-                         msm_accum_scratch_bytes(C->plan_h, 1)});
-  C->accum.alloc(acc);
+  C->accum_w1.alloc(msm_accum_scratch_bytes(C->plan_w, 1));
+  C->accum_w2.alloc(msm_accum_scratch_bytes(C->plan_w, 2));
+  C->accum_h.alloc(msm_accum_scratch_bytes(C->plan_h, 1));
   C->wout.alloc(256 * (size_t)(4 * C->plan_w.windows + C->plan_h.windows));
+  G16_CHECK(hipHostMalloc((void**)&C->host_wout,
+                          256 * (size_t)(4 * C->plan_w.windows + C->plan_h.windows) + 64,
+                          hipHostMallocDefault));
   G16_CHECK(hipStreamSynchronize(s));
   size_t tot = 0;
   DevBuf* bufs[] = {&C->row_ptr[0], &C->row_ptr[1], &C->signal[0], &C->signal[1], &C->value_id[0],
                     &C->value_id[1], &C->values, &C->a_q, &C->b1_q, &C->b2_q, &C->l_q, &C->h_q,
                     &C->tables.fwd, &C->tables.inv, &C->tables.coset, &C->witness, &C->abc,
-                    &C->hbuf, &C->sort_w, &C->sort_h, &C->accum, &C->wout};
+                    &C->hbuf, &C->sort_w, &C->sort_h, &C->accum_w1, &C->accum_w2,
+                    &C->accum_h, &C->wout};
   for (auto b : bufs) tot += b->bytes;
   C->resident = tot;
 }
@@ -406,45 +421,149 @@ static void stages_0_4(g16hip_circuit* C, void* h_out) {
   G16_CHECK(hipEventRecord(C->ev[3], s));
 }
 
-// Stages 5-9: five MSMs, witness from the circuit's device buffer, H from `h` (device).
-static void stages_5_9(g16hip_circuit* C, const void* h, g16hip_msm_out* out,
-                       uint64_t* msm_us) {
-  hipStream_t s = C->stream;
-  G16_CHECK(hipEventRecord(C->ev[4], s));
+// Host work that rides on the MSM results: with `blind`, s*A and r*B1 (the two stage-11
+// products that need an MSM output) are computed as soon as A and B1 are done, while the GPU
+// is still on the H MSM.
+struct BlindProducts {
+  bool want = false;
+  Fr r_std, s_std;
+  G1X sA, rB1;
+};
+
+// Runs fn() on a thread and keeps the first exception for the joiner.
+struct HostTask {
+  std::exception_ptr err;  // declared first: the thread may set it as soon as it starts
+  std::thread th;
+  template <class Fn>
+  explicit HostTask(Fn fn) : th([this, fn] {
+    try {
+      fn();
+    } catch (...) {
+      err = std::current_exception();
+    }
+  }) {}
+  void join() {
+    if (th.joinable()) th.join();
+  }
+};
+
+// Stages 5-9 (and, with `with_h`, stages 0-4 first on the main stream): the witness is
+// already being copied to C->witness on the main stream when this is called; H is read from
+// `h` (device), which stages 0-4 write when `with_h`. Each MSM's window sums go to pinned
+// host memory on its own stream, and a host thread finishes that MSM (Horner, affine) as soon
+// as they land.
+static void run_stages(g16hip_circuit* C, bool with_h, void* h, g16hip_msm_out* out,
+                       g16hip_timings* t, BlindProducts* blind = nullptr) {
+  hipStream_t s0 = C->stream, s1 = C->side[0], s2 = C->side[1];
   MsmPlan& pw = C->plan_w;
   MsmPlan& ph = C->plan_h;
   char* wout = C->wout.as<char>();
-  size_t Ww = pw.windows, Wh = ph.windows;
-  msm_sort(pw, C->witness.p, s);
-  msm_accumulate(pw, 1, C->a_q.p, C->accum.p, wout, s, C->mev[0]);
-  msm_accumulate(pw, 1, C->b1_q.p, C->accum.p, wout + 128 * Ww, s, C->mev[1]);
-  msm_accumulate(pw, 1, C->l_q.p, C->accum.p, wout + 256 * Ww, s, C->mev[2]);
-  msm_accumulate(pw, 2, C->b2_q.p, C->accum.p, wout + 384 * Ww, s, C->mev[3]);
-  msm_sort(ph, h, s);
-  msm_accumulate(ph, 1, C->h_q.p, C->accum.p, wout + 640 * Ww, s, C->mev[4]);
-  G16_CHECK(hipEventRecord(C->ev[5], s));
-  std::vector<uint8_t> host(640 * Ww + 128 * Wh);
-  uint32_t totals[2] = {0, 0};
-  G16_CHECK(hipMemcpyAsync(host.data(), wout, host.size(), hipMemcpyDeviceToHost, s));
-  G16_CHECK(hipMemcpyAsync(&totals[0], pw.total, 4, hipMemcpyDeviceToHost, s));
-  G16_CHECK(hipMemcpyAsync(&totals[1], ph.total, 4, hipMemcpyDeviceToHost, s));
-  G16_CHECK(hipStreamSynchronize(s));
+  uint8_t* hw = C->host_wout;
+  const size_t Ww = pw.windows, Wh = ph.windows;
+  const size_t off[5] = {0, 128 * Ww, 256 * Ww, 384 * Ww, 640 * Ww};
+  const size_t len[5] = {128 * Ww, 128 * Ww, 128 * Ww, 256 * Ww, 128 * Wh};
+  uint32_t* totals = reinterpret_cast<uint32_t*>(hw + 640 * Ww + 128 * Wh);
+  auto finish_on = [&](int k, hipStream_t st) {
+    G16_CHECK(hipMemcpyAsync(hw + off[k], wout + off[k], len[k], hipMemcpyDeviceToHost, st));
+    G16_CHECK(hipEventRecord(C->done_ev[k], st));
+  };
+  G16_CHECK(hipEventRecord(C->sync_ev[0], s0));  // witness on the device
+  G16_CHECK(hipEventRecord(C->ev[4], s0));
+  // side[0]: witness sort, then A, B1, L.
+  G16_CHECK(hipStreamWaitEvent(s1, C->sync_ev[0], 0));
+  msm_sort(pw, C->witness.p, s1);
+  G16_CHECK(hipEventRecord(C->sync_ev[1], s1));
+  G16_CHECK(hipMemcpyAsync(&totals[0], pw.total, 4, hipMemcpyDeviceToHost, s1));
+  msm_accumulate(pw, 1, C->a_q.p, C->accum_w1.p, wout + off[0], s1, C->mev[0]);
+  finish_on(0, s1);
+  msm_accumulate(pw, 1, C->b1_q.p, C->accum_w1.p, wout + off[1], s1, C->mev[1]);
+  finish_on(1, s1);
+  msm_accumulate(pw, 1, C->l_q.p, C->accum_w1.p, wout + off[2], s1, C->mev[2]);
+  finish_on(2, s1);
+  G16_CHECK(hipEventRecord(C->sync_ev[2], s1));
+  // side[1]: B2 on the same sorted entries.
+  G16_CHECK(hipStreamWaitEvent(s2, C->sync_ev[1], 0));
+  msm_accumulate(pw, 2, C->b2_q.p, C->accum_w2.p, wout + off[3], s2, C->mev[3]);
+  finish_on(3, s2);
+  G16_CHECK(hipEventRecord(C->sync_ev[3], s2));
+  // main: stages 0-4, then the H sort and MSM.
+  if (with_h) stages_0_4(C, h);
+  msm_sort(ph, h, s0);
+  G16_CHECK(hipMemcpyAsync(&totals[1], ph.total, 4, hipMemcpyDeviceToHost, s0));
+  msm_accumulate(ph, 1, C->h_q.p, C->accum_h.p, wout + off[4], s0, C->mev[4]);
+  finish_on(4, s0);
+  G16_CHECK(hipStreamWaitEvent(s0, C->sync_ev[2], 0));
+  G16_CHECK(hipStreamWaitEvent(s0, C->sync_ev[3], 0));
+  G16_CHECK(hipEventRecord(C->ev[5], s0));
+
+  auto g1_sum = [&](int k, int c) {
+    G16_CHECK(hipEventSynchronize(C->done_ev[k]));
+    std::vector<G1X> S(len[k] / sizeof(G1X));
+    std::memcpy(S.data(), hw + off[k], len[k]);
+    return horner(S, c);
+  };
+  G1X A, B1;
+  HostTask ta([&] {
+    A = g1_sum(0, pw.c);
+    g1_store(out->a_g1, xyzz_to_aff(A));
+    if (blind && blind->want) blind->sA = xyzz_mul(A, blind->s_std.v);
+  });
+  HostTask tb([&] {
+    B1 = g1_sum(1, pw.c);
+    g1_store(out->b_g1, xyzz_to_aff(B1));
+    if (blind && blind->want) blind->rB1 = xyzz_mul(B1, blind->r_std.v);
+  });
+  HostTask tl([&] { g1_store(out->l_g1, xyzz_to_aff(g1_sum(2, pw.c))); });
+  HostTask t2([&] {
+    G16_CHECK(hipEventSynchronize(C->done_ev[3]));
+    std::vector<G2X> S(Ww);
+    std::memcpy(S.data(), hw + off[3], len[3]);
+    g2_store(out->b_g2, xyzz_to_aff(horner(S, pw.c)));
+  });
+  std::exception_ptr err;
+  try {
+    g1_store(out->h_g1, xyzz_to_aff(g1_sum(4, ph.c)));
+    G16_CHECK(hipStreamSynchronize(s0));
+  } catch (...) {
+    err = std::current_exception();
+  }
+  for (HostTask* task : {&ta, &tb, &tl, &t2}) {
+    task->join();
+    if (!err && task->err) err = task->err;
+  }
+  if (err) {
+    (void)hipStreamSynchronize(s1);
+    (void)hipStreamSynchronize(s2);
+    std::rethrow_exception(err);
+  }
   for (int k = 0; k < 5; ++k)
     note_accumulate(C->dev, k == 3 ? 2 : 1, k == 4 ? totals[1] : totals[0],
                     ev_ms(C->mev[k][0], C->mev[k][1]));
-  auto g1w = [&](size_t off, size_t W) {
-    std::vector<G1X> v(W);
-    std::memcpy(v.data(), host.data() + off, 128 * W);
-    return v;
-  };
-  std::vector<G2X> b2(Ww);
-  std::memcpy(b2.data(), host.data() + 384 * Ww, 256 * Ww);
-  g1_store(out->a_g1, xyzz_to_aff(horner(g1w(0, Ww), pw.c)));
-  g1_store(out->b_g1, xyzz_to_aff(horner(g1w(128 * Ww, Ww), pw.c)));
-  g1_store(out->l_g1, xyzz_to_aff(horner(g1w(256 * Ww, Ww), pw.c)));
-  g2_store(out->b_g2, xyzz_to_aff(horner(b2, pw.c)));
-  g1_store(out->h_g1, xyzz_to_aff(horner(g1w(640 * Ww, Wh), ph.c)));
-  if (msm_us) *msm_us += (uint64_t)(ev_ms(C->ev[4], C->ev[5]) * 1000.0f);
+  static const bool timeline = std::getenv("G16HIP_TIMELINE") != nullptr;
+  if (timeline) {
+    // Milliseconds from the witness arriving on the device, per stream (the
+    // G16_CUDA_MEMORY_TRACE kind of switch, crates/cuda/src/backend.rs:89-93).
+    static const char* names[5] = {"A", "B1", "L", "B2", "H"};
+    std::fprintf(stderr, "[g16hip timeline]");
+    if (with_h)
+      std::fprintf(stderr, " ntt %.2f-%.2f h_done %.2f |", ev_ms(C->ev[4], C->ev[1]),
+                   ev_ms(C->ev[4], C->ev[2]), ev_ms(C->ev[4], C->ev[3]));
+    for (int k = 0; k < 5; ++k)
+      std::fprintf(stderr, " %s acc %.2f-%.2f merge %.2f reduce %.2f |", names[k],
+                   ev_ms(C->ev[4], C->mev[k][0]), ev_ms(C->ev[4], C->mev[k][1]),
+                   ev_ms(C->ev[4], C->mev[k][2]), ev_ms(C->ev[4], C->mev[k][3]));
+    std::fprintf(stderr, " end %.2f\n", ev_ms(C->ev[4], C->ev[5]));
+  }
+  if (t) {
+    if (with_h) {
+      t->gather_us += (uint64_t)(ev_ms(C->ev[0], C->ev[1]) * 1000.0f);
+      t->ntt_us += (uint64_t)(ev_ms(C->ev[1], C->ev[2]) * 1000.0f);
+      t->pointwise_us += (uint64_t)(ev_ms(C->ev[2], C->ev[3]) * 1000.0f);
+    }
+    // Stages 5-9 overlap stages 0-4: msm_us is the wall time from the witness arriving to the
+    // last MSM finishing, as the reference reports overlapped MSMs (crates/groth16/src/lib.rs:333-335).
+    t->msm_us += (uint64_t)(ev_ms(C->ev[4], C->ev[5]) * 1000.0f);
+  }
 }
 
 static void add_stage_timings(g16hip_circuit* C, g16hip_timings* t) {
@@ -463,41 +582,40 @@ static void load_witness(g16hip_circuit* C, const void* w, size_t n, bool on_dev
 static void h_and_msms_locked(g16hip_circuit* C, const void* w, size_t n, bool on_device,
                               g16hip_msm_out* out, g16hip_timings* t) {
   load_witness(C, w, n, on_device);
-  stages_0_4(C, C->hbuf.p);
-  uint64_t msm_us = 0;
-  stages_5_9(C, C->hbuf.p, out, &msm_us);
-  add_stage_timings(C, t);
-  if (t) t->msm_us += msm_us;
+  run_stages(C, true, C->hbuf.p, out, t);
 }
 
-// Stage 11 (crates/groth16/src/prove.rs:391-430), on the host.
+// Stage 11 (crates/groth16/src/prove.rs:391-430), on the host:
+//   pi_a = A + alpha1 + r delta1,  pi_b = B2 + beta2 + s delta2,
+//   pi_c = L + H + s pi_a + r (B1 + beta1 + s delta1) - r s delta1
+//        = L + H + s A + r B1 + (s alpha1 + r beta1 + r s delta1).
+// Everything that needs only the key and (r, s) is computed beside the GPU work; s A and r B1
+// as soon as those MSMs finish (BlindProducts). Group elements are canonical, so the proof is
+// the same point whatever order the terms are added in.
 struct BlinderTerms {
-  G1X d1r, d1s, d1rs;
-  G2X d2s;
-  Fr r_std, s_std;
+  G1X d1r, k1;  // r delta1;  s alpha1 + r beta1 + r s delta1
+  G2X d2s;      // s delta2
 };
 
-static BlinderTerms blinder_terms(const g16hip_circuit* C, const Fr& r, const Fr& s) {
+static BlinderTerms blinder_terms(const g16hip_circuit* C, const Fr& r_std, const Fr& s_std,
+                                  const Fr& rs_std) {
   BlinderTerms b;
-  b.r_std = to_std(r);
-  b.s_std = to_std(s);
-  Fr rs = to_std(mul(r, s));
-  b.d1r = xyzz_mul_plain(C->delta_g1, b.r_std.v);
-  b.d1s = xyzz_mul_plain(C->delta_g1, b.s_std.v);
-  b.d1rs = xyzz_mul_plain(C->delta_g1, rs.v);
-  b.d2s = xyzz_mul_plain(C->delta_g2, b.s_std.v);
+  b.d1r = xyzz_mul_plain(C->delta_g1, r_std.v);
+  b.k1 = xyzz_add(xyzz_add(xyzz_mul_plain(C->alpha_g1, s_std.v),
+                           xyzz_mul_plain(C->beta_g1, r_std.v)),
+                  xyzz_mul_plain(C->delta_g1, rs_std.v));
+  b.d2s = xyzz_mul_plain(C->delta_g2, s_std.v);
   return b;
 }
 
 static void assemble(const g16hip_circuit* C, const g16hip_msm_out& m, const BlinderTerms& b,
-                     g16hip_proof* proof) {
+                     const BlindProducts& bp, g16hip_proof* proof) {
   G1X pi_a = xyzz_add(xyzz_add_aff(xyzz_from_aff(g1_load(m.a_g1)), C->alpha_g1), b.d1r);
   G2X pi_b = xyzz_add(xyzz_add_aff(xyzz_from_aff(g2_load(m.b_g2)), C->beta_g2), b.d2s);
-  G1X pib1 = xyzz_add(xyzz_add_aff(xyzz_from_aff(g1_load(m.b_g1)), C->beta_g1), b.d1s);
   G1X pi_c = xyzz_add_aff(xyzz_from_aff(g1_load(m.l_g1)), g1_load(m.h_g1));
-  pi_c = xyzz_add(pi_c, xyzz_mul(pi_a, b.s_std.v));
-  pi_c = xyzz_add(pi_c, xyzz_mul(pib1, b.r_std.v));
-  pi_c = xyzz_add(pi_c, xyzz_neg(b.d1rs));
+  pi_c = xyzz_add(pi_c, bp.sA);
+  pi_c = xyzz_add(pi_c, bp.rB1);
+  pi_c = xyzz_add(pi_c, b.k1);
   g1_store(proof->a, xyzz_to_aff(pi_a));
   g2_store(proof->b, xyzz_to_aff(pi_b));
   g1_store(proof->c, xyzz_to_aff(pi_c));
@@ -520,21 +638,27 @@ static g16hip_status prove_impl(g16hip_circuit* C, const void* w, size_t n, bool
     if (!is_one(w0)) throw StatusError{G16HIP_CONSTANT_WIRE, "w[0] is the constant-one wire and must be 1"};
     Fr r = fr_load(r8), s = fr_load(s8);
     if (is_zero(r) || is_zero(s)) throw StatusError{G16HIP_ZERO_BLINDER, "a blinder is zero"};
+    BlindProducts bp;
+    bp.want = true;
+    bp.r_std = to_std(r);
+    bp.s_std = to_std(s);
+    Fr rs_std = to_std(mul(r, s));
     BlinderTerms terms;
-    Timer tt;
-    std::thread th([&] { terms = blinder_terms(C, r, s); });
+    HostTask th([&] { terms = blinder_terms(C, bp.r_std, bp.s_std, rs_std); });
     g16hip_msm_out m;
     try {
       std::lock_guard<std::mutex> lk(C->mu);
       G16_CHECK(hipSetDevice(C->dev->ordinal));
-      h_and_msms_locked(C, w, n, on_device, &m, t);
+      load_witness(C, w, n, on_device);
+      run_stages(C, true, C->hbuf.p, &m, t, &bp);
     } catch (...) {
       th.join();
       throw;
     }
     th.join();
+    if (th.err) std::rethrow_exception(th.err);
     Timer ta;
-    assemble(C, m, terms, proof);
+    assemble(C, m, terms, bp, proof);
     if (t) t->assemble_us += ta.us();
     return G16HIP_OK;
   });
@@ -795,16 +919,23 @@ g16hip_status g16hip_prepare(g16hip_device* d, const g16hip_key* key, g16hip_cir
     auto C = std::make_unique<g16hip_circuit>();
     C->dev = d;
     G16_CHECK(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
+    for (auto& st : C->side) G16_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     for (auto& e : C->ev) G16_CHECK(hipEventCreate(&e));
+    for (auto& e : C->sync_ev) G16_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto& e : C->done_ev) G16_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto& row : C->mev)
       for (auto& e : row) G16_CHECK(hipEventCreate(&e));
     try {
       prepare_circuit(C.get(), key);
     } catch (...) {
-      (void)hipStreamSynchronize(C->stream);
+      (void)hipDeviceSynchronize();
       for (auto& e : C->ev) (void)hipEventDestroy(e);
+      for (auto& e : C->sync_ev) (void)hipEventDestroy(e);
+      for (auto& e : C->done_ev) (void)hipEventDestroy(e);
+      if (C->host_wout) (void)hipHostFree(C->host_wout);
       for (auto& row : C->mev)
         for (auto& e : row) (void)hipEventDestroy(e);
+      for (auto& st : C->side) (void)hipStreamDestroy(st);
       (void)hipStreamDestroy(C->stream);
       throw;
     }
@@ -817,9 +948,14 @@ void g16hip_circuit_release(g16hip_circuit* C) {
   if (!C) return;
   (void)hipSetDevice(C->dev->ordinal);
   (void)hipStreamSynchronize(C->stream);
+  for (auto& st : C->side) (void)hipStreamSynchronize(st);
   for (auto& e : C->ev) (void)hipEventDestroy(e);
+  for (auto& e : C->sync_ev) (void)hipEventDestroy(e);
+  for (auto& e : C->done_ev) (void)hipEventDestroy(e);
+  if (C->host_wout) (void)hipHostFree(C->host_wout);
   for (auto& row : C->mev)
     for (auto& e : row) (void)hipEventDestroy(e);
+  for (auto& st : C->side) (void)hipStreamDestroy(st);
   (void)hipStreamDestroy(C->stream);
   delete C;
 }
@@ -881,9 +1017,7 @@ g16hip_status g16hip_msms(g16hip_circuit* C, const uint8_t* w, size_t n, const g
     std::lock_guard<std::mutex> lk(C->mu);
     G16_CHECK(hipSetDevice(C->dev->ordinal));
     load_witness(C, w, n, false);
-    uint64_t us = 0;
-    stages_5_9(C, h->buf.p, out, &us);
-    if (t) t->msm_us += us;
+    run_stages(C, false, h->buf.p, out, t);
     return G16HIP_OK;
   });
 }
