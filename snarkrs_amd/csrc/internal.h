@@ -17,6 +17,8 @@ struct ArgError {
 };
 
 void hip_check(hipError_t e, const char* what);
+// The message g16hip_last_error() returns on this thread.
+void set_last_error(const std::string& msg);
 #define G16_CHECK(call) ::g16::hip_check((call), #call)
 #define G16_LAUNCH_CHECK(what) ::g16::hip_check(hipGetLastError(), what)
 

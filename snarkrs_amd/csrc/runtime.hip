@@ -34,6 +34,10 @@ void hip_check(hipError_t e, const char* what) {
 // ---------------------------------------------------------------------------------------
 static thread_local std::string t_last_error;
 
+namespace g16 {
+void set_last_error(const std::string& msg) { t_last_error = msg; }
+}  // namespace g16
+
 static g16hip_status fail(g16hip_status st, const std::string& msg) {
   t_last_error = msg;
   return st;
