@@ -13,6 +13,8 @@ restated in C++; the Rust original cannot be built here) proving the same key an
 the host cores, and its proof is compared with the GPU's bit for bit.
 
     python bench.py [--steps K] [--warmup W] [--workload large|plain] [--no-cpu] [--no-msm]
+    python bench.py --mode batch   # 64 proofs over 4 circuits of 2^14..2^20 constraints
+    python bench.py --mode serve   # Poisson serving over 8 circuits of 2^16..2^22
 """
 from __future__ import annotations
 
@@ -28,9 +30,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Groth16 proofs/sec + p50/p99 latency at 2^20 constraints; G1 MSM points/sec"
-# MI355X VALU 32x32->64 multiply-add peak: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz at quarter
-# rate (v_mad_u64_u32).
-PEAK_MUL32_PER_S = 256 * 4 * 16 * 2.4e9 / 4
+# MI355X VALU 32x32->64 multiply-add peak: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz, one
+# v_mad_u64_u32 per lane per cycle (a dependent-chain probe measures 2.99e13/s, 76% of it).
+PEAK_MUL32_PER_S = 256 * 4 * 16 * 2.4e9
 # Algorithmic work per bucket addition (mixed add, madd-2008-s): 10 Fq products (G1),
 # 28 (G2: 3 Fq products per Fq2 product); one Fq product = 136 32x32 partial products (CIOS).
 MUL32_PER_MADD = {1: 10 * 136, 2: 28 * 136}
@@ -71,6 +73,219 @@ def cpu_baseline(key, wit, gpu_proof, threads):
     }
 
 
+def _mixed_workloads(count: int, lo: float, hi: float, seed: int, base_seed: int):
+    """`count` circuits whose log2 constraint count is drawn uniformly in [lo, hi] (seeded);
+    40% boolean witness, 3 nonzeros per A/B row, about 9% of the wires private inputs."""
+    from snarkrs_amd.synth import Workload
+    from snarkrs_amd.wire import SplitMix64
+
+    rng = SplitMix64(seed)
+    out = []
+    for i in range(count):
+        x = lo + (hi - lo) * (rng.next_u64() / 2.0**64)
+        nc = int(2.0**x)
+        out.append(Workload(n_constraints=nc, n_public=2, n_private_in=max(8, nc // 11),
+                            bool_permille=400, input_bool_permille=400, seed=base_seed + i,
+                            setup_seed=base_seed + 100 + i, witness_seed=base_seed + 200 + i))
+    return out
+
+
+def _build_circuits(dev, workloads, witnesses_per_circuit: int):
+    from snarkrs_amd.synth import Synth
+
+    circs = []
+    for i, wl in enumerate(workloads):
+        t = time.time()
+        syn = Synth(wl)
+        key, wit = syn.key(dev)
+        wits = [wit] + [syn.witness(wl.witness_seed + 1000 * (k + 1))
+                        for k in range(witnesses_per_circuit - 1)]
+        syn.close()
+        circ = dev.prepare(key)
+        circs.append((wl, circ, wits))
+        log(f"circuit {i}: {wl.n_constraints} constraints, domain {key.domain_size}, "
+            f"{key.n_vars} wires ({time.time() - t:.1f} s)")
+    return circs
+
+
+def _blinders(k: int):
+    from snarkrs_amd.wire import fr_array
+
+    return fr_array([R_BLIND + 7 * k]), fr_array([S_BLIND + 11 * k])
+
+
+def run_batch(args) -> int:
+    """64 proofs with independent witnesses over 4 resident circuits of 2^14..2^20
+    constraints, proved as one batch (each circuit's slots replay their captured proof
+    graphs) and, for comparison, one at a time through the plain path."""
+    from snarkrs_amd import native
+    from snarkrs_amd.native import Device
+
+    dev = Device(0)
+    wls = _mixed_workloads(4, 14.0, 20.0, seed=3, base_seed=500)
+    per = args.batch_proofs // len(wls)
+    circs = _build_circuits(dev, wls, per)
+    for _, circ, _ in circs:
+        circ.add_slots(args.slots - 1)
+    items = []
+    for k in range(args.batch_proofs):
+        wl, circ, wits = circs[k % len(circs)]
+        r, s = _blinders(k)
+        items.append((circ, wits[(k // len(circs)) % len(wits)], r, s))
+    # warmup: captures every slot's graph and runs the plain path once per circuit
+    native.prove_batch(items[: len(circs) * args.slots])
+    for it in items[: len(circs)]:
+        it[0].prove_with_blinders(it[1], it[2], it[3])
+    t0 = time.perf_counter()
+    got = native.prove_batch(items)
+    t_batch = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    plain = [c.prove_with_blinders(w, r, s) for c, w, r, s in items]
+    t_plain = time.perf_counter() - t0
+    same = all((a == b).all() for a, b in zip(got, plain))
+    log(f"batch: {len(items)} proofs in {t_batch * 1e3:.1f} ms -> {len(items) / t_batch:.1f} "
+        f"proofs/s; plain path {t_plain * 1e3:.1f} ms -> {len(items) / t_plain:.1f} proofs/s; "
+        f"bit-identical {same}")
+    line = {
+        "metric": "Groth16 batch proofs/s (64 proofs, 4 resident circuits of 2^14..2^20 constraints)",
+        "value": len(items) / t_batch,
+        "unit": "proofs/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": "batch of independent proofs over resident circuits",
+            "circuits": [{"constraints": wl.n_constraints, "domain": c.domain_size}
+                         for wl, c, _ in circs],
+            "proofs": len(items),
+            "slots_per_circuit": args.slots,
+            "witness": "host memory (PCIe copy inside the timed region, both paths)",
+            "gpu_max_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
+            "graph_replay": os.environ.get("G16HIP_NO_GRAPH") is None,
+        },
+        "batch_ms": t_batch * 1e3,
+        "plain_path": {"value": len(items) / t_plain, "ms": t_plain * 1e3},
+        "speedup_vs_plain": t_plain / t_batch,
+        "bit_identical_to_plain": same,
+    }
+    print(json.dumps(line), flush=True)
+    return 0 if same else 1
+
+
+def run_serve(args) -> int:
+    """Poisson arrivals over 8 resident circuits of 2^16..2^22 constraints: the serving loop
+    (continuous batching, slots admitted by free device memory) against the plain path (one
+    proof at a time, first come first served) on the same arrival schedule."""
+    import math
+    import threading
+
+    from snarkrs_amd import native
+    from snarkrs_amd.native import Device
+    from snarkrs_amd.wire import SplitMix64
+
+    dev = Device(0)
+    wls = _mixed_workloads(8, 16.0, 22.0, seed=4, base_seed=700)
+    circs = _build_circuits(dev, wls, 2)
+    rng = SplitMix64(41)
+    reqs = []
+    for k in range(args.requests):
+        ci = rng.next_u64() % len(circs)
+        wl, circ, wits = circs[ci]
+        r, s = _blinders(k)
+        reqs.append((ci, circ, wits[k % len(wits)], r, s))
+    # service time of each circuit on the plain path (after one warm proof each)
+    svc = []
+    for wl, circ, wits in circs:
+        r, s = _blinders(0)
+        circ.prove_with_blinders(wits[0], r, s)
+        t0 = time.perf_counter()
+        circ.prove_with_blinders(wits[0], r, s)
+        svc.append(time.perf_counter() - t0)
+    mean_svc = sum(svc[ci] for ci, *_ in reqs) / len(reqs)
+    rate = args.load / mean_svc
+    arrivals, t = [], 0.0
+    for _ in reqs:
+        u = (rng.next_u64() + 1) / 2.0**64
+        t += -math.log(u) / rate
+        arrivals.append(t)
+    log(f"serve: {len(reqs)} requests, mean plain service {mean_svc * 1e3:.1f} ms, Poisson rate "
+        f"{rate:.1f}/s (load {args.load} of the plain path)")
+
+    def plain_path():
+        lat, out = [], []
+        t0 = time.perf_counter()
+        for (ci, circ, w, r, s), a in zip(reqs, arrivals):
+            now = time.perf_counter() - t0
+            if now < a:
+                time.sleep(a - now)
+            out.append(circ.prove_with_blinders(w, r, s))
+            lat.append(time.perf_counter() - t0 - a)
+        return lat, out, time.perf_counter() - t0
+
+    def served():
+        srv = native.Server(dev, reserve_bytes=args.reserve_gib << 30,
+                            max_slots_per_circuit=args.slots)
+        tickets = [None] * len(reqs)
+        t0 = time.perf_counter()
+
+        def submitter():
+            for k, ((ci, circ, w, r, s), a) in enumerate(zip(reqs, arrivals)):
+                now = time.perf_counter() - t0
+                if now < a:
+                    time.sleep(a - now)
+                tickets[k] = srv.submit(circ, w, r, s)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat, out = [], []
+        for k in range(len(reqs)):
+            while tickets[k] is None:
+                time.sleep(0.0005)
+            p, ms = srv.wait(tickets[k])
+            out.append(p)
+            lat.append(ms / 1e3)
+        th.join()
+        wall = time.perf_counter() - t0
+        srv.close()
+        return lat, out, wall
+
+    lat_p, out_p, wall_p = plain_path()
+    lat_s, out_s, wall_s = served()
+    same = all((a == b).all() for a, b in zip(out_p, out_s))
+    res = lambda lat, wall: {"proofs_per_s": len(reqs) / wall,
+                             "p50_ms": percentile(lat, 50) * 1e3,
+                             "p99_ms": percentile(lat, 99) * 1e3}
+    sv, pl = res(lat_s, wall_s), res(lat_p, wall_p)
+    log(f"served: {sv}; plain path: {pl}; bit-identical {same}")
+    line = {
+        "metric": "Groth16 serving proofs/s + p50/p99 latency (Poisson arrivals, 8 resident "
+                  "circuits of 2^16..2^22 constraints)",
+        "value": sv["proofs_per_s"],
+        "unit": "proofs/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": "Poisson arrivals, circuit drawn uniformly per request",
+            "circuits": [{"constraints": wl.n_constraints, "domain": c.domain_size,
+                          "plain_service_ms": t * 1e3} for (wl, c, _), t in zip(circs, svc)],
+            "requests": len(reqs),
+            "rate_per_s": rate,
+            "load_of_plain_path": args.load,
+            "max_slots_per_circuit": args.slots,
+            "gpu_max_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
+            "graph_replay": os.environ.get("G16HIP_NO_GRAPH") is None,
+        },
+        "served": sv,
+        "plain_path": pl,
+        "bit_identical_to_plain": same,
+    }
+    print(json.dumps(line), flush=True)
+    return 0 if same else 1
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,12 +296,29 @@ def main() -> int:
     ap.add_argument("--msm-reps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-msm", action="store_true")
+    ap.add_argument("--mode", choices=["prove", "batch", "serve"], default="prove",
+                    help="prove: the single-proof headline (default); batch: 64 proofs over 4 "
+                         "circuits; serve: Poisson serving over 8 circuits")
+    ap.add_argument("--batch-proofs", type=int, default=64)
+    ap.add_argument("--slots", type=int, default=4, help="proof slots per circuit (batch, serve)")
+    ap.add_argument("--requests", type=int, default=200)
+    ap.add_argument("--load", type=float, default=0.8)
+    ap.add_argument("--reserve-gib", type=int, default=16)
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="GPU_MAX_HW_QUEUES for this process (batch, serve; 0 = leave as set)")
     args = ap.parse_args()
+    if args.hw_queues:
+        # HIP reads it when the runtime starts, which has not happened yet in this process.
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(args.hw_queues, 32))
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one, and this prover back end "
               "runs one proof per device (multi-GPU sharding is described in DESIGN.md, not "
               "built). Run with --gpus 1.", file=sys.stderr)
         return 2
+    if args.mode == "batch":
+        return run_batch(args)
+    if args.mode == "serve":
+        return run_serve(args)
 
     from snarkrs_amd.devmem import DeviceArray, synchronize
     from snarkrs_amd.native import Device, Timings

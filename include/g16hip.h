@@ -160,6 +160,46 @@ g16hip_status g16hip_prove_with_blinders_device(g16hip_circuit* c, const void* d
                                                 const uint8_t* s, g16hip_proof* proof,
                                                 g16hip_timings* t);
 
+/* ---- Batched proving and serving (no counterpart in the reference, whose provers run one
+ * proof per call; these are for callers that keep many proofs in flight on one GPU). ---- */
+
+/* Adds up to `count` proof slots to `c` (each: streams + per-proof scratch, so one more proof
+ * can be in flight), stopping when the device would keep less than `reserve_bytes` free:
+ * admission by free device memory. *total_slots receives the slot count. A circuit starts
+ * with one slot. */
+g16hip_status g16hip_circuit_add_slots(g16hip_circuit* c, uint32_t count, size_t reserve_bytes,
+                                       uint32_t* total_slots);
+
+typedef struct {
+  g16hip_circuit* circuit;
+  const uint8_t* witness; /* Fr[n_witness] Montgomery, host memory */
+  size_t n_witness;
+  const uint8_t* r; /* Fr Montgomery */
+  const uint8_t* s;
+} g16hip_batch_item;
+
+/* Proves every item (any mix of circuits) with all slots of each circuit busy at once; each
+ * slot replays its proof pipeline, captured once as a hipGraph. proofs[i] is bit-identical to
+ * g16hip_prove_with_blinders on item i. statuses (optional) receives each item's status; the
+ * return value is the first failure's. */
+g16hip_status g16hip_prove_batch(const g16hip_batch_item* items, size_t n, g16hip_proof* proofs,
+                                 g16hip_status* statuses);
+
+/* A serving loop with continuous batching: submit returns at once; a request that finds every
+ * worker of its circuit busy gets a new slot and worker when the device keeps `reserve_bytes`
+ * free after it (at most `max_slots_per_circuit`), else it queues until a slot frees up.
+ * wait blocks for a ticket's proof and reports its latency (submit to completion). */
+typedef struct g16hip_server g16hip_server;
+g16hip_status g16hip_server_open(g16hip_device* dev, size_t reserve_bytes,
+                                 uint32_t max_slots_per_circuit, g16hip_server** out);
+g16hip_status g16hip_server_submit(g16hip_server* srv, g16hip_circuit* c, const uint8_t* witness,
+                                   size_t n_witness, const uint8_t* r, const uint8_t* s,
+                                   uint64_t* ticket);
+g16hip_status g16hip_server_wait(g16hip_server* srv, uint64_t ticket, g16hip_proof* proof,
+                                 double* latency_ms);
+/* Finishes the queued requests, then stops the workers. */
+void g16hip_server_close(g16hip_server* srv);
+
 /* MsmBackend::try_msm_g1 / try_msm_g2 (crates/msm/src/lib.rs:61-81): bases affine, scalars
  * Fr Montgomery (as `&[Fr]` is), host memory; result affine. Empty input is infinity. */
 g16hip_status g16hip_msm_g1(g16hip_device* dev, const uint8_t* bases, const uint8_t* scalars,

@@ -26,8 +26,9 @@ typedef struct {
   uint32_t bool_permille;      /* share of constraints whose output wire is a bit */
   uint32_t input_bool_permille;/* share of input wires that are bits */
   uint32_t terms;              /* nonzeros per A and B row */
-  uint64_t seed;               /* circuit + witness */
+  uint64_t seed;               /* circuit structure */
   uint64_t setup_seed;         /* toxic waste */
+  uint64_t witness_seed;       /* witness values (any seed satisfies the circuit) */
 } g16synth_params;
 
 typedef struct g16synth g16synth;
@@ -43,6 +44,11 @@ void g16synth_dims(const g16synth* s, uint32_t* out);
  * 13 IC scalars (Fr[n_public+1]); 14 L scalars (Fr[n_vars-n_public-1]); 15 H scalars
  * (Fr[domain]). Returns 0, or -1 for an unknown `what`. */
 int g16synth_get(const g16synth* s, int what, void* dst);
+
+/* Another satisfying witness of the same circuit, for witness seed `witness_seed`
+ * (Fr[n_vars], Montgomery): the keys stay the same, so batches of independent proofs can share
+ * one prepared circuit. Returns 0, or -1 for a null argument. */
+int g16synth_witness(const g16synth* s, uint64_t witness_seed, uint8_t* out);
 
 /* Seeded scalar vectors (Fr Montgomery) for the standalone MSM workloads: each scalar is, with
  * probability zero_one_permille/1000, 0 or 1 (equally likely), else uniform in [0, r). */

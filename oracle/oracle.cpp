@@ -505,13 +505,16 @@ typedef struct {
   uint32_t bool_permille;   // share of constraints whose output wire is a bit
   uint32_t input_bool_permille;
   uint32_t terms;           // nonzeros per A and B row
-  uint64_t seed;            // circuit + witness
+  uint64_t seed;            // circuit structure
   uint64_t setup_seed;      // toxic waste
+  uint64_t witness_seed;    // witness values
 } orc_synth_params;
 }
 
 struct Synth {
   uint32_t n_vars = 0, n_public = 0, domain = 0, n_constraints = 0;
+  uint32_t n_in = 0, input_bool_permille = 0;
+  std::vector<uint8_t> bit_row;
   std::vector<uint32_t> rp[3], sig[3];
   std::vector<Fr> val[3];
   std::vector<Fr> w;
@@ -525,6 +528,34 @@ static uint32_t floor_log2(uint64_t x) {
   return l;
 }
 
+// Witness for seed `ws`: inputs (bits with probability input_bool_permille/1000, else
+// uniform), then per constraint a drawn bit (bit rows) or the product of its A and B rows.
+static void synth_witness(const Synth* S, uint64_t ws, std::vector<Fr>& w) {
+  SplitMix64 rng{ws};
+  w.assign(S->n_vars, Fr::zero());
+  w[0] = Fr::one();
+  for (uint32_t i = 1; i <= S->n_in; ++i) {
+    if (rng.next_u64() % 1000 < S->input_bool_permille)
+      w[i] = (rng.next_u64() & 1) ? Fr::one() : Fr::zero();
+    else
+      w[i] = rng.next_fr();
+  }
+  for (uint32_t j = 0; j < S->n_constraints; ++j) {
+    uint32_t cur = 1 + S->n_in + j;
+    if (S->bit_row[j]) {
+      w[cur] = (rng.next_u64() & 1) ? Fr::one() : Fr::zero();
+      continue;
+    }
+    Fr ab[2];
+    for (int m = 0; m < 2; ++m) {
+      Fr acc = Fr::zero();
+      for (uint32_t e = S->rp[m][j]; e < S->rp[m][j + 1]; ++e) acc += S->val[m][e] * w[S->sig[m][e]];
+      ab[m] = acc;
+    }
+    w[cur] = ab[0] * ab[1];
+  }
+}
+
 static Synth* synth_build(const orc_synth_params* p) {
   Synth* S = new Synth();
   SplitMix64 rng{p->seed};
@@ -532,48 +563,45 @@ static Synth* synth_build(const orc_synth_params* p) {
   uint32_t n_in = p->n_public + p->n_private_in;
   S->n_constraints = N;
   S->n_public = p->n_public;
+  S->n_in = n_in;
+  S->input_bool_permille = p->input_bool_permille;
   S->n_vars = 1 + n_in + N;
   S->domain = 1u << (floor_log2((uint64_t)N + p->n_public) + 1);
   uint32_t n = S->domain;
-  std::vector<Fr>& w = S->w;
-  w.resize(S->n_vars);
-  w[0] = Fr::one();
-  for (uint32_t i = 1; i <= n_in; ++i) {
-    if (rng.next_u64() % 1000 < p->input_bool_permille)
-      w[i] = (rng.next_u64() & 1) ? Fr::one() : Fr::zero();
-    else
-      w[i] = rng.next_fr();
-  }
   for (int m = 0; m < 3; ++m) S->rp[m].assign(n + 1, 0);
+  S->bit_row.assign(N, 0);
   uint32_t cur = 1 + n_in;
   for (uint32_t j = 0; j < N; ++j) {
-    Fr ab[2];
+    // Bit row: b * b = b on the fresh wire, A and B padded to T terms with k x - k x pairs
+    // (and a zero term when T - 1 is odd). Product row: T seeded terms each in A and B.
+    bool bit = rng.next_u64() % 1000 < p->bool_permille;
+    S->bit_row[j] = bit;
     for (int m = 0; m < 2; ++m) {
-      Fr acc = Fr::zero();
-      for (uint32_t t = 0; t < T; ++t) {
-        uint32_t s = (uint32_t)(rng.next_u64() % cur);
-        Fr cf = rng.next_fr();
-        S->sig[m].push_back(s);
-        S->val[m].push_back(cf);
-        acc += cf * w[s];
+      auto put = [&](uint32_t sg, const Fr& v) {
+        S->sig[m].push_back(sg);
+        S->val[m].push_back(v);
+      };
+      if (bit) {
+        put(cur, Fr::one());
+        uint32_t pads = T - 1;
+        while (pads >= 2) {
+          uint32_t x = (uint32_t)(rng.next_u64() % cur);
+          Fr k = rng.next_fr();
+          put(x, k);
+          put(x, Fr::zero() - k);
+          pads -= 2;
+        }
+        if (pads) put((uint32_t)(rng.next_u64() % cur), Fr::zero());
+      } else {
+        for (uint32_t t = 0; t < T; ++t) {
+          uint32_t sg = (uint32_t)(rng.next_u64() % cur);
+          put(sg, rng.next_fr());
+        }
       }
-      ab[m] = acc;
       S->rp[m][j + 1] = (uint32_t)S->sig[m].size();
     }
-    Fr prod = ab[0] * ab[1];
-    bool is_bool = rng.next_u64() % 1000 < p->bool_permille;
-    if (!is_bool) {
-      w[cur] = prod;
-      S->sig[2].push_back(cur);
-      S->val[2].push_back(Fr::one());
-    } else {
-      Fr bit = (rng.next_u64() & 1) ? Fr::one() : Fr::zero();
-      w[cur] = bit;
-      S->sig[2].push_back(cur);
-      S->val[2].push_back(Fr::one());
-      S->sig[2].push_back(0);
-      S->val[2].push_back(prod - bit);
-    }
+    S->sig[2].push_back(cur);
+    S->val[2].push_back(Fr::one());
     S->rp[2][j + 1] = (uint32_t)S->sig[2].size();
     ++cur;
   }
@@ -586,6 +614,7 @@ static Synth* synth_build(const orc_synth_params* p) {
   for (int m = 0; m < 3; ++m)
     for (uint32_t c = 1; c <= n; ++c)
       if (S->rp[m][c] < S->rp[m][c - 1]) S->rp[m][c] = S->rp[m][c - 1];
+  synth_witness(S, p->witness_seed, S->w);
 
   // Toxic waste.
   SplitMix64 trng{p->setup_seed};
@@ -847,6 +876,12 @@ int orc_prove(const orc_key* k, const uint8_t* wit, const uint8_t* r_m, const ui
 // Synthetic circuit handle.
 void* orc_synth_new(const orc_synth_params* p) { return synth_build(p); }
 void orc_synth_free(void* h) { delete static_cast<Synth*>(h); }
+
+void orc_synth_witness(void* h, uint64_t ws, uint8_t* out) {
+  std::vector<Fr> w;
+  synth_witness(static_cast<Synth*>(h), ws, w);
+  std::memcpy(out, w.data(), 32 * w.size());
+}
 
 // dims: n_vars, n_public, domain, n_constraints, nnz A, nnz B, nnz C
 void orc_synth_dims(void* h, uint32_t* d) {

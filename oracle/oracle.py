@@ -38,6 +38,7 @@ class SynthParams(ctypes.Structure):
         ("terms", ctypes.c_uint32),
         ("seed", ctypes.c_uint64),
         ("setup_seed", ctypes.c_uint64),
+        ("witness_seed", ctypes.c_uint64),
     ]
 
 
@@ -58,6 +59,7 @@ class Oracle:
         lib.orc_synth_new.restype = ctypes.c_void_p
         lib.orc_synth_new.argtypes = [ctypes.POINTER(SynthParams)]
         lib.orc_synth_free.argtypes = [ctypes.c_void_p]
+        lib.orc_synth_witness.argtypes = [ctypes.c_void_p, ctypes.c_uint64, _u8p]
         lib.orc_synth_dims.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]
         lib.orc_synth_get.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
         lib.orc_shadow.argtypes = [ctypes.c_void_p, _u8p, _u8p, _u8p]
@@ -201,10 +203,10 @@ class Synth:
 
     def __init__(self, orc: Oracle, n_constraints: int, n_public: int = 2, n_private_in: int = 8,
                  bool_permille: int = 0, input_bool_permille: int = 0, terms: int = 3,
-                 seed: int = 1, setup_seed: int = 2):
+                 seed: int = 1, setup_seed: int = 2, witness_seed: int = 3):
         self.orc = orc
         self.params = SynthParams(n_constraints, n_public, n_private_in, bool_permille,
-                                  input_bool_permille, terms, seed, setup_seed)
+                                  input_bool_permille, terms, seed, setup_seed, witness_seed)
         self.h = orc.lib.orc_synth_new(ctypes.byref(self.params))
         d = (ctypes.c_uint32 * 7)()
         orc.lib.orc_synth_dims(self.h, d)
@@ -260,6 +262,12 @@ class Synth:
             b_g2_query=g2(a["v"]), l_query=g1(a["l_s"]), h_query=g1(a["h_s"]),
         )
         return key, a["witness"]
+
+    def witness(self, witness_seed: int) -> np.ndarray:
+        """Another satisfying witness of the same circuit."""
+        out = np.zeros((self.n_vars, 32), dtype=np.uint8)
+        self.orc.lib.orc_synth_witness(self.h, witness_seed, _p(out))
+        return out
 
     def shadow(self, r: int, s: int) -> list[int]:
         """Closed-form scalars: msm A, B, L, H, pi_a, pi_b, pi_c, verify lhs, verify rhs."""

@@ -60,14 +60,15 @@ struct MsmPlan {
   uint32_t nb = 0;        // buckets per window, 2^(c-1)
   uint32_t seg = 0;       // entries per accumulation segment
   size_t max_entries = 0; // n * windows
+  uint32_t chunks = 0;    // scalar chunks of the counting sort
   // device scratch (owned by the caller)
-  uint32_t* count = nullptr;    // [windows*nb + 1]
-  uint32_t* offset = nullptr;   // [windows*nb + 1]
-  uint32_t* cursor = nullptr;   // [windows*nb]
+  uint32_t* count = nullptr;    // [windows*nb + 1] entries per bucket
+  uint32_t* offset = nullptr;   // [windows*nb + 1] first entry of each bucket; [K] = total
   uint32_t* block_sums = nullptr;
   uint32_t* keys = nullptr;     // [max_entries]
   uint32_t* vals = nullptr;     // [max_entries]  (index << 1 | sign)
-  uint32_t* scalars_std = nullptr;  // [n * 8]
+  uint32_t* digits = nullptr;   // [windows * n] encoded signed digits, window-major
+  uint32_t* chunk_count = nullptr;  // [windows][chunks][nb] per-chunk counts, then prefixes
   uint32_t* total = nullptr;    // [4] entry count
 };
 size_t msm_sort_scratch_bytes(uint32_t n, int c);
@@ -76,11 +77,17 @@ void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c);
 void msm_sort(MsmPlan& p, const void* scalars_mont, hipStream_t s);
 size_t msm_accum_scratch_bytes(const MsmPlan& p, int group);
 // Bucket accumulation + window reduction for one base vector over the sorted entries;
-// writes `windows` XYZZ window sums to `window_out` (device memory).
+// writes msm_out_points(p) XYZZ points to `window_out` (device memory): `rows` per window,
+// row 0 = sum_g U_g and row k + 1 = sum of T_g over the groups g with bit k set, where the
+// window's buckets fall in groups of 2^lg, T_g is a group's sum and U_g its sum weighted
+// 1, 2, ..., 2^lg. The window sum is then S_w = row 0 + sum_k 2^(k + lg) row k+1
+// (msm_combine on the host).
 // If `ev` is given, ev[0..3] are recorded before the accumulation kernel, after it, after the
 // merges and after the window reduction.
 void msm_accumulate(const MsmPlan& p, int group, const void* bases, void* scratch,
                     void* window_out, hipStream_t s, hipEvent_t* ev = nullptr);
+void msm_reduce_shape(const MsmPlan& p, int* rows, int* lg);
+size_t msm_out_points(const MsmPlan& p);
 int msm_default_c(uint32_t n);
 
 }  // namespace g16

@@ -3,17 +3,19 @@
 // Pipeline per scalar vector (shared by every base vector multiplied by the same scalars,
 // so the witness MSMs A, B1, B2 and L sort once):
 //   1. digits:  Montgomery -> standard scalar, carry-free signed digits per window
-//               (crates/msm/src/lib.rs:275-331), histogram of (window, |digit|) keys;
-//   2. sort:    exclusive scan of the histogram and a scatter of (key, index|sign) entries
-//               into key order (a one-pass counting sort with an 18-bit digit: the on-GPU
-//               radix sort of the digits into buckets);
+//               (crates/msm/src/lib.rs:275-331), stored window-major;
+//   2. sort:    a counting sort of (window, |digit|) keys without global atomics: each
+//               workgroup counts one chunk of one window in LDS, a column scan over the
+//               chunks and a scan over the buckets place every (bucket, chunk) pair, and the
+//               scatter ranks a chunk's entries with LDS atomics;
 //   3. accumulate: one thread per fixed-length segment of the sorted entries adds its runs
 //               into XYZZ accumulators (madd-2008-s); a run that fills its bucket inside
 //               the segment is stored as the bucket, the partial runs at a segment's ends go
 //               to side arrays and a merge pass folds them (heavy buckets, e.g. every 1
-//               scalar in bucket 0 of window 0, get a whole workgroup and an LDS tree);
-//   4. reduce:  per window S_w = sum_j (j+1) B_j: groups of buckets by running sums, then a
-//               per-window workgroup tree; the host finishes sum_w 2^(cw) S_w.
+//               scalar in bucket 0 of window 0, are cut into tasks summed by LDS trees);
+//   4. reduce:  per window S_w = sum_j (j+1) B_j, as group sums and weighted group sums of
+//               8 buckets, then one LDS tree per (window, bit of the group index); the host
+//               finishes with one Horner pass over every window and bit.
 // Zero scalars produce no entries and a scalar equal to 1 produces one (window 0, bucket 0),
 // which is the reference prescan's effect (crates/msm/src/lib.rs:534-690) without a pass of
 // its own; bases at infinity are skipped where they are loaded.
@@ -51,81 +53,109 @@ __device__ __forceinline__ int signed_digit(const uint32_t s[8], int i, int c) {
   return d + carry;
 }
 
-__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
+// Digit encoding: (|d| - 1) | sign << 16, or DIGIT_ZERO for d = 0.
+static constexpr uint32_t DIGIT_ZERO = 1u << 17;
+// Scalars per counting-sort workgroup: a bucket's count within a chunk fits 16 bits, so the
+// LDS histogram packs two buckets per word (64 KB at c = 16).
+static constexpr uint32_t SORT_CHUNK = 32768;
+static constexpr int SORT_THREADS = 1024;
+static constexpr uint32_t SCATTER_SPAN = 16384;  // buckets per scatter workgroup (64 KB LDS)
 
-static void fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
-  if (!n) return;
-  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
-  G16_LAUNCH_CHECK("fill");
-}
-
-__device__ __forceinline__ bool scalar_is_one(const uint32_t s[8]) {
-  return (s[0] == 1u) && ((s[1] | s[2] | s[3] | s[4] | s[5] | s[6] | s[7]) == 0u);
-}
-
-// A scalar equal to 1 has one nonzero digit, window 0 digit 1: key 0. Witnesses are full of
-// them (every set bit), so their counter is taken once per wavefront (ballot + popcount)
-// rather than once per lane, which would serialise a quarter of a million atomics on one
-// address.
-__global__ void __launch_bounds__(256) k_digits_count(const uint4* scalars, uint32_t n, int c,
-                                                      int W, uint32_t nb, uint4* std_out,
-                                                      uint32_t* count) {
+__global__ void __launch_bounds__(256) k_digits(const uint4* scalars, uint32_t n, int c, int W,
+                                                uint32_t* digits) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool live = i < n;
-  Fr s = fp_zero<FrCfg>();
-  if (live) {
-    s = to_std(ld_fr(scalars + 2 * (size_t)i));
-    st_fr(std_out + 2 * (size_t)i, s);
-  }
-  bool one = live && scalar_is_one(s.v);
-  uint64_t ones = __ballot(one);
-  if (ones && __lane_id() == (uint32_t)(__ffsll((unsigned long long)ones) - 1))
-    atomicAdd(&count[0], (uint32_t)__popcll(ones));
-  if (!live || one) return;
+  if (i >= n) return;
+  Fr s = to_std(ld_fr(scalars + 2 * (size_t)i));
   for (int w = 0; w < W; ++w) {
     int d = signed_digit(s.v, w, c);
-    if (d == 0) continue;
-    uint32_t key = (uint32_t)w * nb + (uint32_t)(d < 0 ? -d : d) - 1;
-    atomicAdd(&count[key], 1u);
+    uint32_t e = d == 0 ? DIGIT_ZERO
+                        : ((uint32_t)(d < 0 ? -d : d) - 1u) | (d < 0 ? 1u << 16 : 0u);
+    digits[(size_t)w * n + i] = e;
   }
 }
 
-__global__ void __launch_bounds__(256) k_scatter(const uint4* std_in, uint32_t n, int c, int W,
-                                                 uint32_t nb, uint32_t* cursor, uint32_t* keys,
-                                                 uint32_t* vals) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool live = i < n;
-  Fr s = fp_zero<FrCfg>();
-  if (live) s = ld_fr(std_in + 2 * (size_t)i);
-  bool one = live && scalar_is_one(s.v);
-  uint64_t ones = __ballot(one);
-  if (ones) {
-    uint32_t leader = (uint32_t)(__ffsll((unsigned long long)ones) - 1);
-    uint32_t base = 0;
-    if (__lane_id() == leader) base = atomicAdd(&cursor[0], (uint32_t)__popcll(ones));
-    base = __shfl(base, (int)leader);
-    if (one) {
-      uint64_t below = ones & ((1ull << __lane_id()) - 1ull);
-      uint32_t pos = base + (uint32_t)__popcll(below);
-      keys[pos] = 0;
-      vals[pos] = i << 1;
-    }
+// Counts of chunk blockIdx.x of window blockIdx.y: chunk_count[w][p][b].
+__global__ void __launch_bounds__(SORT_THREADS) k_chunk_count(const uint32_t* digits,
+                                                              uint32_t n, uint32_t nb,
+                                                              uint32_t chunks,
+                                                              uint32_t* chunk_count) {
+  extern __shared__ uint32_t hist[];  // nb / 2 words
+  uint32_t p = blockIdx.x, w = blockIdx.y;
+  uint32_t words = nb / 2;
+  for (uint32_t j = threadIdx.x; j < words; j += SORT_THREADS) hist[j] = 0;
+  __syncthreads();
+  size_t lo = (size_t)p * SORT_CHUNK;
+  size_t hi = lo + SORT_CHUNK < n ? lo + SORT_CHUNK : n;
+  const uint32_t* d = digits + (size_t)w * n;
+  for (size_t i = lo + threadIdx.x; i < hi; i += SORT_THREADS) {
+    uint32_t e = d[i];
+    if (e & DIGIT_ZERO) continue;
+    uint32_t b = e & 0xFFFFu;
+    atomicAdd(&hist[b >> 1], 1u << ((b & 1u) << 4));
   }
-  if (!live || one) return;
-  for (int w = 0; w < W; ++w) {
-    int d = signed_digit(s.v, w, c);
-    if (d == 0) continue;
-    uint32_t key = (uint32_t)w * nb + (uint32_t)(d < 0 ? -d : d) - 1;
-    uint32_t pos = atomicAdd(&cursor[key], 1u);
-    keys[pos] = key;
-    vals[pos] = (i << 1) | (d < 0 ? 1u : 0u);
+  __syncthreads();
+  uint2* out = reinterpret_cast<uint2*>(chunk_count + ((size_t)w * chunks + p) * nb);
+  for (uint32_t j = threadIdx.x; j < words; j += SORT_THREADS) {
+    uint32_t v = hist[j];
+    out[j] = make_uint2(v & 0xFFFFu, v >> 16);
   }
 }
 
-// Exclusive scan of count[0..K) into offset[0..K], offset[K] = total; cursor = offset.
+// Per bucket (w, b): chunk_count[w][p][b] becomes the bucket's entries in chunks before p,
+// count[w * nb + b] its total.
+__global__ void __launch_bounds__(256) k_chunk_scan(uint32_t* chunk_count, uint32_t nb,
+                                                    uint32_t chunks, uint32_t K,
+                                                    uint32_t* count) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= K) return;
+  uint32_t w = t / nb, b = t % nb;
+  uint32_t* col = chunk_count + (size_t)w * chunks * nb + b;
+  uint32_t run = 0;
+  uint32_t p = 0;
+  for (; p + 4 <= chunks; p += 4) {
+    uint32_t v0 = col[(size_t)p * nb], v1 = col[(size_t)(p + 1) * nb];
+    uint32_t v2 = col[(size_t)(p + 2) * nb], v3 = col[(size_t)(p + 3) * nb];
+    col[(size_t)p * nb] = run;
+    col[(size_t)(p + 1) * nb] = run + v0;
+    col[(size_t)(p + 2) * nb] = run + v0 + v1;
+    col[(size_t)(p + 3) * nb] = run + v0 + v1 + v2;
+    run += v0 + v1 + v2 + v3;
+  }
+  for (; p < chunks; ++p) {
+    uint32_t v = col[(size_t)p * nb];
+    col[(size_t)p * nb] = run;
+    run += v;
+  }
+  count[t] = run;
+}
+
+// Entries of chunk blockIdx.x of window blockIdx.y whose bucket lies in
+// [blockIdx.z * span, + span): positions from the bucket offset, the chunk prefix and an LDS
+// cursor.
+__global__ void __launch_bounds__(SORT_THREADS) k_chunk_scatter(
+    const uint32_t* digits, uint32_t n, uint32_t nb, uint32_t chunks, uint32_t span,
+    const uint32_t* chunk_count, const uint32_t* offset, uint32_t* keys, uint32_t* vals) {
+  extern __shared__ uint32_t cur[];  // span words
+  uint32_t p = blockIdx.x, w = blockIdx.y, b0 = blockIdx.z * span;
+  const uint32_t* cc = chunk_count + ((size_t)w * chunks + p) * nb + b0;
+  const uint32_t* off = offset + (size_t)w * nb + b0;
+  for (uint32_t j = threadIdx.x; j < span; j += SORT_THREADS) cur[j] = off[j] + cc[j];
+  __syncthreads();
+  size_t lo = (size_t)p * SORT_CHUNK;
+  size_t hi = lo + SORT_CHUNK < n ? lo + SORT_CHUNK : n;
+  const uint32_t* d = digits + (size_t)w * n;
+  for (size_t i = lo + threadIdx.x; i < hi; i += SORT_THREADS) {
+    uint32_t e = d[i];
+    if (e & DIGIT_ZERO) continue;
+    uint32_t b = (e & 0xFFFFu) - b0;
+    if (b >= span) continue;
+    uint32_t pos = atomicAdd(&cur[b], 1u);
+    keys[pos] = w * nb + b0 + b;
+    vals[pos] = ((uint32_t)i << 1) | ((e >> 16) & 1u);
+  }
+}
+
+// Exclusive scan of count[0..K) into offset[0..K], offset[K] = total.
 // Three launches: per-1024-element block scans, a scan of the block sums, the fix-up.
 __global__ void __launch_bounds__(256) k_scan_blocks(const uint32_t* count, uint32_t K,
                                                      uint32_t* offset, uint32_t* bsum) {
@@ -170,14 +200,10 @@ __global__ void __launch_bounds__(1024) k_scan_top(uint32_t* bsum, uint32_t nblk
   if (t == 0) bsum[1024] = sh[1023];   // grand total
 }
 
-__global__ void k_scan_fix(uint32_t* offset, uint32_t* cursor, const uint32_t* bsum,
-                           uint32_t K, uint32_t* total) {
+__global__ void k_scan_fix(uint32_t* offset, const uint32_t* bsum, uint32_t K,
+                           uint32_t* total) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < K) {
-    uint32_t o = offset[i] + bsum[i / 1024];
-    offset[i] = o;
-    cursor[i] = o;
-  }
+  if (i < K) offset[i] += bsum[i / 1024];
   if (i == 0) {
     offset[K] = bsum[1024];
     total[0] = bsum[1024];
@@ -203,24 +229,30 @@ int msm_default_c(uint32_t n) {
 
 static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
+static uint32_t sort_chunks(uint32_t n) { return (n + SORT_CHUNK - 1) / SORT_CHUNK; }
+
 size_t msm_sort_scratch_bytes(uint32_t n, int c) {
   int W = (RECODE_BITS + c - 1) / c;
-  size_t K = (size_t)W << (c - 1);
+  size_t nb = (size_t)1 << (c - 1);
+  size_t K = (size_t)W * nb;
   size_t E = (size_t)n * W;
-  return align_up(4 * (K + 1)) * 3 + align_up(4 * 1025) + align_up(4 * E) * 2 +
-         align_up((size_t)n * 32) + align_up(16);
+  return align_up(4 * (K + 1)) * 2 + align_up(4 * 1025) + align_up(4 * E) * 3 +
+         align_up(4 * K * sort_chunks(n)) + align_up(16);
 }
 
 void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c) {
+  if (c < 2 || c > 16) throw ArgError{"MSM window width must be in [2, 16]"};
   p.c = c;
   p.windows = (RECODE_BITS + c - 1) / c;
   p.n = n;
   p.nb = 1u << (c - 1);
   p.seg = SEG;
   p.max_entries = (size_t)n * p.windows;
+  p.chunks = sort_chunks(n);
   size_t K = (size_t)p.windows * p.nb;
   if ((K + 1023) / 1024 > 1024) throw ArgError{"MSM window too wide for the scan"};
   if ((size_t)n >= (1u << 31)) throw ArgError{"MSM larger than 2^31 points"};
+  if (p.max_entries >= (1ull << 32)) throw ArgError{"MSM entry count exceeds 2^32"};
   char* q = static_cast<char*>(scratch);
   auto take = [&](size_t bytes) {
     char* r = q;
@@ -229,38 +261,55 @@ void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c) {
   };
   p.count = reinterpret_cast<uint32_t*>(take(4 * (K + 1)));
   p.offset = reinterpret_cast<uint32_t*>(take(4 * (K + 1)));
-  p.cursor = reinterpret_cast<uint32_t*>(take(4 * (K + 1)));
   p.block_sums = reinterpret_cast<uint32_t*>(take(4 * 1025));
   p.keys = reinterpret_cast<uint32_t*>(take(4 * p.max_entries));
   p.vals = reinterpret_cast<uint32_t*>(take(4 * p.max_entries));
-  p.scalars_std = reinterpret_cast<uint32_t*>(take((size_t)n * 32));
+  p.digits = reinterpret_cast<uint32_t*>(take(4 * p.max_entries));
+  p.chunk_count = reinterpret_cast<uint32_t*>(take(4 * K * p.chunks));
   p.total = reinterpret_cast<uint32_t*>(take(16));
 }
 
 void msm_sort(MsmPlan& p, const void* scalars, hipStream_t s) {
   uint32_t K = (uint32_t)p.windows * p.nb;
-  fill_u32(p.count, K + 1, 0, s);
   if (p.n) {
-    hipLaunchKernelGGL(k_digits_count, dim3((p.n + 255) / 256), dim3(256), 0, s,
-                       static_cast<const uint4*>(scalars), p.n, p.c, p.windows, p.nb,
-                       reinterpret_cast<uint4*>(p.scalars_std), p.count);
+    hipLaunchKernelGGL(k_digits, dim3((p.n + 255) / 256), dim3(256), 0, s,
+                       static_cast<const uint4*>(scalars), p.n, p.c, p.windows, p.digits);
     G16_LAUNCH_CHECK("msm digits");
+    hipLaunchKernelGGL(k_chunk_count, dim3(p.chunks, p.windows), dim3(SORT_THREADS),
+                       4 * (p.nb / 2), s, p.digits, p.n, p.nb, p.chunks, p.chunk_count);
+    G16_LAUNCH_CHECK("msm chunk count");
   }
+  hipLaunchKernelGGL(k_chunk_scan, dim3((K + 255) / 256), dim3(256), 0, s, p.chunk_count, p.nb,
+                     p.chunks, K, p.count);
+  G16_LAUNCH_CHECK("msm chunk scan");
   uint32_t nblk = (K + 1023) / 1024;
   hipLaunchKernelGGL(k_scan_blocks, dim3(nblk), dim3(256), 0, s, p.count, K, p.offset,
                      p.block_sums);
   G16_LAUNCH_CHECK("msm scan");
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, p.block_sums, nblk);
   G16_LAUNCH_CHECK("msm scan top");
-  hipLaunchKernelGGL(k_scan_fix, dim3((K + 255) / 256), dim3(256), 0, s, p.offset, p.cursor,
+  hipLaunchKernelGGL(k_scan_fix, dim3((K + 255) / 256), dim3(256), 0, s, p.offset,
                      p.block_sums, K, p.total);
   G16_LAUNCH_CHECK("msm scan fix");
   if (p.n) {
-    hipLaunchKernelGGL(k_scatter, dim3((p.n + 255) / 256), dim3(256), 0, s,
-                       reinterpret_cast<const uint4*>(p.scalars_std), p.n, p.c, p.windows, p.nb,
-                       p.cursor, p.keys, p.vals);
-    G16_LAUNCH_CHECK("msm scatter");
+    uint32_t span = p.nb < SCATTER_SPAN ? p.nb : SCATTER_SPAN;
+    hipLaunchKernelGGL(k_chunk_scatter, dim3(p.chunks, p.windows, p.nb / span),
+                       dim3(SORT_THREADS), 4 * span, s, p.digits, p.n, p.nb, p.chunks, span,
+                       p.chunk_count, p.offset, p.keys, p.vals);
+    G16_LAUNCH_CHECK("msm chunk scatter");
   }
+}
+
+void msm_reduce_shape(const MsmPlan& p, int* rows, int* lg) {
+  uint32_t B = reduce_group_size(p);
+  *lg = ilog2_u32(B);
+  *rows = 1 + ilog2_u32(p.nb / B);
+}
+
+size_t msm_out_points(const MsmPlan& p) {
+  int rows, lg;
+  msm_reduce_shape(p, &rows, &lg);
+  return (size_t)p.windows * rows;
 }
 
 size_t msm_accum_scratch_bytes(const MsmPlan& p, int group) {

@@ -241,53 +241,64 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_finish(const uint32_t* offset,
 // ---------------------------------------------------------------------------------------
 // window reduction: S_w = sum_j (j+1) B_{w,j}
 // ---------------------------------------------------------------------------------------
-// Group g of window w covers buckets [g*B, (g+1)*B): X = sum_i (i+1) B_{gB+i} + gB * sum_i B.
+// The buckets of a window fall in G groups of B = 2^lg. With T_g = sum_i B_{gB+i} and
+// U_g = sum_i (i+1) B_{gB+i},
+//   S_w = sum_g U_g + B sum_g g T_g = sum_g U_g + sum_k 2^(k+lg) Q_k,
+//   Q_k = sum of T_g over the g whose bit k is set.
+// Level 1 (one thread per group, 2B additions) makes T and U; level 2 is a plain sum per
+// (window, row) — row 0 sums U, row k+1 is Q_k — with strided per-thread sums and an LDS
+// tree. Every chain is short; the doublings of the bit weights happen once, in the host's
+// Horner pass over all windows.
 template <class F>
 __global__ void __launch_bounds__(128) k_reduce_groups(const uint4* buckets, uint32_t nb,
-                                                       int W, uint32_t B, uint4* out) {
+                                                       int W, uint32_t B, uint4* T,
+                                                       uint4* U) {
   uint32_t G = nb / B;
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (uint32_t)W * G) return;
-  uint32_t w = t / G, g = t % G;
-  size_t base = (size_t)w * nb + (size_t)g * B;
+  size_t base = (size_t)t * B;  // = w * nb + g * B
   Xyzz<F> run = xyzz_inf<F>(), acc = xyzz_inf<F>();
   for (int i = (int)B - 1; i >= 0; --i) {
     run = xyzz_add(run, ld_xyzz<F>(buckets, base + i));
     acc = xyzz_add(acc, run);
   }
-  acc = xyzz_add(acc, xyzz_mul_small(run, g * B));
-  st_xyzz(out, t, acc);
+  st_xyzz(T, t, run);
+  st_xyzz(U, t, acc);
 }
 
 template <class F, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_reduce_windows(const uint4* groups, uint32_t G,
-                                                          uint4* out) {
+__global__ void __launch_bounds__(BLOCK) k_reduce_rows(const uint4* T, const uint4* U,
+                                                       uint32_t G, uint32_t rows, uint4* out) {
   __shared__ Xyzz<F> sh[BLOCK];
-  uint32_t w = blockIdx.x;
+  uint32_t w = blockIdx.x / rows, r = blockIdx.x % rows;
+  size_t base = (size_t)w * G;
   Xyzz<F> acc = xyzz_inf<F>();
-  for (uint32_t g = threadIdx.x; g < G; g += BLOCK)
-    acc = xyzz_add(acc, ld_xyzz<F>(groups, (size_t)w * G + g));
-  sh[threadIdx.x] = acc;
-  __syncthreads();
-  for (int off = BLOCK / 2; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) sh[threadIdx.x] = xyzz_add(sh[threadIdx.x], sh[threadIdx.x + off]);
-    __syncthreads();
+  if (r == 0) {
+    for (uint32_t g = threadIdx.x; g < G; g += BLOCK) acc = xyzz_add(acc, ld_xyzz<F>(U, base + g));
+  } else {
+    uint32_t k = r - 1, low = (1u << k) - 1u;
+    for (uint32_t m = threadIdx.x; m < G / 2; m += BLOCK) {
+      uint32_t g = ((m & ~low) << 1) | (1u << k) | (m & low);  // m with a 1 inserted at bit k
+      acc = xyzz_add(acc, ld_xyzz<F>(T, base + g));
+    }
   }
-  if (threadIdx.x == 0) st_xyzz(out, w, sh[0]);
+  sh[threadIdx.x] = acc;
+  block_tree<F, BLOCK>(sh);
+  if (threadIdx.x == 0) st_xyzz(out, blockIdx.x, sh[0]);
 }
 
-inline uint32_t reduce_group_size(const MsmPlan& p) {
-  // About 32k group threads over all windows, at least 2 buckets each.
-  uint64_t total = (uint64_t)p.windows * p.nb;
-  uint32_t B = 2;
-  while ((total / B) > 32768 && B < p.nb) B <<= 1;
-  return B > p.nb ? p.nb : B;
+inline uint32_t reduce_group_size(const MsmPlan& p) { return p.nb < 8 ? p.nb : 8; }
+
+inline int ilog2_u32(uint32_t x) {
+  int l = 0;
+  while ((2u << l) <= x) ++l;
+  return l;
 }
 
 // Scratch of one accumulation (one base vector over one sorted entry list): buckets, the
 // segment end partials, heavy-bucket bookkeeping, task sums, reduction groups, counters.
 struct AccumLayout {
-  size_t buckets, pfirst, plast, heavy, tasks, task_sum, groups, cnt, total;
+  size_t buckets, pfirst, plast, heavy, tasks, task_sum, gT, gU, cnt, total;
 };
 
 inline AccumLayout accum_layout(const MsmPlan& p, size_t pt) {
@@ -310,7 +321,8 @@ inline AccumLayout accum_layout(const MsmPlan& p, size_t pt) {
   L.heavy = take(12 * max_heavy);
   L.tasks = take(8 * max_tasks);
   L.task_sum = take(pt * max_tasks);
-  L.groups = take(pt * G * p.windows);
+  L.gT = take(pt * G * p.windows);
+  L.gU = take(pt * G * p.windows);
   L.cnt = take(16);
   L.total = off;
   return L;
@@ -322,7 +334,7 @@ inline size_t msm_accum_scratch_bytes_impl(const MsmPlan& p, int group) {
 
 // Pointers into one accumulation's scratch.
 struct AccumPtrs {
-  uint4 *buckets, *pfirst, *plast, *task_sum, *groups;
+  uint4 *buckets, *pfirst, *plast, *task_sum, *gT, *gU;
   uint32_t *heavy, *tasks, *cnt;
 };
 
@@ -357,14 +369,14 @@ template <class F>
 inline void tail_reduce_impl(const MsmPlan& p, const AccumPtrs& a, uint32_t B, void* window_out,
                              hipStream_t s) {
   uint32_t G = p.nb / B;
+  uint32_t rows = 1 + (uint32_t)ilog2_u32(G);
   uint32_t nthreads = (uint32_t)p.windows * G;
   hipLaunchKernelGGL(k_reduce_groups<F>, dim3((nthreads + 127) / 128), dim3(128), 0, s,
-                     a.buckets, p.nb, p.windows, B, a.groups);
+                     a.buckets, p.nb, p.windows, B, a.gT, a.gU);
   G16_LAUNCH_CHECK("msm reduce groups");
-  constexpr int RB = sizeof(F) == sizeof(Fq) ? 128 : 64;
-  hipLaunchKernelGGL((k_reduce_windows<F, RB>), dim3(p.windows), dim3(RB), 0, s, a.groups, G,
-                     static_cast<uint4*>(window_out));
-  G16_LAUNCH_CHECK("msm reduce windows");
+  hipLaunchKernelGGL((k_reduce_rows<F, 256>), dim3((unsigned)p.windows * rows), dim3(256), 0, s,
+                     a.gT, a.gU, G, rows, static_cast<uint4*>(window_out));
+  G16_LAUNCH_CHECK("msm reduce rows");
 }
 
 // F: the coordinate field (Fq, or Fq2i for G2).
@@ -383,7 +395,8 @@ inline void accumulate_impl(const MsmPlan& p, const void* bases, void* scratch, 
   a.heavy = reinterpret_cast<uint32_t*>(q + L.heavy);
   a.tasks = reinterpret_cast<uint32_t*>(q + L.tasks);
   a.task_sum = reinterpret_cast<uint4*>(q + L.task_sum);
-  a.groups = reinterpret_cast<uint4*>(q + L.groups);
+  a.gT = reinterpret_cast<uint4*>(q + L.gT);
+  a.gU = reinterpret_cast<uint4*>(q + L.gU);
   a.cnt = reinterpret_cast<uint32_t*>(q + L.cnt);
   auto bp = static_cast<const uint4*>(bases);
 

@@ -1,0 +1,959 @@
+// The C ABI of include/g16hip.h, part 2: prepared circuits (Backend::prepare /
+// PreparedCircuit, crates/groth16/src/lib.rs:371-455), the prove pipeline (stages 0-9 on the
+// GPU, stage 11 on the host, crates/groth16/src/prove.rs:175-229, :391-430), batched proving
+// and the serving loop.
+//
+// A circuit owns its resident key and one or more proof slots. A slot is everything one
+// proof in flight needs: three streams, events, and the per-proof device scratch (witness,
+// A/B/C, H, sort and accumulation buffers, window sums) plus pinned host memory for the
+// window sums. Single proofs run on slot 0 with per-stage events (timings, host work started
+// per MSM); batches and the server run many slots at once, each replaying its proof pipeline
+// captured once as a hipGraph.
+#include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
+#include <deque>
+#include <unordered_map>
+
+#include "runtime_impl.h"
+
+using namespace g16;
+
+namespace g16 {
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipStream_t side[2] = {nullptr, nullptr};
+  hipEvent_t ev[8] = {};
+  hipEvent_t sync_ev[4] = {};  // witness ready, witness sorted, side[0] done, side[1] done
+  hipEvent_t done_ev[5] = {};  // window sums of A, B1, L, B2, H copied to host_wout
+  hipEvent_t mev[5][4] = {};   // per MSM: before accumulation, after it, after merges, after reduction
+  DevBuf witness, abc, hbuf, sort_w, sort_h, accum_w1, accum_w2, accum_h, wout;
+  MsmPlan plan_w, plan_h;
+  uint8_t* host_wout = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  size_t bytes = 0;
+
+  ~Slot() {
+    for (hipStream_t st : {stream, side[0], side[1]})
+      if (st) (void)hipStreamSynchronize(st);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : sync_ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : done_ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& row : mev)
+      for (auto& e : row)
+        if (e) (void)hipEventDestroy(e);
+    if (host_wout) (void)hipHostFree(host_wout);
+    for (hipStream_t st : {side[0], side[1], stream})
+      if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+}  // namespace g16
+
+struct g16hip_circuit {
+  g16hip_device* dev = nullptr;
+  uint32_t n_vars = 0, n_public = 0, domain = 0;
+  int log_n = 0;
+  int cw = 0, ch = 0;  // window widths of the witness and H MSMs
+  G1Aff alpha_g1, beta_g1, delta_g1;
+  G2Aff beta_g2, delta_g2;
+  DevBuf row_ptr[2], signal[2], value_id[2], values;
+  DevBuf a_q, b1_q, b2_q, l_q, h_q;
+  NttTables tables;
+  size_t key_bytes = 0;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::vector<std::unique_ptr<Slot>> slots;
+  std::vector<bool> busy;
+};
+
+struct g16hip_h {
+  g16hip_circuit* owner = nullptr;
+  DevBuf buf;
+};
+
+namespace g16 {
+
+// ---------------------------------------------------------------------------------------
+// key validation (crates/groth16/src/cpu.rs:73-122) and upload
+// ---------------------------------------------------------------------------------------
+static void validate_key(const g16hip_key* k) {
+  if (!k) throw ArgError{"null key"};
+  uint32_t n = k->domain_size;
+  if (n == 0 || (n & (n - 1))) throw ArgError{"domain size " + std::to_string(n) + " is not a power of two"};
+  if (n > (1u << 27)) throw ArgError{"domain size exceeds 2^27"};
+  if ((uint64_t)k->n_public + 1 > k->n_vars)
+    throw ArgError{"n_public " + std::to_string(k->n_public) + " exceeds n_vars " + std::to_string(k->n_vars)};
+  for (int m = 0; m < 2; ++m) {
+    const uint32_t* rp = k->row_ptr[m];
+    if (!rp) throw ArgError{"missing row_ptr"};
+    if (rp[0] != 0) throw ArgError{"matrix " + std::to_string(m) + " row_ptr starts at " + std::to_string(rp[0])};
+    for (uint32_t c = 0; c < n; ++c)
+      if (rp[c + 1] < rp[c]) throw ArgError{"matrix " + std::to_string(m) + " row_ptr is not monotone"};
+    uint32_t nnz = rp[n];
+    for (uint32_t e = 0; e < nnz; ++e) {
+      if (k->signal[m][e] >= k->n_vars) throw ArgError{"signal index out of range"};
+      if (k->value_id[m][e] >= k->n_values) throw ArgError{"value index out of range"};
+    }
+  }
+  const uint8_t* pts[] = {k->alpha_g1, k->beta_g1, k->delta_g1};
+  for (auto p : pts) {
+    G1Aff a = g1_load(p);
+    if (!fq_canonical(a.x) || !fq_canonical(a.y)) throw ArgError{"non-canonical key point"};
+    if (aff_is_inf(a)) throw ArgError{"a key point is the point at infinity, which no honest setup produces"};
+  }
+  if (aff_is_inf(g2_load(k->beta_g2)) || aff_is_inf(g2_load(k->delta_g2)))
+    throw ArgError{"a key point is the point at infinity, which no honest setup produces"};
+}
+
+static void upload_key(g16hip_circuit* C, const g16hip_key* k, hipStream_t s) {
+  uint32_t n = k->domain_size;
+  C->n_vars = k->n_vars;
+  C->n_public = k->n_public;
+  C->domain = n;
+  C->log_n = 0;
+  while ((1u << C->log_n) < n) ++C->log_n;
+  C->alpha_g1 = g1_load(k->alpha_g1);
+  C->beta_g1 = g1_load(k->beta_g1);
+  C->delta_g1 = g1_load(k->delta_g1);
+  C->beta_g2 = g2_load(k->beta_g2);
+  C->delta_g2 = g2_load(k->delta_g2);
+  for (int m = 0; m < 2; ++m) {
+    uint32_t nnz = k->row_ptr[m][n];
+    upload(C->row_ptr[m], k->row_ptr[m], 4 * (size_t)(n + 1), s);
+    upload(C->signal[m], k->signal[m], 4 * (size_t)nnz, s);
+    upload(C->value_id[m], k->value_id[m], 4 * (size_t)nnz, s);
+  }
+  upload(C->values, k->values, 32 * (size_t)k->n_values, s);
+  size_t nv = k->n_vars;
+  upload(C->a_q, k->a_query, 64 * nv, s);
+  upload(C->b1_q, k->b_g1_query, 64 * nv, s);
+  upload(C->b2_q, k->b_g2_query, 128 * nv, s);
+  // L covers the private wires only; it is stored behind n_public + 1 points at infinity so
+  // that it multiplies the full witness and shares the witness sort with A, B1 and B2.
+  size_t pub = (size_t)k->n_public + 1;
+  C->l_q.alloc(64 * nv);
+  G16_CHECK(hipMemsetAsync(C->l_q.p, 0, 64 * pub, s));
+  if (nv > pub)
+    G16_CHECK(hipMemcpyAsync(static_cast<char*>(C->l_q.p) + 64 * pub, k->l_query, 64 * (nv - pub),
+                             hipMemcpyHostToDevice, s));
+  upload(C->h_q, k->h_query, 64 * (size_t)n, s);
+  build_ntt_tables(C->tables, C->log_n, s);
+  C->cw = msm_default_c((uint32_t)nv);
+  C->ch = msm_default_c(n);
+  G16_CHECK(hipStreamSynchronize(s));
+  DevBuf* bufs[] = {&C->row_ptr[0], &C->row_ptr[1], &C->signal[0], &C->signal[1],
+                    &C->value_id[0], &C->value_id[1], &C->values, &C->a_q, &C->b1_q, &C->b2_q,
+                    &C->l_q, &C->h_q, &C->tables.fwd, &C->tables.inv, &C->tables.coset};
+  C->key_bytes = 0;
+  for (auto b : bufs) C->key_bytes += b->bytes;
+}
+
+// A slot's streams and events. HIP spreads a process's streams over a few hardware queues
+// (GPU_MAX_HW_QUEUES, 4 by default) in creation order, and streams that share a queue run
+// one after the other; a slot's three streams are therefore created back to back, before
+// anything else of the circuit, so that they land on different queues.
+static std::unique_ptr<Slot> new_slot() {
+  auto S = std::make_unique<Slot>();
+  G16_CHECK(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+  for (auto& st : S->side) G16_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  for (auto& e : S->ev) G16_CHECK(hipEventCreate(&e));
+  for (auto& e : S->sync_ev) G16_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  for (auto& e : S->done_ev) G16_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  for (auto& row : S->mev)
+    for (auto& e : row) G16_CHECK(hipEventCreate(&e));
+  return S;
+}
+
+// Per-proof scratch for one slot, sized once so a proof allocates nothing.
+static void size_slot(const g16hip_circuit* C, Slot* S) {
+  size_t nv = C->n_vars, n = C->domain;
+  S->witness.alloc(32 * nv);
+  S->abc.alloc(32 * 3 * n);
+  S->hbuf.alloc(32 * n);
+  S->sort_w.alloc(msm_sort_scratch_bytes((uint32_t)nv, C->cw));
+  S->sort_h.alloc(msm_sort_scratch_bytes((uint32_t)n, C->ch));
+  msm_plan_bind(S->plan_w, S->sort_w.p, (uint32_t)nv, C->cw);
+  msm_plan_bind(S->plan_h, S->sort_h.p, (uint32_t)n, C->ch);
+  S->accum_w1.alloc(msm_accum_scratch_bytes(S->plan_w, 1));
+  S->accum_w2.alloc(msm_accum_scratch_bytes(S->plan_w, 2));
+  S->accum_h.alloc(msm_accum_scratch_bytes(S->plan_h, 1));
+  size_t wbytes = 256 * (4 * msm_out_points(S->plan_w) + msm_out_points(S->plan_h));
+  S->wout.alloc(wbytes);
+  G16_CHECK(hipHostMalloc((void**)&S->host_wout, wbytes + 64, hipHostMallocDefault));
+  DevBuf* bufs[] = {&S->witness, &S->abc, &S->hbuf, &S->sort_w, &S->sort_h, &S->accum_w1,
+                    &S->accum_w2, &S->accum_h, &S->wout};
+  S->bytes = 0;
+  for (auto b : bufs) S->bytes += b->bytes;
+}
+
+static std::unique_ptr<Slot> make_slot(const g16hip_circuit* C) {
+  auto S = new_slot();
+  size_slot(C, S.get());
+  return S;
+}
+
+static size_t slot_bytes_estimate(const g16hip_circuit* C) {
+  size_t nv = C->n_vars, n = C->domain;
+  MsmPlan pw, ph;
+  pw.c = C->cw;
+  pw.windows = (255 + C->cw - 1) / C->cw;
+  pw.n = (uint32_t)nv;
+  pw.nb = 1u << (C->cw - 1);
+  pw.max_entries = nv * pw.windows;
+  ph.c = C->ch;
+  ph.windows = (255 + C->ch - 1) / C->ch;
+  ph.n = (uint32_t)n;
+  ph.nb = 1u << (C->ch - 1);
+  ph.max_entries = n * ph.windows;
+  return 32 * nv + 32 * 4 * n + msm_sort_scratch_bytes((uint32_t)nv, C->cw) +
+         msm_sort_scratch_bytes((uint32_t)n, C->ch) + msm_accum_scratch_bytes(pw, 1) +
+         msm_accum_scratch_bytes(pw, 2) + msm_accum_scratch_bytes(ph, 1) +
+         256 * (4 * msm_out_points(pw) + msm_out_points(ph));
+}
+
+// Exclusive use of one of the circuit's slots (waits for a free one).
+struct SlotLease {
+  g16hip_circuit* C;
+  size_t idx;
+  Slot* S;
+  explicit SlotLease(g16hip_circuit* c) : C(c) {
+    std::unique_lock<std::mutex> lk(C->mu);
+    C->cv.wait(lk, [&] {
+      for (size_t i = 0; i < C->busy.size(); ++i)
+        if (!C->busy[i]) return true;
+      return false;
+    });
+    for (idx = 0; C->busy[idx]; ++idx) {
+    }
+    C->busy[idx] = true;
+    S = C->slots[idx].get();
+  }
+  ~SlotLease() {
+    {
+      std::lock_guard<std::mutex> lk(C->mu);
+      C->busy[idx] = false;
+    }
+    C->cv.notify_all();
+  }
+};
+
+// ---------------------------------------------------------------------------------------
+// the pipeline
+// ---------------------------------------------------------------------------------------
+static void check_witness_len(const g16hip_circuit* C, size_t n) {
+  if (n != C->n_vars)
+    throw StatusError{G16HIP_WITNESS_LENGTH, "witness has " + std::to_string(n) +
+                                                 " entries, proving key expects " +
+                                                 std::to_string(C->n_vars)};
+}
+
+static void load_witness(const g16hip_circuit* C, Slot* S, const void* w, size_t n,
+                         bool on_device) {
+  check_witness_len(C, n);
+  G16_CHECK(hipMemcpyAsync(S->witness.p, w, 32 * n,
+                           on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                           S->stream));
+}
+
+// Stages 0-4 on the slot's main stream into `h_out` (device).
+static void stages_0_4(const g16hip_circuit* C, Slot* S, void* h_out, bool timed) {
+  hipStream_t s = S->stream;
+  size_t n = C->domain;
+  char* abc = S->abc.as<char>();
+  void* A = abc;
+  void* B = abc + 32 * n;
+  void* Cv = abc + 64 * n;
+  if (timed) G16_CHECK(hipEventRecord(S->ev[0], s));
+  launch_gather(C->row_ptr[0].as<uint32_t>(), C->signal[0].as<uint32_t>(),
+                C->value_id[0].as<uint32_t>(), C->values.p, S->witness.p, A, C->domain, s);
+  launch_gather(C->row_ptr[1].as<uint32_t>(), C->signal[1].as<uint32_t>(),
+                C->value_id[1].as<uint32_t>(), C->values.p, S->witness.p, B, C->domain, s);
+  launch_mul_into(Cv, A, B, n, s);
+  if (timed) G16_CHECK(hipEventRecord(S->ev[1], s));
+  ntt_dif_inverse(abc, 3, C->log_n, C->tables.inv.p, s);
+  ntt_dit_forward(abc, 3, C->log_n, C->tables.fwd.p, C->tables.coset.p, s);
+  if (timed) G16_CHECK(hipEventRecord(S->ev[2], s));
+  launch_h(h_out, A, B, Cv, n, s);
+  if (timed) G16_CHECK(hipEventRecord(S->ev[3], s));
+}
+
+static const size_t kMsmCount = 5;  // A, B1, L, B2, H
+
+struct WinLayout {
+  size_t off[5], len[5], totals;
+};
+
+// Reduction rows of each MSM in the slot's window-sum buffer (device and pinned host).
+static WinLayout win_layout(const Slot* S) {
+  size_t Ww = msm_out_points(S->plan_w), Wh = msm_out_points(S->plan_h);
+  return {{0, 128 * Ww, 256 * Ww, 384 * Ww, 640 * Ww},
+          {128 * Ww, 128 * Ww, 128 * Ww, 256 * Ww, 128 * Wh},
+          640 * Ww + 128 * Wh};
+}
+
+// Enqueues stages (0-4 if `with_h`) 5-9 on the slot's three streams: side[0] sorts the
+// witness digits once and accumulates A, B1 and L; side[1] accumulates B2 on the same
+// entries; the main stream runs stages 0-4 and then the H sort and MSM. Each MSM's window
+// sums go to pinned host memory on its own stream. With `timed`, events bracket every phase
+// (and done_ev[k] marks MSM k's sums on the host); a graph capture passes timed = false.
+static void enqueue_stages(const g16hip_circuit* C, Slot* S, bool with_h, void* h, bool timed) {
+  hipStream_t s0 = S->stream, s1 = S->side[0], s2 = S->side[1];
+  MsmPlan& pw = S->plan_w;
+  MsmPlan& ph = S->plan_h;
+  char* wout = S->wout.as<char>();
+  uint8_t* hw = S->host_wout;
+  WinLayout L = win_layout(S);
+  uint32_t* totals = reinterpret_cast<uint32_t*>(hw + L.totals);
+  auto sums_to_host = [&](int k, hipStream_t st) {
+    G16_CHECK(hipMemcpyAsync(hw + L.off[k], wout + L.off[k], L.len[k], hipMemcpyDeviceToHost, st));
+    if (timed) G16_CHECK(hipEventRecord(S->done_ev[k], st));
+  };
+  hipEvent_t* mev0 = timed ? S->mev[0] : nullptr;
+  hipEvent_t* mev1 = timed ? S->mev[1] : nullptr;
+  hipEvent_t* mev2 = timed ? S->mev[2] : nullptr;
+  hipEvent_t* mev3 = timed ? S->mev[3] : nullptr;
+  hipEvent_t* mev4 = timed ? S->mev[4] : nullptr;
+  G16_CHECK(hipEventRecord(S->sync_ev[0], s0));  // the witness is on the device
+  if (timed) G16_CHECK(hipEventRecord(S->ev[4], s0));
+  G16_CHECK(hipStreamWaitEvent(s1, S->sync_ev[0], 0));
+  msm_sort(pw, S->witness.p, s1);
+  G16_CHECK(hipEventRecord(S->sync_ev[1], s1));
+  // B2 is the longest chain: it is submitted first, so it also starts first where streams
+  // end up sharing a hardware queue.
+  G16_CHECK(hipStreamWaitEvent(s2, S->sync_ev[1], 0));
+  msm_accumulate(pw, 2, C->b2_q.p, S->accum_w2.p, wout + L.off[3], s2, mev3);
+  sums_to_host(3, s2);
+  G16_CHECK(hipEventRecord(S->sync_ev[3], s2));
+  G16_CHECK(hipMemcpyAsync(&totals[0], pw.total, 4, hipMemcpyDeviceToHost, s1));
+  msm_accumulate(pw, 1, C->a_q.p, S->accum_w1.p, wout + L.off[0], s1, mev0);
+  sums_to_host(0, s1);
+  msm_accumulate(pw, 1, C->b1_q.p, S->accum_w1.p, wout + L.off[1], s1, mev1);
+  sums_to_host(1, s1);
+  msm_accumulate(pw, 1, C->l_q.p, S->accum_w1.p, wout + L.off[2], s1, mev2);
+  sums_to_host(2, s1);
+  G16_CHECK(hipEventRecord(S->sync_ev[2], s1));
+  if (with_h) stages_0_4(C, S, h, timed);
+  msm_sort(ph, h, s0);
+  G16_CHECK(hipMemcpyAsync(&totals[1], ph.total, 4, hipMemcpyDeviceToHost, s0));
+  msm_accumulate(ph, 1, C->h_q.p, S->accum_h.p, wout + L.off[4], s0, mev4);
+  sums_to_host(4, s0);
+  G16_CHECK(hipStreamWaitEvent(s0, S->sync_ev[2], 0));
+  G16_CHECK(hipStreamWaitEvent(s0, S->sync_ev[3], 0));
+  if (timed) G16_CHECK(hipEventRecord(S->ev[5], s0));
+}
+
+// The stage-11 products that need an MSM output: s*A and r*B1 (prove.rs:391-430 groups the
+// rest with the key, see BlinderTerms).
+struct BlindProducts {
+  bool want = false;
+  Fr r_std, s_std;
+  G1X sA, rB1;
+};
+
+// Finishes MSM k on the host: Horner over its reduction rows, affine output, and the blinding
+// product where one rides on it. `wait` blocks until the sums are on the host.
+static void finish_msms(const g16hip_circuit* C, Slot* S, bool per_msm_events,
+                        g16hip_msm_out* out, BlindProducts* blind) {
+  WinLayout L = win_layout(S);
+  const uint8_t* hw = S->host_wout;
+  const MsmPlan& pw = S->plan_w;
+  const MsmPlan& ph = S->plan_h;
+  auto wait = [&](int k) {
+    if (per_msm_events) G16_CHECK(hipEventSynchronize(S->done_ev[k]));
+  };
+  auto g1_sum = [&](int k, const MsmPlan& p) {
+    wait(k);
+    std::vector<G1X> v(L.len[k] / sizeof(G1X));
+    std::memcpy(v.data(), hw + L.off[k], L.len[k]);
+    return msm_combine(v.data(), p);
+  };
+  bool want = blind && blind->want;
+  HostTask ta([&] {
+    G1X A = g1_sum(0, pw);
+    g1_store(out->a_g1, xyzz_to_aff(A));
+    if (want) blind->sA = xyzz_mul(A, blind->s_std.v);
+  });
+  HostTask tb([&] {
+    G1X B1 = g1_sum(1, pw);
+    g1_store(out->b_g1, xyzz_to_aff(B1));
+    if (want) blind->rB1 = xyzz_mul(B1, blind->r_std.v);
+  });
+  HostTask tl([&] { g1_store(out->l_g1, xyzz_to_aff(g1_sum(2, pw))); });
+  HostTask t2([&] {
+    wait(3);
+    std::vector<G2X> v(L.len[3] / sizeof(G2X));
+    std::memcpy(v.data(), hw + L.off[3], L.len[3]);
+    g2_store(out->b_g2, xyzz_to_aff(msm_combine(v.data(), pw)));
+  });
+  std::exception_ptr err;
+  try {
+    g1_store(out->h_g1, xyzz_to_aff(g1_sum(4, ph)));
+  } catch (...) {
+    err = std::current_exception();
+  }
+  for (HostTask* t : {&ta, &tb, &tl, &t2}) {
+    t->join();
+    if (!err && t->err) err = t->err;
+  }
+  if (err) std::rethrow_exception(err);
+}
+
+static void sync_slot(Slot* S) {
+  G16_CHECK(hipStreamSynchronize(S->stream));
+  G16_CHECK(hipStreamSynchronize(S->side[0]));
+  G16_CHECK(hipStreamSynchronize(S->side[1]));
+}
+
+// Eager run with events: timings, kernel statistics, the optional per-proof timeline.
+static void run_timed(g16hip_circuit* C, Slot* S, bool with_h, void* h, g16hip_msm_out* out,
+                      g16hip_timings* t, BlindProducts* blind) {
+  enqueue_stages(C, S, with_h, h, true);
+  std::exception_ptr err;
+  try {
+    finish_msms(C, S, true, out, blind);
+  } catch (...) {
+    err = std::current_exception();
+  }
+  sync_slot(S);
+  if (err) std::rethrow_exception(err);
+  WinLayout L = win_layout(S);
+  const uint32_t* totals = reinterpret_cast<const uint32_t*>(S->host_wout + L.totals);
+  for (size_t k = 0; k < kMsmCount; ++k)
+    note_accumulate(C->dev, k == 3 ? 2 : 1, k == 4 ? totals[1] : totals[0],
+                    ev_ms(S->mev[k][0], S->mev[k][1]));
+  static const bool timeline = std::getenv("G16HIP_TIMELINE") != nullptr;
+  if (timeline) {
+    // Milliseconds from the witness arriving on the device, per stream (the
+    // G16_CUDA_MEMORY_TRACE kind of switch, crates/cuda/src/backend.rs:89-93).
+    static const char* names[5] = {"A", "B1", "L", "B2", "H"};
+    std::fprintf(stderr, "[g16hip timeline]");
+    if (with_h)
+      std::fprintf(stderr, " ntt %.2f-%.2f h_done %.2f |", ev_ms(S->ev[4], S->ev[1]),
+                   ev_ms(S->ev[4], S->ev[2]), ev_ms(S->ev[4], S->ev[3]));
+    for (size_t k = 0; k < kMsmCount; ++k)
+      std::fprintf(stderr, " %s acc %.2f-%.2f merge %.2f reduce %.2f |", names[k],
+                   ev_ms(S->ev[4], S->mev[k][0]), ev_ms(S->ev[4], S->mev[k][1]),
+                   ev_ms(S->ev[4], S->mev[k][2]), ev_ms(S->ev[4], S->mev[k][3]));
+    std::fprintf(stderr, " end %.2f\n", ev_ms(S->ev[4], S->ev[5]));
+  }
+  if (t) {
+    if (with_h) {
+      t->gather_us += (uint64_t)(ev_ms(S->ev[0], S->ev[1]) * 1000.0f);
+      t->ntt_us += (uint64_t)(ev_ms(S->ev[1], S->ev[2]) * 1000.0f);
+      t->pointwise_us += (uint64_t)(ev_ms(S->ev[2], S->ev[3]) * 1000.0f);
+    }
+    // Stages 5-9 overlap stages 0-4: msm_us is the wall time from the witness arriving to the
+    // last MSM finishing, as the reference reports overlapped MSMs
+    // (crates/groth16/src/lib.rs:333-335).
+    t->msm_us += (uint64_t)(ev_ms(S->ev[4], S->ev[5]) * 1000.0f);
+  }
+}
+
+// Graph run: stages 0-9 captured once per slot (the fork/join of the three streams is part of
+// the graph; there are no memset nodes: every clear is a fill kernel), then replayed.
+static void run_graph(g16hip_circuit* C, Slot* S, g16hip_msm_out* out, BlindProducts* blind) {
+  if (!S->exec) {
+    G16_CHECK(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue_stages(C, S, true, S->hbuf.p, false);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(S->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    G16_CHECK(hipStreamEndCapture(S->stream, &S->graph));
+    G16_CHECK(hipGraphInstantiate(&S->exec, S->graph, nullptr, nullptr, 0));
+  }
+  G16_CHECK(hipGraphLaunch(S->exec, S->stream));
+  G16_CHECK(hipStreamSynchronize(S->stream));
+  finish_msms(C, S, false, out, blind);
+}
+
+// ---------------------------------------------------------------------------------------
+// stage 11 (crates/groth16/src/prove.rs:391-430), on the host:
+//   pi_a = A + alpha1 + r delta1,  pi_b = B2 + beta2 + s delta2,
+//   pi_c = L + H + s pi_a + r (B1 + beta1 + s delta1) - r s delta1
+//        = L + H + s A + r B1 + (s alpha1 + r beta1 + r s delta1).
+// The key-only terms are computed beside the GPU work; s A and r B1 as soon as those MSMs
+// finish. Group elements are canonical, so the proof is the same point whatever order the
+// terms are added in.
+// ---------------------------------------------------------------------------------------
+struct BlinderTerms {
+  G1X d1r, k1;  // r delta1;  s alpha1 + r beta1 + r s delta1
+  G2X d2s;      // s delta2
+};
+
+static BlinderTerms blinder_terms(const g16hip_circuit* C, const Fr& r_std, const Fr& s_std,
+                                  const Fr& rs_std) {
+  BlinderTerms b;
+  b.d1r = xyzz_mul_plain(C->delta_g1, r_std.v);
+  b.k1 = xyzz_add(xyzz_add(xyzz_mul_plain(C->alpha_g1, s_std.v),
+                           xyzz_mul_plain(C->beta_g1, r_std.v)),
+                  xyzz_mul_plain(C->delta_g1, rs_std.v));
+  b.d2s = xyzz_mul_plain(C->delta_g2, s_std.v);
+  return b;
+}
+
+static void assemble(const g16hip_circuit* C, const g16hip_msm_out& m, const BlinderTerms& b,
+                     const BlindProducts& bp, g16hip_proof* proof) {
+  G1X pi_a = xyzz_add(xyzz_add_aff(xyzz_from_aff(g1_load(m.a_g1)), C->alpha_g1), b.d1r);
+  G2X pi_b = xyzz_add(xyzz_add_aff(xyzz_from_aff(g2_load(m.b_g2)), C->beta_g2), b.d2s);
+  G1X pi_c = xyzz_add_aff(xyzz_from_aff(g1_load(m.l_g1)), g1_load(m.h_g1));
+  pi_c = xyzz_add(pi_c, bp.sA);
+  pi_c = xyzz_add(pi_c, bp.rB1);
+  pi_c = xyzz_add(pi_c, b.k1);
+  g1_store(proof->a, xyzz_to_aff(pi_a));
+  g2_store(proof->b, xyzz_to_aff(pi_b));
+  g1_store(proof->c, xyzz_to_aff(pi_c));
+}
+
+// Stages 0-11 for one witness on a leased slot: eager with timings (`graph` false) or as a
+// graph replay.
+static void prove_on_slot(g16hip_circuit* C, Slot* S, const void* w, size_t n, bool on_device,
+                          const uint8_t* r8, const uint8_t* s8, g16hip_proof* proof,
+                          g16hip_timings* t, bool graph) {
+  check_witness_len(C, n);
+  // w[0] is the constant-one wire (crates/groth16/src/prove.rs:272-283).
+  Fr w0;
+  if (on_device) G16_CHECK(hipMemcpy(w0.v, w, 32, hipMemcpyDeviceToHost));
+  else std::memcpy(w0.v, w, 32);
+  if (!is_one(w0)) throw StatusError{G16HIP_CONSTANT_WIRE, "w[0] is the constant-one wire and must be 1"};
+  Fr r = fr_load(r8), s = fr_load(s8);
+  if (is_zero(r) || is_zero(s)) throw StatusError{G16HIP_ZERO_BLINDER, "a blinder is zero"};
+  BlindProducts bp;
+  bp.want = true;
+  bp.r_std = to_std(r);
+  bp.s_std = to_std(s);
+  Fr rs_std = to_std(mul(r, s));
+  BlinderTerms terms;
+  HostTask th([&] { terms = blinder_terms(C, bp.r_std, bp.s_std, rs_std); });
+  g16hip_msm_out m;
+  load_witness(C, S, w, n, on_device);
+  if (graph) run_graph(C, S, &m, &bp);
+  else run_timed(C, S, true, S->hbuf.p, &m, t, &bp);
+  th.join();
+  if (th.err) std::rethrow_exception(th.err);
+  Timer ta;
+  assemble(C, m, terms, bp, proof);
+  if (t) t->assemble_us += ta.us();
+}
+
+}  // namespace g16
+
+// ---------------------------------------------------------------------------------------
+// server (continuous batching)
+// ---------------------------------------------------------------------------------------
+namespace g16 {
+
+struct Request {
+  g16hip_circuit* circuit = nullptr;
+  std::vector<uint8_t> witness;
+  uint8_t r[32], s[32];
+  Timer submitted;
+  g16hip_proof proof{};
+  g16hip_status status = G16HIP_OK;
+  std::string error;
+  double latency_ms = 0;
+  bool done = false;
+};
+
+struct CircuitQueue {
+  std::deque<std::shared_ptr<Request>> q;
+  uint32_t workers = 0;
+  uint32_t idle = 0;  // workers waiting for a request
+};
+
+}  // namespace g16
+
+struct g16hip_server {
+  g16hip_device* dev = nullptr;
+  size_t reserve = 0;
+  uint32_t max_slots = 1;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::unordered_map<g16hip_circuit*, g16::CircuitQueue> queues;
+  std::unordered_map<uint64_t, std::shared_ptr<g16::Request>> tickets;
+  uint64_t next_ticket = 1;
+  std::vector<std::thread> workers;
+  bool closing = false;
+};
+
+namespace g16 {
+
+// Adds one slot to `C` if the device keeps `reserve` bytes free after it (admission by free
+// device memory); returns whether it did.
+static bool try_add_slot(g16hip_circuit* C, size_t reserve) {
+  size_t free_b = 0, total_b = 0;
+  G16_CHECK(hipMemGetInfo(&free_b, &total_b));
+  if (free_b < slot_bytes_estimate(C) + reserve) return false;
+  auto S = make_slot(C);
+  std::lock_guard<std::mutex> lk(C->mu);
+  C->slots.push_back(std::move(S));
+  C->busy.push_back(false);
+  C->cv.notify_all();
+  return true;
+}
+
+// Batches and the server replay captured graphs; G16HIP_NO_GRAPH=1 makes them run the eager
+// path instead (the same kernels and streams, launched one by one).
+static bool replay_graphs() {
+  static const bool on = std::getenv("G16HIP_NO_GRAPH") == nullptr;
+  return on;
+}
+
+static void server_worker(g16hip_server* srv, g16hip_circuit* C) {
+  (void)hipSetDevice(srv->dev->ordinal);
+  for (;;) {
+    std::shared_ptr<Request> req;
+    {
+      std::unique_lock<std::mutex> lk(srv->mu);
+      CircuitQueue& cq = srv->queues[C];
+      cq.idle++;
+      srv->cv_work.wait(lk, [&] { return srv->closing || !cq.q.empty(); });
+      cq.idle--;
+      if (cq.q.empty()) return;  // closing, nothing left
+      req = cq.q.front();
+      cq.q.pop_front();
+    }
+    g16hip_status st = guarded([&] {
+      SlotLease lease(C);
+      prove_on_slot(C, lease.S, req->witness.data(), req->witness.size() / 32, false, req->r,
+                    req->s, &req->proof, nullptr, replay_graphs());
+      return G16HIP_OK;
+    });
+    {
+      std::lock_guard<std::mutex> lk(srv->mu);
+      req->status = st;
+      if (st != G16HIP_OK) req->error = g16hip_last_error();
+      req->latency_ms = req->submitted.ms();
+      req->done = true;
+    }
+    srv->cv_done.notify_all();
+  }
+}
+
+}  // namespace g16
+
+// ---------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------
+extern "C" {
+
+g16hip_status g16hip_prepare(g16hip_device* d, const g16hip_key* key, g16hip_circuit** out) {
+  return guarded([&] {
+    if (!d || !out) throw ArgError{"null argument"};
+    validate_key(key);
+    set_device(d);
+    auto C = std::make_unique<g16hip_circuit>();
+    C->dev = d;
+    // The key goes up on slot 0's main stream (a failure destroys the slot, which waits for
+    // its streams first).
+    auto S = new_slot();
+    upload_key(C.get(), key, S->stream);
+    size_slot(C.get(), S.get());
+    C->slots.push_back(std::move(S));
+    C->busy.push_back(false);
+    *out = C.release();
+    return G16HIP_OK;
+  });
+}
+
+void g16hip_circuit_release(g16hip_circuit* C) {
+  if (!C) return;
+  (void)hipSetDevice(C->dev->ordinal);
+  C->slots.clear();
+  delete C;
+}
+
+g16hip_status g16hip_circuit_dims(const g16hip_circuit* C, uint32_t* nv, uint32_t* np,
+                                  uint32_t* dom) {
+  return guarded([&] {
+    if (!C) throw ArgError{"null circuit"};
+    if (nv) *nv = C->n_vars;
+    if (np) *np = C->n_public;
+    if (dom) *dom = C->domain;
+    return G16HIP_OK;
+  });
+}
+
+size_t g16hip_circuit_resident_bytes(const g16hip_circuit* C) {
+  if (!C) return 0;
+  std::lock_guard<std::mutex> lk(const_cast<g16hip_circuit*>(C)->mu);
+  size_t tot = C->key_bytes;
+  for (auto& S : C->slots) tot += S->bytes;
+  return tot;
+}
+
+g16hip_status g16hip_circuit_add_slots(g16hip_circuit* C, uint32_t count, size_t reserve_bytes,
+                                       uint32_t* total_slots) {
+  return guarded([&] {
+    if (!C) throw ArgError{"null circuit"};
+    set_device(C->dev);
+    for (uint32_t i = 0; i < count; ++i)
+      if (!try_add_slot(C, reserve_bytes)) break;
+    if (total_slots) {
+      std::lock_guard<std::mutex> lk(C->mu);
+      *total_slots = (uint32_t)C->slots.size();
+    }
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_compute_h(g16hip_circuit* C, const uint8_t* w, size_t n, g16hip_h** h_out,
+                               g16hip_timings* t) {
+  return guarded([&] {
+    if (!C || !w || !h_out) throw ArgError{"null argument"};
+    set_device(C->dev);
+    SlotLease lease(C);
+    Slot* S = lease.S;
+    load_witness(C, S, w, n, false);
+    auto H = std::make_unique<g16hip_h>();
+    H->owner = C;
+    H->buf.alloc(32 * (size_t)C->domain);
+    stages_0_4(C, S, H->buf.p, true);
+    G16_CHECK(hipStreamSynchronize(S->stream));
+    if (t) {
+      t->gather_us += (uint64_t)(ev_ms(S->ev[0], S->ev[1]) * 1000.0f);
+      t->ntt_us += (uint64_t)(ev_ms(S->ev[1], S->ev[2]) * 1000.0f);
+      t->pointwise_us += (uint64_t)(ev_ms(S->ev[2], S->ev[3]) * 1000.0f);
+    }
+    *h_out = H.release();
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_h_to_host(g16hip_circuit* C, const g16hip_h* h, uint8_t* out) {
+  return guarded([&] {
+    if (!C || !h || !out) throw ArgError{"null argument"};
+    if (h->owner != C) throw ArgError{"H handle came from another circuit's compute_h"};
+    set_device(C->dev);
+    G16_CHECK(hipMemcpy(out, h->buf.p, 32 * (size_t)C->domain, hipMemcpyDeviceToHost));
+    return G16HIP_OK;
+  });
+}
+
+void g16hip_h_release(g16hip_h* h) {
+  if (!h) return;
+  (void)hipSetDevice(h->owner->dev->ordinal);
+  delete h;
+}
+
+g16hip_status g16hip_msms(g16hip_circuit* C, const uint8_t* w, size_t n, const g16hip_h* h,
+                          g16hip_msm_out* out, g16hip_timings* t) {
+  return guarded([&] {
+    if (!C || !w || !h || !out) throw ArgError{"null argument"};
+    if (h->owner != C)
+      throw ArgError{"H is a device handle from another circuit's compute_h, which this circuit cannot read"};
+    set_device(C->dev);
+    SlotLease lease(C);
+    load_witness(C, lease.S, w, n, false);
+    run_timed(C, lease.S, false, h->buf.p, out, t, nullptr);
+    return G16HIP_OK;
+  });
+}
+
+static g16hip_status h_and_msms_impl(g16hip_circuit* C, const void* w, size_t n, bool on_device,
+                                     g16hip_msm_out* out, g16hip_timings* t) {
+  return guarded([&] {
+    if (!C || !w || !out) throw ArgError{"null argument"};
+    set_device(C->dev);
+    SlotLease lease(C);
+    load_witness(C, lease.S, w, n, on_device);
+    run_timed(C, lease.S, true, lease.S->hbuf.p, out, t, nullptr);
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_h_and_msms(g16hip_circuit* C, const uint8_t* w, size_t n,
+                                g16hip_msm_out* out, g16hip_timings* t) {
+  return h_and_msms_impl(C, w, n, false, out, t);
+}
+
+g16hip_status g16hip_h_and_msms_device(g16hip_circuit* C, const void* w, size_t n,
+                                       g16hip_msm_out* out, g16hip_timings* t) {
+  return h_and_msms_impl(C, w, n, true, out, t);
+}
+
+static g16hip_status prove_impl(g16hip_circuit* C, const void* w, size_t n, bool on_device,
+                                const uint8_t* r, const uint8_t* s, g16hip_proof* proof,
+                                g16hip_timings* t) {
+  return guarded([&] {
+    if (!C || !w || !r || !s || !proof) throw ArgError{"null argument"};
+    set_device(C->dev);
+    SlotLease lease(C);
+    prove_on_slot(C, lease.S, w, n, on_device, r, s, proof, t, false);
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_prove_with_blinders(g16hip_circuit* C, const uint8_t* w, size_t n,
+                                         const uint8_t* r, const uint8_t* s, g16hip_proof* proof,
+                                         g16hip_timings* t) {
+  return prove_impl(C, w, n, false, r, s, proof, t);
+}
+
+g16hip_status g16hip_prove_with_blinders_device(g16hip_circuit* C, const void* w, size_t n,
+                                                const uint8_t* r, const uint8_t* s,
+                                                g16hip_proof* proof, g16hip_timings* t) {
+  return prove_impl(C, w, n, true, r, s, proof, t);
+}
+
+g16hip_status g16hip_prove_batch(const g16hip_batch_item* items, size_t n, g16hip_proof* proofs,
+                                 g16hip_status* statuses) {
+  return guarded([&] {
+    if (n && (!items || !proofs)) throw ArgError{"null argument"};
+    // One worker per slot of each circuit in the batch; each takes that circuit's items in
+    // order and replays its slot's captured pipeline.
+    std::unordered_map<g16hip_circuit*, std::vector<size_t>> by_circuit;
+    for (size_t i = 0; i < n; ++i) {
+      if (!items[i].circuit) throw ArgError{"null circuit in batch item " + std::to_string(i)};
+      by_circuit[items[i].circuit].push_back(i);
+    }
+    std::vector<g16hip_status> st(n, G16HIP_OK);
+    std::vector<std::string> msg(n);
+    std::vector<std::unique_ptr<HostTask>> workers;
+    std::vector<std::unique_ptr<std::atomic<size_t>>> cursors;
+    for (auto& kv : by_circuit) {
+      g16hip_circuit* C = kv.first;
+      const std::vector<size_t>* list = &kv.second;
+      cursors.push_back(std::make_unique<std::atomic<size_t>>(0));
+      std::atomic<size_t>* cur = cursors.back().get();
+      size_t nslots;
+      {
+        std::lock_guard<std::mutex> lk(C->mu);
+        nslots = C->slots.size();
+      }
+      size_t nw = std::min(nslots, list->size());
+      for (size_t k = 0; k < nw; ++k)
+        workers.push_back(std::make_unique<HostTask>([=, &st, &msg] {
+          (void)hipSetDevice(C->dev->ordinal);
+          for (size_t j; (j = cur->fetch_add(1)) < list->size();) {
+            size_t i = (*list)[j];
+            st[i] = guarded([&] {
+              SlotLease lease(C);
+              prove_on_slot(C, lease.S, items[i].witness, items[i].n_witness, false, items[i].r,
+                            items[i].s, &proofs[i], nullptr, replay_graphs());
+              return G16HIP_OK;
+            });
+            if (st[i] != G16HIP_OK) msg[i] = g16hip_last_error();
+          }
+        }));
+    }
+    for (auto& w : workers) w->join();
+    g16hip_status first = G16HIP_OK;
+    std::string first_msg;
+    for (size_t i = 0; i < n; ++i) {
+      if (statuses) statuses[i] = st[i];
+      if (first == G16HIP_OK && st[i] != G16HIP_OK) {
+        first = st[i];
+        first_msg = "batch item " + std::to_string(i) + ": " + msg[i];
+      }
+    }
+    if (first != G16HIP_OK) throw StatusError{first, first_msg};
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_server_open(g16hip_device* d, size_t reserve_bytes,
+                                 uint32_t max_slots_per_circuit, g16hip_server** out) {
+  return guarded([&] {
+    if (!d || !out) throw ArgError{"null argument"};
+    auto* srv = new g16hip_server();
+    srv->dev = d;
+    srv->reserve = reserve_bytes;
+    srv->max_slots = std::max<uint32_t>(1, max_slots_per_circuit);
+    *out = srv;
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_server_submit(g16hip_server* srv, g16hip_circuit* C, const uint8_t* w,
+                                   size_t n, const uint8_t* r, const uint8_t* s,
+                                   uint64_t* ticket) {
+  return guarded([&] {
+    if (!srv || !C || !w || !r || !s || !ticket) throw ArgError{"null argument"};
+    if (C->dev != srv->dev) throw ArgError{"circuit belongs to another device"};
+    check_witness_len(C, n);
+    auto req = std::make_shared<Request>();
+    req->circuit = C;
+    req->witness.assign(w, w + 32 * n);
+    std::memcpy(req->r, r, 32);
+    std::memcpy(req->s, s, 32);
+    bool spawn = false;
+    {
+      std::lock_guard<std::mutex> lk(srv->mu);
+      if (srv->closing) throw ArgError{"server is closing"};
+      CircuitQueue& cq = srv->queues[C];
+      cq.q.push_back(req);
+      *ticket = srv->next_ticket++;
+      srv->tickets[*ticket] = req;
+      // Continuous batching: a request that finds every worker of its circuit busy gets a new
+      // worker and slot if the device has the memory for it; otherwise it waits in the queue
+      // for the next slot to free up.
+      size_t slots;
+      {
+        std::lock_guard<std::mutex> lc(C->mu);
+        slots = C->slots.size();
+      }
+      if (cq.workers < srv->max_slots && cq.idle < cq.q.size()) {
+        if (cq.workers < slots) {
+          spawn = true;
+        } else {
+          set_device(srv->dev);
+          if (try_add_slot(C, srv->reserve)) spawn = true;
+        }
+        if (spawn) {
+          cq.workers++;
+          srv->workers.emplace_back(server_worker, srv, C);
+        }
+      }
+    }
+    srv->cv_work.notify_all();
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_server_wait(g16hip_server* srv, uint64_t ticket, g16hip_proof* proof,
+                                 double* latency_ms) {
+  return guarded([&] {
+    if (!srv || !proof) throw ArgError{"null argument"};
+    std::shared_ptr<Request> req;
+    {
+      std::unique_lock<std::mutex> lk(srv->mu);
+      auto it = srv->tickets.find(ticket);
+      if (it == srv->tickets.end()) throw ArgError{"unknown ticket"};
+      req = it->second;
+      srv->cv_done.wait(lk, [&] { return req->done; });
+      srv->tickets.erase(ticket);
+    }
+    if (latency_ms) *latency_ms = req->latency_ms;
+    if (req->status != G16HIP_OK) throw StatusError{req->status, req->error};
+    *proof = req->proof;
+    return G16HIP_OK;
+  });
+}
+
+void g16hip_server_close(g16hip_server* srv) {
+  if (!srv) return;
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    srv->closing = true;
+  }
+  srv->cv_work.notify_all();
+  for (auto& t : srv->workers)
+    if (t.joinable()) t.join();
+  delete srv;
+}
+
+}  // extern "C"

@@ -1,0 +1,219 @@
+// Host-side types shared by runtime.hip (devices, standalone entry points) and prove.hip
+// (prepared circuits, proofs, batches, the server).
+#pragma once
+#include <chrono>
+#include <cstring>
+#include <exception>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/g16hip.h"
+#include "bn254.cuh"
+#include "internal.h"
+
+namespace g16 {
+
+// ---- errors ---------------------------------------------------------------------------
+struct StatusError {
+  g16hip_status st;
+  std::string msg;
+};
+
+inline g16hip_status fail(g16hip_status st, const std::string& msg) {
+  set_last_error(msg);
+  return st;
+}
+
+template <class F>
+g16hip_status guarded(F&& f) {
+  try {
+    set_last_error("");
+    return f();
+  } catch (const HipError& e) {
+    bool oom = e.msg.find("out of memory") != std::string::npos;
+    return fail(oom ? G16HIP_DEVICE_MEMORY : G16HIP_DEVICE, e.msg);
+  } catch (const ArgError& e) {
+    return fail(G16HIP_BACKEND, e.msg);
+  } catch (const StatusError& e) {
+    return fail(e.st, e.msg);
+  } catch (const std::bad_alloc&) {
+    return fail(G16HIP_BACKEND, "host allocation failed");
+  }
+}
+
+// ---- device memory ---------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t n) {
+    if (n <= bytes && p) return;
+    release();
+    if (n == 0) return;
+    G16_CHECK(hipMalloc(&p, n));
+    bytes = n;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+inline void upload(DevBuf& b, const void* src, size_t n, hipStream_t s) {
+  b.alloc(n);
+  if (n) G16_CHECK(hipMemcpyAsync(b.p, src, n, hipMemcpyHostToDevice, s));
+}
+
+// ---- host-side arithmetic helpers (bn254.cuh compiled for the CPU) ----------------------
+inline Fr fr_load(const uint8_t* p) {
+  Fr x;
+  std::memcpy(x.v, p, 32);
+  return x;
+}
+inline G1Aff g1_load(const uint8_t* p) {
+  G1Aff a;
+  std::memcpy(a.x.v, p, 32);
+  std::memcpy(a.y.v, p + 32, 32);
+  return a;
+}
+inline G2Aff g2_load(const uint8_t* p) {
+  G2Aff a;
+  std::memcpy(a.x.c0.v, p, 32);
+  std::memcpy(a.x.c1.v, p + 32, 32);
+  std::memcpy(a.y.c0.v, p + 64, 32);
+  std::memcpy(a.y.c1.v, p + 96, 32);
+  return a;
+}
+inline void g1_store(uint8_t* p, const G1Aff& a) {
+  std::memcpy(p, a.x.v, 32);
+  std::memcpy(p + 32, a.y.v, 32);
+}
+inline void g2_store(uint8_t* p, const G2Aff& a) {
+  std::memcpy(p, a.x.c0.v, 32);
+  std::memcpy(p + 32, a.x.c1.v, 32);
+  std::memcpy(p + 64, a.y.c0.v, 32);
+  std::memcpy(p + 96, a.y.c1.v, 32);
+}
+
+inline bool fq_canonical(const Fq& x) {
+  for (int i = 7; i >= 0; --i) {
+    if (x.v[i] != FqCfg::p(i)) return x.v[i] < FqCfg::p(i);
+  }
+  return false;
+}
+
+// Domain root of order 2^L: 5^((r-1)/2^28) squared 28 - L times (crates/field/src/lib.rs:48-82).
+inline Fr domain_root(int L) {
+  Fr w;
+  const uint32_t W[8] = {0x725b19f0u, 0x9bd61b6eu, 0x41112ed4u, 0x402d111eu,
+                         0x8ef62abcu, 0x00e0a7ebu, 0xa58a7e85u, 0x2a3c09f0u};
+  std::memcpy(w.v, W, 32);
+  w = from_std(w);
+  for (int i = L; i < 28; ++i) w = sqr(w);
+  return w;
+}
+
+inline Fr fr_from_u64(uint64_t x) {
+  Fr r = fp_zero<FrCfg>();
+  r.v[0] = (uint32_t)x;
+  r.v[1] = (uint32_t)(x >> 32);
+  return from_std(r);
+}
+
+// The MSM from the reduction's rows (msm_accumulate): sum over windows w and rows j of
+// 2^e P[w][j], e = c w for row 0 and c w + lg + j - 1 for row j >= 1 — one Horner pass over
+// the exponents, one doubling each (the reference finishes sum_w 2^(cw) S_w on the host the
+// same way, crates/cuda/src/msm.rs:3976-4010).
+template <class F>
+Xyzz<F> msm_combine(const Xyzz<F>* P, const MsmPlan& p) {
+  int rows, lg;
+  msm_reduce_shape(p, &rows, &lg);
+  const int c = p.c, W = p.windows;
+  int emax = c * (W - 1) + (rows > 1 ? lg + rows - 2 : 0);
+  std::vector<Xyzz<F>> E((size_t)emax + 1, xyzz_inf<F>());
+  for (int w = 0; w < W; ++w)
+    for (int j = 0; j < rows; ++j) {
+      int e = c * w + (j == 0 ? 0 : lg + j - 1);
+      E[e] = xyzz_add(E[e], P[(size_t)w * rows + j]);
+    }
+  Xyzz<F> acc = E[emax];
+  for (int e = emax - 1; e >= 0; --e) acc = xyzz_add(xyzz_dbl(acc), E[e]);
+  return acc;
+}
+
+inline float ev_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  G16_CHECK(hipEventElapsedTime(&ms, a, b));
+  return ms;
+}
+
+struct Timer {
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  uint64_t us() const {
+    return (uint64_t)std::chrono::duration_cast<std::chrono::microseconds>(
+               std::chrono::steady_clock::now() - t0).count();
+  }
+  double ms() const { return us() / 1000.0; }
+};
+
+// Runs fn() on a thread and keeps the first exception for the joiner.
+struct HostTask {
+  std::exception_ptr err;  // declared first: the thread may set it as soon as it starts
+  std::thread th;
+  template <class Fn>
+  explicit HostTask(Fn fn) : th([this, fn] {
+    try {
+      fn();
+    } catch (...) {
+      err = std::current_exception();
+    }
+  }) {}
+  HostTask(const HostTask&) = delete;
+  HostTask& operator=(const HostTask&) = delete;
+  ~HostTask() { join(); }
+  void join() {
+    if (th.joinable()) th.join();
+  }
+};
+
+// ---- devices ---------------------------------------------------------------------------
+struct NttTables {
+  DevBuf fwd, inv, coset, scalars;  // scalars: [w, w^-1, shift, 1/n, 1, 0]
+};
+void build_ntt_tables(NttTables& t, int L, hipStream_t s);
+
+}  // namespace g16
+
+struct g16hip_device {
+  int ordinal = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;  // serialises the standalone entry points' scratch
+  g16::DevBuf scratch_a, scratch_b, scratch_c, scratch_d, scratch_e;
+  std::map<int, std::unique_ptr<g16::NttTables>> ntt;
+  hipEvent_t ev[6] = {};
+  g16hip_msm_stats stats = {};
+  std::mutex stats_mu;
+  g16hip_kernel_stats kstats = {};
+};
+
+namespace g16 {
+
+inline void note_accumulate(g16hip_device* d, int group, uint64_t entries, double ms) {
+  std::lock_guard<std::mutex> lk(d->stats_mu);
+  d->kstats.launches[group - 1] += 1;
+  d->kstats.entries[group - 1] += entries;
+  d->kstats.accumulate_ms[group - 1] += ms;
+}
+
+inline void set_device(const g16hip_device* d) { G16_CHECK(hipSetDevice(d->ordinal)); }
+
+}  // namespace g16

@@ -103,12 +103,45 @@ void invert_range(Fr* x, size_t lo, size_t hi) {
 
 struct g16synth {
   uint32_t n_vars = 0, n_public = 0, domain = 0, n_constraints = 0;
+  uint32_t n_in = 0, input_bool_permille = 0;
   std::vector<uint32_t> rp[3], sig[3];
   std::vector<Fr> val[3];
+  std::vector<uint8_t> bit_row;  // constraint j's output wire is a bit (b * b = b)
   std::vector<Fr> w;
   Fr toxic[5];  // tau, alpha, beta, gamma, delta
   std::vector<Fr> u, v, ic, l, h;
 };
+
+// The witness for witness seed `ws`: inputs first (a bit with probability
+// input_bool_permille / 1000, else uniform), then each constraint's output wire in order — a
+// drawn bit on a bit row, else (A_j . w) * (B_j . w). Any seed satisfies the circuit.
+static void fill_witness(const g16synth* S, uint64_t ws, std::vector<Fr>& w) {
+  Rng rng{ws};
+  const Fr one = fp_one<FrCfg>(), zero = fp_zero<FrCfg>();
+  w.assign(S->n_vars, zero);
+  w[0] = one;
+  for (uint32_t i = 1; i <= S->n_in; ++i) {
+    if (rng.u64() % 1000 < S->input_bool_permille)
+      w[i] = (rng.u64() & 1) ? one : zero;
+    else
+      w[i] = rng.fr();
+  }
+  for (uint32_t j = 0; j < S->n_constraints; ++j) {
+    uint32_t wire = 1 + S->n_in + j;
+    if (S->bit_row[j]) {
+      w[wire] = (rng.u64() & 1) ? one : zero;
+      continue;
+    }
+    Fr lin[2];
+    for (int m = 0; m < 2; ++m) {
+      Fr acc = zero;
+      for (uint32_t e = S->rp[m][j]; e < S->rp[m][j + 1]; ++e)
+        acc = add(acc, mul(S->val[m][e], w[S->sig[m][e]]));
+      lin[m] = acc;
+    }
+    w[wire] = mul(lin[0], lin[1]);
+  }
+}
 
 static void build(g16synth* S, const g16synth_params* p) {
   Rng rng{p->seed};
@@ -116,6 +149,8 @@ static void build(g16synth* S, const g16synth_params* p) {
   const uint32_t n_in = p->n_public + p->n_private_in;
   S->n_constraints = N;
   S->n_public = p->n_public;
+  S->n_in = n_in;
+  S->input_bool_permille = p->input_bool_permille;
   S->n_vars = 1 + n_in + N;
   uint32_t lg = 0;
   for (uint64_t x = (uint64_t)N + p->n_public; x >>= 1;) ++lg;
@@ -123,48 +158,49 @@ static void build(g16synth* S, const g16synth_params* p) {
   const uint32_t n = S->domain;
   const Fr one = fp_one<FrCfg>(), zero = fp_zero<FrCfg>();
 
-  // Witness inputs, then one constraint (and one fresh wire) at a time.
-  S->w.resize(S->n_vars);
-  S->w[0] = one;
-  for (uint32_t i = 1; i <= n_in; ++i) {
-    if (rng.u64() % 1000 < p->input_bool_permille)
-      S->w[i] = (rng.u64() & 1) ? one : zero;
-    else
-      S->w[i] = rng.fr();
-  }
+  // Structure, one constraint (and one fresh wire) at a time. A product row has T seeded
+  // terms over earlier wires in A and in B, and C = [wire]. A bit row is b * b = b on the
+  // fresh wire b, padded to T terms per A and B row with pairs k x - k x on earlier wires
+  // (and one zero term when T - 1 is odd), so every A and B row has T nonzeros and the
+  // constraint holds whatever bit the witness picks.
   for (int m = 0; m < 3; ++m) {
     S->rp[m].assign((size_t)n + 1, 0);
-    size_t cap = m < 2 ? (size_t)N * T : 2 * (size_t)N;
+    size_t cap = m < 2 ? (size_t)N * T : (size_t)N;
     S->sig[m].reserve(cap + (m == 0 ? p->n_public + 1 : 0));
     S->val[m].reserve(cap + (m == 0 ? p->n_public + 1 : 0));
   }
+  S->bit_row.assign(N, 0);
   uint32_t wire = 1 + n_in;
   for (uint32_t j = 0; j < N; ++j) {
-    Fr lin[2];
+    bool bit = rng.u64() % 1000 < p->bool_permille;
+    S->bit_row[j] = bit;
     for (int m = 0; m < 2; ++m) {
-      Fr acc = zero;
-      for (uint32_t t = 0; t < T; ++t) {
-        uint32_t s = (uint32_t)(rng.u64() % wire);
-        Fr coef = rng.fr();
-        S->sig[m].push_back(s);
-        S->val[m].push_back(coef);
-        acc = add(acc, mul(coef, S->w[s]));
+      if (bit) {
+        S->sig[m].push_back(wire);
+        S->val[m].push_back(one);
+        uint32_t left = T - 1;
+        for (; left >= 2; left -= 2) {
+          uint32_t x = (uint32_t)(rng.u64() % wire);
+          Fr k = rng.fr();
+          S->sig[m].push_back(x);
+          S->val[m].push_back(k);
+          S->sig[m].push_back(x);
+          S->val[m].push_back(neg(k));
+        }
+        if (left) {
+          S->sig[m].push_back((uint32_t)(rng.u64() % wire));
+          S->val[m].push_back(zero);
+        }
+      } else {
+        for (uint32_t t = 0; t < T; ++t) {
+          S->sig[m].push_back((uint32_t)(rng.u64() % wire));
+          S->val[m].push_back(rng.fr());
+        }
       }
-      lin[m] = acc;
       S->rp[m][j + 1] = (uint32_t)S->sig[m].size();
     }
-    Fr prod = mul(lin[0], lin[1]);
     S->sig[2].push_back(wire);
     S->val[2].push_back(one);
-    if (rng.u64() % 1000 < p->bool_permille) {
-      // A bit on the output wire; the constant wire absorbs the rest of the product.
-      Fr bit = (rng.u64() & 1) ? one : zero;
-      S->w[wire] = bit;
-      S->sig[2].push_back(0);
-      S->val[2].push_back(sub(prod, bit));
-    } else {
-      S->w[wire] = prod;
-    }
     S->rp[2][j + 1] = (uint32_t)S->sig[2].size();
     ++wire;
   }
@@ -176,6 +212,7 @@ static void build(g16synth* S, const g16synth_params* p) {
   for (int m = 0; m < 3; ++m)
     for (size_t c = 1; c <= n; ++c)
       if (S->rp[m][c] < S->rp[m][c - 1]) S->rp[m][c] = S->rp[m][c - 1];
+  fill_witness(S, p->witness_seed, S->w);
 
   Rng trng{p->setup_seed};
   for (auto& x : S->toxic) x = trng.fr();
@@ -271,6 +308,14 @@ int g16synth_get(const g16synth* s, int what, void* dst) {
     case 15: fr(s->h); return 0;
     default: return -1;
   }
+}
+
+int g16synth_witness(const g16synth* s, uint64_t witness_seed, uint8_t* out) {
+  if (!s || !out) return -1;
+  std::vector<Fr> w;
+  fill_witness(s, witness_seed, w);
+  std::memcpy(out, w.data(), 32 * w.size());
+  return 0;
 }
 
 void g16synth_scalars(uint64_t seed, size_t n, uint32_t zero_one_permille, uint8_t* out) {

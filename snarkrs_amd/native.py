@@ -95,6 +95,13 @@ class KernelStats(ctypes.Structure):
                 ("accumulate_ms", ctypes.c_double * 2)]
 
 
+class BatchItem(ctypes.Structure):
+    """``g16hip_batch_item``."""
+
+    _fields_ = [("circuit", _vp), ("witness", _u8p), ("n_witness", ctypes.c_size_t),
+                ("r", _u8p), ("s", _u8p)]
+
+
 class MsmOut(ctypes.Structure):
     _fields_ = [("a_g1", ctypes.c_uint8 * 64), ("b_g2", ctypes.c_uint8 * 128),
                 ("b_g1", ctypes.c_uint8 * 64), ("l_g1", ctypes.c_uint8 * 64),
@@ -159,6 +166,17 @@ EXPORTS = {
     "g16hip_prove_with_blinders_device": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _u8p, _u8p,
                                                          ctypes.POINTER(ProofOut),
                                                          ctypes.POINTER(Timings)]),
+    "g16hip_circuit_add_slots": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_size_t,
+                                                ctypes.POINTER(ctypes.c_uint32)]),
+    "g16hip_prove_batch": (ctypes.c_int, [ctypes.POINTER(BatchItem), ctypes.c_size_t,
+                                          ctypes.POINTER(ProofOut), ctypes.POINTER(ctypes.c_int)]),
+    "g16hip_server_open": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_uint32,
+                                          ctypes.POINTER(_vp)]),
+    "g16hip_server_submit": (ctypes.c_int, [_vp, _vp, _u8p, ctypes.c_size_t, _u8p, _u8p,
+                                            ctypes.POINTER(ctypes.c_uint64)]),
+    "g16hip_server_wait": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(ProofOut),
+                                          ctypes.POINTER(ctypes.c_double)]),
+    "g16hip_server_close": (None, [_vp]),
     "g16hip_msm_g1": (ctypes.c_int, [_vp, _u8p, _u8p, ctypes.c_size_t, _u8p]),
     "g16hip_msm_g2": (ctypes.c_int, [_vp, _u8p, _u8p, ctypes.c_size_t, _u8p]),
     "g16hip_msm_g1_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t, ctypes.c_int, _u8p]),
@@ -341,6 +359,12 @@ class Circuit:
     def resident_bytes(self) -> int:
         return int(self.lib.g16hip_circuit_resident_bytes(self.h))
 
+    def add_slots(self, count: int, reserve_bytes: int = 8 << 30) -> int:
+        """More proofs in flight on this circuit, admitted while device memory allows."""
+        total = ctypes.c_uint32()
+        _check(self.lib.g16hip_circuit_add_slots(self.h, count, reserve_bytes, ctypes.byref(total)))
+        return total.value
+
     def compute_h(self, witness: np.ndarray, t: Timings | None = None) -> HandleH:
         w = _c8(witness).reshape(-1, 32)
         h = _vp()
@@ -390,3 +414,52 @@ class Circuit:
                                                           ctypes.byref(out),
                                                           ctypes.byref(t or Timings())))
         return out.as_array()
+
+
+def prove_batch(items) -> list:
+    """items: (circuit, witness, r, s) tuples (arrays as for prove_with_blinders). Returns the
+    proofs in order; raises on the first failure."""
+    lib = load()
+    keep = []
+    arr = (BatchItem * len(items))()
+    for i, (c, w, r, s) in enumerate(items):
+        w, r, s = _c8(w).reshape(-1, 32), _c8(r), _c8(s)
+        keep += [w, r, s]
+        arr[i] = BatchItem(c.h, _p8(w), len(w), _p8(r), _p8(s))
+    out = (ProofOut * len(items))()
+    st = (ctypes.c_int * len(items))()
+    _check(lib.g16hip_prove_batch(arr, len(items), out, st))
+    return [out[i].as_array() for i in range(len(items))]
+
+
+class Server:
+    """The continuous-batching serving loop (``g16hip_server``)."""
+
+    def __init__(self, dev: Device, reserve_bytes: int = 8 << 30, max_slots_per_circuit: int = 4):
+        self.lib = dev.lib
+        h = _vp()
+        _check(self.lib.g16hip_server_open(dev.h, reserve_bytes, max_slots_per_circuit,
+                                           ctypes.byref(h)))
+        self.h = h
+
+    def submit(self, circ: Circuit, witness, r, s) -> int:
+        w = _c8(witness).reshape(-1, 32)
+        t = ctypes.c_uint64()
+        _check(self.lib.g16hip_server_submit(self.h, circ.h, _p8(w), len(w), _p8(_c8(r)),
+                                             _p8(_c8(s)), ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int):
+        """(proof bytes, latency in ms)."""
+        out = ProofOut()
+        lat = ctypes.c_double()
+        _check(self.lib.g16hip_server_wait(self.h, ticket, ctypes.byref(out), ctypes.byref(lat)))
+        return out.as_array(), lat.value
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.g16hip_server_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

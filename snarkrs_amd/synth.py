@@ -22,6 +22,7 @@ class SynthParams(ctypes.Structure):
         ("terms", ctypes.c_uint32),
         ("seed", ctypes.c_uint64),
         ("setup_seed", ctypes.c_uint64),
+        ("witness_seed", ctypes.c_uint64),
     ]
 
 
@@ -34,6 +35,8 @@ def _declare(lib):
     lib.g16synth_dims.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]
     lib.g16synth_get.restype = ctypes.c_int
     lib.g16synth_get.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    lib.g16synth_witness.restype = ctypes.c_int
+    lib.g16synth_witness.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint8)]
     lib.g16synth_scalars.argtypes = [ctypes.c_uint64, ctypes.c_size_t, ctypes.c_uint32,
                                      ctypes.POINTER(ctypes.c_uint8)]
     lib._synth_declared = True
@@ -51,14 +54,17 @@ class Workload:
     terms: int = 3
     seed: int = 1
     setup_seed: int = 2
+    witness_seed: int = 3
 
 
 # The plain-prove configuration: 2^16 multiplication-chain constraints, uniform inputs.
-PLAIN_2_16 = Workload(n_constraints=1 << 16, n_public=2, n_private_in=14, seed=16, setup_seed=17)
+PLAIN_2_16 = Workload(n_constraints=1 << 16, n_public=2, n_private_in=14, seed=16, setup_seed=17,
+                      witness_seed=18)
 # The 2^20-constraint class: 1.1M constraints (domain 2^21), about 1.2M wires, 3 nonzeros per
 # A/B row, 40% of wires boolean.
 LARGE_1_1M = Workload(n_constraints=1_100_000, n_public=2, n_private_in=100_000,
-                      bool_permille=400, input_bool_permille=400, seed=21, setup_seed=22)
+                      bool_permille=400, input_bool_permille=400, seed=21, setup_seed=22,
+                      witness_seed=23)
 
 
 class Synth:
@@ -67,7 +73,7 @@ class Synth:
         _declare(self.lib)
         self.w = w
         p = SynthParams(w.n_constraints, w.n_public, w.n_private_in, w.bool_permille,
-                        w.input_bool_permille, w.terms, w.seed, w.setup_seed)
+                        w.input_bool_permille, w.terms, w.seed, w.setup_seed, w.witness_seed)
         self.h = self.lib.g16synth_new(ctypes.byref(p))
         if not self.h:
             raise ValueError(f"bad synthetic workload {w}")
@@ -124,6 +130,14 @@ class Synth:
             b_g2_query=g2(a["v"]), l_query=g1(a["l_s"]), h_query=g1(a["h_s"]),
         )
         return key, a["witness"]
+
+    def witness(self, witness_seed: int) -> np.ndarray:
+        """Another satisfying witness of the same circuit (same key)."""
+        out = np.zeros((self.n_vars, 32), np.uint8)
+        rc = self.lib.g16synth_witness(self.h, witness_seed,
+                                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+        assert rc == 0
+        return out
 
 
 def scalars(seed: int, n: int, zero_one_permille: int = 300) -> np.ndarray:

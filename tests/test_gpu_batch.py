@@ -1,0 +1,95 @@
+"""Batched proving and the serving loop (g16hip_prove_batch, g16hip_server_*): every proof
+is bit-identical to the same proof made alone (and to the oracle's), whatever else is in
+flight; graph replays interleaved with eager proofs on the same circuit all agree."""
+import threading
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Synth
+from snarkrs_amd import native
+from snarkrs_amd.wire import fr_array, fr_ints
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def circuits(dev, orc):
+    out = []
+    for i, nc in enumerate([40, 700, 3000]):
+        syn = Synth(orc, nc, n_public=2, n_private_in=4, bool_permille=300,
+                    input_bool_permille=300, seed=100 + i, setup_seed=200 + i)
+        key, wit = syn.key()
+        circ = dev.prepare(key)
+        out.append((syn, key, wit, circ))
+    return out
+
+
+def _blind(i):
+    return fr_array([1000 + i]), fr_array([2000 + 3 * i])
+
+
+def test_batch_matches_single_proofs_and_oracle(dev, orc, circuits):
+    for _, _, _, circ in circuits:
+        circ.add_slots(2)
+    # independent witnesses of the same circuits
+    items, want, wits = [], [], []
+    for i in range(12):
+        syn, key, _, circ = circuits[i % 3]
+        wit = syn.witness(900 + i)
+        r, s = _blind(i)
+        items.append((circ, wit, r, s))
+        wits.append(wit)
+        want.append(circ.prove_with_blinders(wit, r, s))
+    got = native.prove_batch(items)
+    for i, (g, w) in enumerate(zip(got, want)):
+        assert (g == w).all(), i
+    # and against the oracle for a few
+    for i in (0, 4, 8):
+        syn, key, _, circ = circuits[i % 3]
+        r, s = _blind(i)
+        assert (got[i] == orc.prove(key, wits[i], fr_ints(r)[0], fr_ints(s)[0])[1]).all()
+
+
+def test_graph_replays_interleaved_with_eager_proofs(dev, circuits):
+    syn, key, wit, circ = circuits[2]
+    r, s = _blind(7)
+    eager = circ.prove_with_blinders(wit, r, s)
+    for k in range(4):
+        (g,) = native.prove_batch([(circ, wit, r, s)])
+        assert (g == eager).all(), f"replay {k}"
+        assert (circ.prove_with_blinders(wit, r, s) == eager).all()
+
+
+def test_server_results_are_the_same_alone_and_in_the_mix(dev, circuits):
+    alone = []
+    for i in range(9):
+        _, _, wit, circ = circuits[i % 3]
+        r, s = _blind(50 + i)
+        alone.append(circ.prove_with_blinders(wit, r, s))
+    srv = native.Server(dev, reserve_bytes=1 << 30, max_slots_per_circuit=3)
+    tickets = []
+    lock = threading.Lock()
+
+    def submit(i):
+        _, _, wit, circ = circuits[i % 3]
+        r, s = _blind(50 + i)
+        t = srv.submit(circ, wit, r, s)
+        with lock:
+            tickets.append((i, t))
+
+    ths = [threading.Thread(target=submit, args=(i,)) for i in range(9)]
+    [t.start() for t in ths]
+    [t.join() for t in ths]
+    for i, t in tickets:
+        proof, lat = srv.wait(t)
+        assert lat > 0
+        assert (proof == alone[i]).all(), i
+    srv.close()
+
+
+def test_batch_reports_bad_items(dev, circuits):
+    _, _, wit, circ = circuits[0]
+    r, s = _blind(1)
+    with pytest.raises(native.WitnessLength):
+        native.prove_batch([(circ, wit[:-1], r, s)])

@@ -5,6 +5,7 @@ closed form (sum k_i s_i) * G at the 2^20 standalone size."""
 import numpy as np
 import pytest
 
+from snarkrs_amd import native
 from snarkrs_amd.wire import R_MOD, SplitMix64, fr_array, g1_array
 from tests import pyref
 
@@ -64,9 +65,24 @@ def test_every_window_width(dev, orc):
     sc = fr_array(ss)
     want = orc.msm(1, bases, sc)
     db, ds = DeviceArray(bases), DeviceArray(sc)
-    for c in range(2, 18):
+    for c in range(2, 17):
         got = dev.msm_device(1, db.addr, ds.addr, n, c)
         assert (got == want).all(), c
+    with pytest.raises(native.BackendError):
+        dev.msm_device(1, db.addr, ds.addr, n, 17)
+
+
+@pytest.mark.parametrize("group,n", [(1, 32768), (1, 32769), (1, 100_001), (2, 40_000)])
+def test_sort_chunk_boundaries_closed_form(dev, group, n):
+    # The counting sort works in chunks of 32768 scalars: exact multiples, one past, several
+    # chunks; bases k_i G so the MSM is (sum k_i s_i) G.
+    rng = SplitMix64(31 + n)
+    ks = [rng.next_fr() for _ in range(n)]
+    ss = pyref.mixed_scalars(rng, n)
+    bases = _bases(dev, group, ks)
+    got = dev.msm(group, bases, fr_array(ss))
+    total = sum(k * s for k, s in zip(ks, ss)) % R_MOD
+    assert (got == dev.fixed_base(group, fr_array([total]))[0]).all()
 
 
 def test_standalone_2_20_closed_form(dev, orc):
