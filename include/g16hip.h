@@ -185,13 +185,18 @@ typedef struct {
 g16hip_status g16hip_prove_batch(const g16hip_batch_item* items, size_t n, g16hip_proof* proofs,
                                  g16hip_status* statuses);
 
-/* A serving loop with continuous batching: submit returns at once; a request that finds every
- * worker of its circuit busy gets a new slot and worker when the device keeps `reserve_bytes`
- * free after it (at most `max_slots_per_circuit`), else it queues until a slot frees up.
- * wait blocks for a ticket's proof and reports its latency (submit to completion). */
+/* A serving loop with continuous batching. `lanes` proofs are in flight at once (a host
+ * thread each; 1..64): a free lane takes the oldest request of any circuit, leases a slot of
+ * that circuit — adding one, up to `lanes` per circuit, while the device keeps
+ * `reserve_bytes` free (admission by free device memory), else waiting for one — and replays
+ * the slot's captured proof graph. submit returns at once; the witness stays owned by the
+ * caller and must stay valid until wait returns for its ticket. wait blocks for a ticket's
+ * proof and reports its latency (submit to completion). Slots' streams spread over the
+ * process's hardware queues (GPU_MAX_HW_QUEUES, 4 by default) and streams that share a queue
+ * run one after the other, so more queues let more proofs overlap. */
 typedef struct g16hip_server g16hip_server;
-g16hip_status g16hip_server_open(g16hip_device* dev, size_t reserve_bytes,
-                                 uint32_t max_slots_per_circuit, g16hip_server** out);
+g16hip_status g16hip_server_open(g16hip_device* dev, size_t reserve_bytes, uint32_t lanes,
+                                 g16hip_server** out);
 g16hip_status g16hip_server_submit(g16hip_server* srv, g16hip_circuit* c, const uint8_t* witness,
                                    size_t n_witness, const uint8_t* r, const uint8_t* s,
                                    uint64_t* ticket);

@@ -74,6 +74,7 @@ struct g16hip_circuit {
   std::condition_variable cv;
   std::vector<std::unique_ptr<Slot>> slots;
   std::vector<bool> busy;
+  uint32_t growing = 0;  // slots being added outside the lock
 };
 
 struct g16hip_h {
@@ -221,19 +222,38 @@ static size_t slot_bytes_estimate(const g16hip_circuit* C) {
          256 * (4 * msm_out_points(pw) + msm_out_points(ph));
 }
 
-// Exclusive use of one of the circuit's slots (waits for a free one).
+static bool try_add_slot(g16hip_circuit* C, size_t reserve);
+
+// Exclusive use of one of the circuit's slots. Waits for a free one; with `cap` > 0, first
+// adds a slot (up to `cap` in all, while the device keeps `reserve` bytes free) when every
+// slot is busy.
 struct SlotLease {
   g16hip_circuit* C;
   size_t idx;
   Slot* S;
-  explicit SlotLease(g16hip_circuit* c) : C(c) {
+  explicit SlotLease(g16hip_circuit* c, uint32_t cap = 0, size_t reserve = 0) : C(c) {
     std::unique_lock<std::mutex> lk(C->mu);
-    C->cv.wait(lk, [&] {
-      for (size_t i = 0; i < C->busy.size(); ++i)
-        if (!C->busy[i]) return true;
-      return false;
-    });
-    for (idx = 0; C->busy[idx]; ++idx) {
+    for (;;) {
+      for (idx = 0; idx < C->busy.size() && C->busy[idx]; ++idx) {
+      }
+      if (idx < C->busy.size()) break;
+      if (C->slots.size() + C->growing < cap) {
+        C->growing++;
+        lk.unlock();
+        bool added = false;
+        try {
+          added = try_add_slot(C, reserve);
+        } catch (...) {
+          lk.lock();
+          C->growing--;
+          throw;
+        }
+        lk.lock();
+        C->growing--;
+        if (added) continue;
+        cap = 0;  // no memory for another: wait for a slot to free up
+      }
+      C->cv.wait(lk);
     }
     C->busy[idx] = true;
     S = C->slots[idx].get();
@@ -461,6 +481,8 @@ static void run_timed(g16hip_circuit* C, Slot* S, bool with_h, void* h, g16hip_m
 
 // Graph run: stages 0-9 captured once per slot (the fork/join of the three streams is part of
 // the graph; there are no memset nodes: every clear is a fill kernel), then replayed.
+// Replays go on the slot's own main stream, which lives as long as the graph: launching a
+// slot's graph on a stream that is destroyed before the graph's next launch is not done.
 static void run_graph(g16hip_circuit* C, Slot* S, g16hip_msm_out* out, BlindProducts* blind) {
   if (!S->exec) {
     G16_CHECK(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
@@ -558,7 +580,8 @@ namespace g16 {
 
 struct Request {
   g16hip_circuit* circuit = nullptr;
-  std::vector<uint8_t> witness;
+  const uint8_t* witness = nullptr;  // caller-owned until the ticket is waited for
+  size_t n_witness = 0;
   uint8_t r[32], s[32];
   Timer submitted;
   g16hip_proof proof{};
@@ -568,21 +591,15 @@ struct Request {
   bool done = false;
 };
 
-struct CircuitQueue {
-  std::deque<std::shared_ptr<Request>> q;
-  uint32_t workers = 0;
-  uint32_t idle = 0;  // workers waiting for a request
-};
-
 }  // namespace g16
 
 struct g16hip_server {
   g16hip_device* dev = nullptr;
   size_t reserve = 0;
-  uint32_t max_slots = 1;
+  uint32_t lanes = 1;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
-  std::unordered_map<g16hip_circuit*, g16::CircuitQueue> queues;
+  std::deque<std::shared_ptr<g16::Request>> queue;  // first come, first served
   std::unordered_map<uint64_t, std::shared_ptr<g16::Request>> tickets;
   uint64_t next_ticket = 1;
   std::vector<std::thread> workers;
@@ -612,24 +629,24 @@ static bool replay_graphs() {
   return on;
 }
 
-static void server_worker(g16hip_server* srv, g16hip_circuit* C) {
+// A lane: takes the oldest request of any circuit, leases (or adds) a slot of its circuit and
+// replays that slot's proof graph.
+static void server_lane(g16hip_server* srv) {
   (void)hipSetDevice(srv->dev->ordinal);
   for (;;) {
     std::shared_ptr<Request> req;
     {
       std::unique_lock<std::mutex> lk(srv->mu);
-      CircuitQueue& cq = srv->queues[C];
-      cq.idle++;
-      srv->cv_work.wait(lk, [&] { return srv->closing || !cq.q.empty(); });
-      cq.idle--;
-      if (cq.q.empty()) return;  // closing, nothing left
-      req = cq.q.front();
-      cq.q.pop_front();
+      srv->cv_work.wait(lk, [&] { return srv->closing || !srv->queue.empty(); });
+      if (srv->queue.empty()) return;  // closing, nothing left
+      req = srv->queue.front();
+      srv->queue.pop_front();
     }
     g16hip_status st = guarded([&] {
-      SlotLease lease(C);
-      prove_on_slot(C, lease.S, req->witness.data(), req->witness.size() / 32, false, req->r,
-                    req->s, &req->proof, nullptr, replay_graphs());
+      g16hip_circuit* C = req->circuit;
+      SlotLease lease(C, srv->lanes, srv->reserve);
+      prove_on_slot(C, lease.S, req->witness, req->n_witness, false, req->r, req->s, &req->proof,
+                    nullptr, replay_graphs());
       return G16HIP_OK;
     });
     {
@@ -865,15 +882,17 @@ g16hip_status g16hip_prove_batch(const g16hip_batch_item* items, size_t n, g16hi
   });
 }
 
-g16hip_status g16hip_server_open(g16hip_device* d, size_t reserve_bytes,
-                                 uint32_t max_slots_per_circuit, g16hip_server** out) {
+g16hip_status g16hip_server_open(g16hip_device* d, size_t reserve_bytes, uint32_t lanes,
+                                 g16hip_server** out) {
   return guarded([&] {
     if (!d || !out) throw ArgError{"null argument"};
-    auto* srv = new g16hip_server();
+    if (lanes == 0 || lanes > 64) throw ArgError{"lanes must be in [1, 64]"};
+    auto srv = std::make_unique<g16hip_server>();
     srv->dev = d;
     srv->reserve = reserve_bytes;
-    srv->max_slots = std::max<uint32_t>(1, max_slots_per_circuit);
-    *out = srv;
+    srv->lanes = lanes;
+    for (uint32_t i = 0; i < lanes; ++i) srv->workers.emplace_back(server_lane, srv.get());
+    *out = srv.release();
     return G16HIP_OK;
   });
 }
@@ -887,39 +906,18 @@ g16hip_status g16hip_server_submit(g16hip_server* srv, g16hip_circuit* C, const 
     check_witness_len(C, n);
     auto req = std::make_shared<Request>();
     req->circuit = C;
-    req->witness.assign(w, w + 32 * n);
+    req->witness = w;
+    req->n_witness = n;
     std::memcpy(req->r, r, 32);
     std::memcpy(req->s, s, 32);
-    bool spawn = false;
     {
       std::lock_guard<std::mutex> lk(srv->mu);
       if (srv->closing) throw ArgError{"server is closing"};
-      CircuitQueue& cq = srv->queues[C];
-      cq.q.push_back(req);
+      srv->queue.push_back(req);
       *ticket = srv->next_ticket++;
       srv->tickets[*ticket] = req;
-      // Continuous batching: a request that finds every worker of its circuit busy gets a new
-      // worker and slot if the device has the memory for it; otherwise it waits in the queue
-      // for the next slot to free up.
-      size_t slots;
-      {
-        std::lock_guard<std::mutex> lc(C->mu);
-        slots = C->slots.size();
-      }
-      if (cq.workers < srv->max_slots && cq.idle < cq.q.size()) {
-        if (cq.workers < slots) {
-          spawn = true;
-        } else {
-          set_device(srv->dev);
-          if (try_add_slot(C, srv->reserve)) spawn = true;
-        }
-        if (spawn) {
-          cq.workers++;
-          srv->workers.emplace_back(server_worker, srv, C);
-        }
-      }
     }
-    srv->cv_work.notify_all();
+    srv->cv_work.notify_one();
     return G16HIP_OK;
   });
 }

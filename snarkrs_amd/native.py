@@ -435,11 +435,11 @@ def prove_batch(items) -> list:
 class Server:
     """The continuous-batching serving loop (``g16hip_server``)."""
 
-    def __init__(self, dev: Device, reserve_bytes: int = 8 << 30, max_slots_per_circuit: int = 4):
+    def __init__(self, dev: Device, reserve_bytes: int = 8 << 30, lanes: int = 4):
         self.lib = dev.lib
+        self._held = {}  # ticket -> witness array, alive until the ticket is waited for
         h = _vp()
-        _check(self.lib.g16hip_server_open(dev.h, reserve_bytes, max_slots_per_circuit,
-                                           ctypes.byref(h)))
+        _check(self.lib.g16hip_server_open(dev.h, reserve_bytes, lanes, ctypes.byref(h)))
         self.h = h
 
     def submit(self, circ: Circuit, witness, r, s) -> int:
@@ -447,13 +447,18 @@ class Server:
         t = ctypes.c_uint64()
         _check(self.lib.g16hip_server_submit(self.h, circ.h, _p8(w), len(w), _p8(_c8(r)),
                                              _p8(_c8(s)), ctypes.byref(t)))
+        self._held[t.value] = w
         return t.value
 
     def wait(self, ticket: int):
         """(proof bytes, latency in ms)."""
         out = ProofOut()
         lat = ctypes.c_double()
-        _check(self.lib.g16hip_server_wait(self.h, ticket, ctypes.byref(out), ctypes.byref(lat)))
+        try:
+            _check(self.lib.g16hip_server_wait(self.h, ticket, ctypes.byref(out),
+                                               ctypes.byref(lat)))
+        finally:
+            self._held.pop(ticket, None)
         return out.as_array(), lat.value
 
     def close(self) -> None:

@@ -67,7 +67,7 @@ def test_server_results_are_the_same_alone_and_in_the_mix(dev, circuits):
         _, _, wit, circ = circuits[i % 3]
         r, s = _blind(50 + i)
         alone.append(circ.prove_with_blinders(wit, r, s))
-    srv = native.Server(dev, reserve_bytes=1 << 30, max_slots_per_circuit=3)
+    srv = native.Server(dev, reserve_bytes=1 << 30, lanes=3)
     tickets = []
     lock = threading.Lock()
 
