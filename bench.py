@@ -370,7 +370,7 @@ def main() -> int:
     launches, entries, acc_ms = ks["launches"][0], ks["entries"][0], ks["accumulate_ms"][0]
     achieved = entries * MUL32_PER_MADD[1] / (acc_ms / 1e3) if acc_ms else 0.0
     roofline = {
-        "kernel": "k_accumulate<Fq> (G1 bucket accumulation)",
+        "kernel": "k_accumulate<Fm> (G1 bucket accumulation)",
         "bound": "valu",
         "achieved": achieved / 1e12,
         "peak": PEAK_MUL32_PER_S / 1e12,

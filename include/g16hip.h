@@ -251,8 +251,9 @@ g16hip_status g16hip_ntt(g16hip_device* dev, uint8_t* data, int log_n, int kind)
 g16hip_status g16hip_ntt_device(g16hip_device* dev, void* d_data, int log_n, int kind);
 
 /* Field arithmetic probe (the role of crates/cuda/src/kernels/field_probe.cu:16-112):
- * field 0 = Fr, 1 = Fq, 2 = Fq2; op 0 add, 1 sub, 2 mul, 3 sqr, 4 inv, 5 neg,
- * 6 to Montgomery, 7 from Montgomery. n elements, host memory. */
+ * field 0 = Fr, 1 = Fq, 2 = Fq2, 3 = Fq in the MSM kernels' Montgomery form R' = 2^261 (no
+ * reference counterpart); op 0 add, 1 sub, 2 mul, 3 sqr, 4 inv, 5 neg, 6 to Montgomery,
+ * 7 from Montgomery. n elements, host memory. */
 g16hip_status g16hip_field_probe(g16hip_device* dev, int field, int op, const uint8_t* a,
                                  const uint8_t* b, uint8_t* out, size_t n);
 

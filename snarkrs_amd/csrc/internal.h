@@ -87,6 +87,9 @@ size_t msm_accum_scratch_bytes(const MsmPlan& p, int group);
 void msm_accumulate(const MsmPlan& p, int group, const void* bases, void* scratch,
                     void* window_out, hipStream_t s, hipEvent_t* ev = nullptr);
 void msm_reduce_shape(const MsmPlan& p, int* rows, int* lg);
+// The MSM kernels keep coordinates in Montgomery form with R' = 2^261 (bn254.cuh Fm): bases go
+// in as 32 x mod q. dst may equal src; n_fq counts 32-byte coordinates.
+void msm_to_msm_form(const void* src, void* dst, size_t n_fq, hipStream_t s);
 size_t msm_out_points(const MsmPlan& p);
 int msm_default_c(uint32_t n);
 

@@ -70,8 +70,10 @@ void launch_field_probe(int field, int op, const void* a, const void* b, void* o
     hipLaunchKernelGGL(k_probe_fp<FrCfg>, dim3(grid), dim3(256), 0, s, op, A, B, O, n);
   else if (field == 1)
     hipLaunchKernelGGL(k_probe_fp<FqCfg>, dim3(grid), dim3(256), 0, s, op, A, B, O, n);
-  else
+  else if (field == 2)
     hipLaunchKernelGGL(k_probe_fq2, dim3(grid), dim3(256), 0, s, op, A, B, O, n);
+  else
+    hipLaunchKernelGGL(k_probe_fp<FqmCfg>, dim3(grid), dim3(256), 0, s, op, A, B, O, n);
   G16_LAUNCH_CHECK("field probe");
 }
 

@@ -317,14 +317,31 @@ size_t msm_accum_scratch_bytes(const MsmPlan& p, int group) {
 }
 
 template <>
-void Tail<Fq>::merge(const MsmPlan& p, const AccumPtrs& a, hipStream_t s) {
-  tail_merge_impl<Fq>(p, a, s);
+void Tail<Fm>::merge(const MsmPlan& p, const AccumPtrs& a, hipStream_t s) {
+  tail_merge_impl<Fm>(p, a, s);
 }
 
 template <>
-void Tail<Fq>::reduce(const MsmPlan& p, const AccumPtrs& a, uint32_t B, void* window_out,
+void Tail<Fm>::reduce(const MsmPlan& p, const AccumPtrs& a, uint32_t B, void* window_out,
                       hipStream_t s) {
-  tail_reduce_impl<Fq>(p, a, B, window_out, s);
+  tail_reduce_impl<Fm>(p, a, B, window_out, s);
+}
+
+// x -> 32 x mod q (five modular doublings): the R = 2^256 Montgomery form to R' = 2^261.
+__global__ void __launch_bounds__(256) k_to_msm_form(const uint4* src, uint4* dst, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Fq x = ld_fq(src + 2 * i);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) x = add(x, x);
+  st_fq(dst + 2 * i, x);
+}
+
+void msm_to_msm_form(const void* src, void* dst, size_t n_fq, hipStream_t s) {
+  if (!n_fq) return;
+  hipLaunchKernelGGL(k_to_msm_form, dim3((unsigned)((n_fq + 255) / 256)), dim3(256), 0, s,
+                     static_cast<const uint4*>(src), static_cast<uint4*>(dst), n_fq);
+  G16_LAUNCH_CHECK("msm bases to R' form");
 }
 
 // G2 lives in kernels_msm_g2*.hip, compiled on their own (the slow ones to compile).
@@ -333,7 +350,7 @@ void msm_accumulate_g2(const MsmPlan& p, const void* bases, void* scratch, void*
 
 void msm_accumulate(const MsmPlan& p, int group, const void* bases, void* scratch,
                     void* window_out, hipStream_t s, hipEvent_t* ev) {
-  if (group == 1) accumulate_impl<Fq>(p, bases, scratch, window_out, s, ev);
+  if (group == 1) accumulate_impl<Fm>(p, bases, scratch, window_out, s, ev);
   else msm_accumulate_g2(p, bases, scratch, window_out, s, ev);
 }
 

@@ -1,4 +1,4 @@
-// G2 bucket accumulation (msm_accum.cuh) on Fq2 with its products expanded in place (Fq2i),
+// G2 bucket accumulation (msm_accum.cuh) on Fq2 with its products expanded in place (Fm2i),
 // in a translation unit of its own so it compiles in parallel with the rest; the merge and
 // reduction launchers are in kernels_msm_g2_tail.hip and kernels_msm_g2_reduce.hip.
 #include "msm_accum.cuh"
@@ -7,7 +7,7 @@ namespace g16 {
 
 void msm_accumulate_g2(const MsmPlan& p, const void* bases, void* scratch, void* window_out,
                        hipStream_t s, hipEvent_t* ev) {
-  accumulate_impl<Fq2i>(p, bases, scratch, window_out, s, ev);
+  accumulate_impl<Fm2i>(p, bases, scratch, window_out, s, ev);
 }
 
 }  // namespace g16

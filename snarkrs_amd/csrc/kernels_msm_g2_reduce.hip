@@ -1,12 +1,12 @@
-// G2 window reduction on Fq2i: see msm_accum.cuh.
+// G2 window reduction on Fm2i: see msm_accum.cuh.
 #include "msm_accum.cuh"
 
 namespace g16 {
 
 template <>
-void Tail<Fq2i>::reduce(const MsmPlan& p, const AccumPtrs& a, uint32_t B, void* window_out,
+void Tail<Fm2i>::reduce(const MsmPlan& p, const AccumPtrs& a, uint32_t B, void* window_out,
                         hipStream_t s) {
-  tail_reduce_impl<Fq2i>(p, a, B, window_out, s);
+  tail_reduce_impl<Fm2i>(p, a, B, window_out, s);
 }
 
 }  // namespace g16

@@ -35,17 +35,25 @@ __device__ __forceinline__ void st_fq(uint4* p, const Fq& x) {
 }
 
 // Field element loads in units of 32 bytes (2 x uint4).
-__device__ __forceinline__ void ld_f(Fq& x, const uint4* p) { x = ld_fq(p); }
-template <bool INL>
-__device__ __forceinline__ void ld_f(Fq2T<INL>& x, const uint4* p) {
-  x.c0 = ld_fq(p);
-  x.c1 = ld_fq(p + 2);
+template <class C>
+__device__ __forceinline__ void ld_f(Fp<C>& x, const uint4* p) {
+  uint4 a = p[0], b = p[1];
+  x = {{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w}};
 }
-__device__ __forceinline__ void st_f(uint4* p, const Fq& x) { st_fq(p, x); }
-template <bool INL>
-__device__ __forceinline__ void st_f(uint4* p, const Fq2T<INL>& x) {
-  st_fq(p, x.c0);
-  st_fq(p + 2, x.c1);
+template <bool INL, class B>
+__device__ __forceinline__ void ld_f(Fq2T<INL, B>& x, const uint4* p) {
+  ld_f(x.c0, p);
+  ld_f(x.c1, p + 2);
+}
+template <class C>
+__device__ __forceinline__ void st_f(uint4* p, const Fp<C>& x) {
+  p[0] = make_uint4(x.v[0], x.v[1], x.v[2], x.v[3]);
+  p[1] = make_uint4(x.v[4], x.v[5], x.v[6], x.v[7]);
+}
+template <bool INL, class B>
+__device__ __forceinline__ void st_f(uint4* p, const Fq2T<INL, B>& x) {
+  st_f(p, x.c0);
+  st_f(p + 2, x.c1);
 }
 
 template <class F> struct Words { static constexpr int N = sizeof(F) / 16; };  // uint4s per F
@@ -104,7 +112,7 @@ __global__ void __launch_bounds__(128) k_accumulate(const uint32_t* __restrict__
   Xyzz<F> acc = xyzz_inf<F>();
   // G1: the next entry's point is loaded before this one is added, so the load latency of a
   // random base hides behind a mixed addition (the G2 loop has no registers to spare).
-  constexpr bool PREFETCH = sizeof(F) == sizeof(Fq);
+  constexpr bool PREFETCH = sizeof(F) == 32;
   uint32_t v = vals[b];
   Aff<F> a = ld_aff<F>(bases, v >> 1);
   for (size_t k = b; k < e; ++k) {
@@ -339,7 +347,7 @@ struct AccumPtrs {
 };
 
 // Launchers of the merge and reduction kernels over field F. The G1 ones are defined here;
-// the G2 ones (F = Fq2i, products expanded in place so the latency-bound tail does not spill)
+// the G2 ones (F = Fm2i, products expanded in place so the latency-bound tail does not spill)
 // live in kernels_msm_g2_tail.hip and kernels_msm_g2_reduce.hip, so the slow G2 kernels
 // compile in parallel translation units.
 template <class F>
@@ -379,7 +387,8 @@ inline void tail_reduce_impl(const MsmPlan& p, const AccumPtrs& a, uint32_t B, v
   G16_LAUNCH_CHECK("msm reduce rows");
 }
 
-// F: the coordinate field (Fq, or Fq2i for G2).
+// F: the coordinate field in the MSM's Montgomery form R' = 2^261 (Fm, or Fm2i for G2); the
+// bases must already be in that form (msm_bases_to_msm_form).
 template <class F>
 inline void accumulate_impl(const MsmPlan& p, const void* bases, void* scratch, void* window_out,
                             hipStream_t s, hipEvent_t* ev) {

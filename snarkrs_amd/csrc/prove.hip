@@ -52,8 +52,9 @@ struct Slot {
       for (auto& e : row)
         if (e) (void)hipEventDestroy(e);
     if (host_wout) (void)hipHostFree(host_wout);
-    for (hipStream_t st : {side[0], side[1], stream})
-      if (st) (void)hipStreamDestroy(st);
+    for (hipStream_t st : side)
+      if (st && st != stream) (void)hipStreamDestroy(st);
+    if (stream) (void)hipStreamDestroy(stream);
   }
 };
 
@@ -148,6 +149,12 @@ static void upload_key(g16hip_circuit* C, const g16hip_key* k, hipStream_t s) {
     G16_CHECK(hipMemcpyAsync(static_cast<char*>(C->l_q.p) + 64 * pub, k->l_query, 64 * (nv - pub),
                              hipMemcpyHostToDevice, s));
   upload(C->h_q, k->h_query, 64 * (size_t)n, s);
+  // The bases serve only the MSMs: keep them in the MSM kernels' R' = 2^261 form.
+  msm_to_msm_form(C->a_q.p, C->a_q.p, 2 * nv, s);
+  msm_to_msm_form(C->b1_q.p, C->b1_q.p, 2 * nv, s);
+  msm_to_msm_form(C->l_q.p, C->l_q.p, 2 * nv, s);
+  msm_to_msm_form(C->h_q.p, C->h_q.p, 2 * (size_t)n, s);
+  msm_to_msm_form(C->b2_q.p, C->b2_q.p, 4 * nv, s);
   build_ntt_tables(C->tables, C->log_n, s);
   C->cw = msm_default_c((uint32_t)nv);
   C->ch = msm_default_c(n);
@@ -163,10 +170,20 @@ static void upload_key(g16hip_circuit* C, const g16hip_key* k, hipStream_t s) {
 // (GPU_MAX_HW_QUEUES, 4 by default) in creation order, and streams that share a queue run
 // one after the other; a slot's three streams are therefore created back to back, before
 // anything else of the circuit, so that they land on different queues.
+// G16HIP_SERIAL=1 runs a slot's three streams' work on its main stream, one kernel at a time
+// (for traces whose kernel durations are each kernel's alone).
+static bool serial_slots() {
+  static const bool on = std::getenv("G16HIP_SERIAL") != nullptr;
+  return on;
+}
+
 static std::unique_ptr<Slot> new_slot() {
   auto S = std::make_unique<Slot>();
   G16_CHECK(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
-  for (auto& st : S->side) G16_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  if (serial_slots())
+    S->side[0] = S->side[1] = S->stream;
+  else
+    for (auto& st : S->side) G16_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   for (auto& e : S->ev) G16_CHECK(hipEventCreate(&e));
   for (auto& e : S->sync_ev) G16_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   for (auto& e : S->done_ev) G16_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -395,6 +412,7 @@ static void finish_msms(const g16hip_circuit* C, Slot* S, bool per_msm_events,
     wait(k);
     std::vector<G1X> v(L.len[k] / sizeof(G1X));
     std::memcpy(v.data(), hw + L.off[k], L.len[k]);
+    rows_from_msm_form(v);
     return msm_combine(v.data(), p);
   };
   bool want = blind && blind->want;
@@ -413,6 +431,7 @@ static void finish_msms(const g16hip_circuit* C, Slot* S, bool per_msm_events,
     wait(3);
     std::vector<G2X> v(L.len[3] / sizeof(G2X));
     std::memcpy(v.data(), hw + L.off[3], L.len[3]);
+    rows_from_msm_form(v);
     g2_store(out->b_g2, xyzz_to_aff(msm_combine(v.data(), pw)));
   });
   std::exception_ptr err;

@@ -59,10 +59,13 @@ static Xyzz<F> run_msm(g16hip_device* d, int group, const void* bases, const voi
   msm_plan_bind(plan, d->scratch_d.p, n, c);
   const size_t npts = msm_out_points(plan);
   d->scratch_e.alloc(msm_accum_scratch_bytes(plan, group) + sizeof(Xyzz<F>) * npts);
+  const size_t coords = (size_t)n * (group == 1 ? 2 : 4);
+  d->msm_bases.alloc(32 * coords);
   G16_CHECK(hipEventRecord(d->ev[0], d->stream));
+  msm_to_msm_form(bases, d->msm_bases.p, coords, d->stream);
   msm_sort(plan, scalars, d->stream);
   char* wout = static_cast<char*>(d->scratch_e.p) + msm_accum_scratch_bytes(plan, group);
-  msm_accumulate(plan, group, bases, d->scratch_e.p, wout, d->stream, d->ev + 1);
+  msm_accumulate(plan, group, d->msm_bases.p, d->scratch_e.p, wout, d->stream, d->ev + 1);
   std::vector<Xyzz<F>> S(npts);
   uint32_t entries = 0;
   G16_CHECK(hipMemcpyAsync(S.data(), wout, sizeof(Xyzz<F>) * npts, hipMemcpyDeviceToHost,
@@ -80,6 +83,7 @@ static Xyzz<F> run_msm(g16hip_device* d, int group, const void* bases, const voi
   st.c = (uint32_t)c;
   st.windows = (uint32_t)plan.windows;
   note_accumulate(d, group, entries, st.accumulate_ms);
+  rows_from_msm_form(S);
   return msm_combine(S.data(), plan);
 }
 
@@ -118,6 +122,7 @@ void g16hip_close(g16hip_device* d) {
   d->scratch_a.release();
   d->scratch_b.release();
   d->scratch_c.release();
+  d->msm_bases.release();
   d->scratch_d.release();
   d->scratch_e.release();
   for (auto& e : d->ev)
@@ -137,7 +142,7 @@ g16hip_status g16hip_mem_info(g16hip_device* d, size_t* free_b, size_t* total_b)
 g16hip_status g16hip_field_probe(g16hip_device* d, int field, int op, const uint8_t* a,
                                  const uint8_t* b, uint8_t* out, size_t n) {
   return guarded([&] {
-    if (field < 0 || field > 2 || op < 0 || op > 7) throw ArgError{"bad field/op"};
+    if (field < 0 || field > 3 || op < 0 || op > 7) throw ArgError{"bad field/op"};
     if (n == 0) return G16HIP_OK;
     std::lock_guard<std::mutex> lk(d->mu);
     set_device(d);

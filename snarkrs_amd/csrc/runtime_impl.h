@@ -129,6 +129,24 @@ inline Fr fr_from_u64(uint64_t x) {
   return from_std(r);
 }
 
+// Rows come off the MSM kernels in the R' = 2^261 Montgomery form: x_R = x_R' / 32, one
+// R-form product by 2^251 (= 2^256 / 32).
+inline Fq from_msm_form(const Fq& x) {
+  Fq c = fp_zero<FqCfg>();
+  c.v[7] = 0x08000000u;
+  return mul(x, c);
+}
+inline Fq2 from_msm_form(const Fq2& x) { return {from_msm_form(x.c0), from_msm_form(x.c1)}; }
+template <class F>
+void rows_from_msm_form(std::vector<Xyzz<F>>& rows) {
+  for (auto& r : rows) {
+    r.X = from_msm_form(r.X);
+    r.Y = from_msm_form(r.Y);
+    r.ZZ = from_msm_form(r.ZZ);
+    r.ZZZ = from_msm_form(r.ZZZ);
+  }
+}
+
 // The MSM from the reduction's rows (msm_accumulate): sum over windows w and rows j of
 // 2^e P[w][j], e = c w for row 0 and c w + lg + j - 1 for row j >= 1 — one Horner pass over
 // the exponents, one doubling each (the reference finishes sum_w 2^(cw) S_w on the host the
@@ -198,6 +216,7 @@ struct g16hip_device {
   hipStream_t stream = nullptr;
   std::mutex mu;  // serialises the standalone entry points' scratch
   g16::DevBuf scratch_a, scratch_b, scratch_c, scratch_d, scratch_e;
+  g16::DevBuf msm_bases;  // the standalone MSM's bases in the MSM kernels' R' form
   std::map<int, std::unique_ptr<g16::NttTables>> ntt;
   hipEvent_t ev[6] = {};
   g16hip_msm_stats stats = {};

@@ -51,6 +51,31 @@ def test_fq2_ops_match_oracle(dev, orc, op):
     assert (got == want).all(), f"fq2 op {op}"
 
 
+@pytest.mark.parametrize("op", range(8))
+def test_msm_form_fq_ops_match_python(dev, op):
+    # Field 3: Fq in Montgomery form with R' = 2^261 (the MSM kernels' nine-limb product).
+    # Closed forms with Python integers: a value x is stored as x R' mod q.
+    A, B = _pairs(Q_MOD, seed=33)
+    if op == 4:
+        A, B = A[:600], B[:600]
+    got = dev.field_probe(3, op, A, B)
+    a = [int.from_bytes(bytes(x), "little") for x in A]
+    b = [int.from_bytes(bytes(x), "little") for x in B]
+    q, Rp = Q_MOD, 1 << 261
+    Ri = pow(Rp, -1, q)
+    f = {0: lambda x, y: (x + y) % q,
+         1: lambda x, y: (x - y) % q,
+         2: lambda x, y: x * y * Ri % q,
+         3: lambda x, y: x * x * Ri % q,
+         4: lambda x, y: (Rp * Rp * pow(x, -1, q)) % q if x else 0,
+         5: lambda x, y: (-x) % q,
+         6: lambda x, y: x * Rp % q,
+         7: lambda x, y: x * Ri % q}[op]
+    want = _limbs([f(x, y) for x, y in zip(a, b)])
+    bad = np.nonzero((got != want).any(axis=1))[0]
+    assert len(bad) == 0, f"op {op}: first mismatch at {bad[:5]}"
+
+
 @pytest.mark.parametrize("group", [1, 2])
 def test_fixed_base_matches_oracle(dev, orc, group):
     rng = SplitMix64(40 + group)
