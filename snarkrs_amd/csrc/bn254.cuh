@@ -225,30 +225,47 @@ HD Fp<C> mul(const Fp<C>& a, const Fp<C>& b) {
   return r;
 }
 
-// Montgomery product for R' = 2^261 (FqmCfg): CIOS over nine 29-bit limbs with 64-bit column
+// Montgomery product for R' = 2^261: CIOS over nine 29-bit limbs with 64-bit column
 // accumulators. Every partial product is one 29x29+64 multiply-add; the columns take the
 // carries lazily (each stays below 2^63: at most 18 products of 2^58 land in one column), so
 // the loop has no carry flags at all, which the 32-bit form above needs on every step. On
 // gfx950 this runs 1.42x the 32-bit form's products per second. Inputs and output are the
-// usual 8 x 32-bit limbs, fully reduced.
+// usual 8 x 32-bit limbs, the output fully reduced. M29 gives the modulus in 29-bit limbs and
+// -modulus^-1 mod 2^29.
 HD uint32_t limb29(const uint32_t v[8], int k) {
   const int o = 29 * k, w = o >> 5, sh = o & 31;
   const uint64_t pair = (uint64_t)v[w] | ((uint64_t)(w + 1 < 8 ? v[w + 1] : 0u) << 32);
   return (uint32_t)(pair >> sh) & ((1u << 29) - 1u);
 }
 
-template <>
-HD Fp<FqmCfg> mul<FqmCfg>(const Fp<FqmCfg>& a, const Fp<FqmCfg>& b) {
-  constexpr uint32_t MASK = (1u << 29) - 1u;
-  // q in 29-bit limbs and -q^-1 mod 2^29.
-  constexpr uint32_t P29[9] = {0x187cfd47u, 0x010460b6u, 0x1c72a34fu, 0x02d522d0u, 0x1585d978u,
+struct Fq29 {
+  using Cfg = FqCfg;
+  static HD uint32_t p29(int j) {
+    constexpr uint32_t P[9] = {0x187cfd47u, 0x010460b6u, 0x1c72a34fu, 0x02d522d0u, 0x1585d978u,
                                0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu};
-  constexpr uint32_t N29 = 0x04866389u;
+    return P[j];
+  }
+  static constexpr uint32_t N29 = 0x04866389u;
+};
+
+struct Fr29 {
+  using Cfg = FrCfg;
+  static HD uint32_t p29(int j) {
+    constexpr uint32_t P[9] = {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u,
+                               0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu};
+    return P[j];
+  }
+  static constexpr uint32_t N29 = 0x0fffffffu;
+};
+
+template <class M29>
+HD void mont29(const uint32_t a[8], const uint32_t b[8], uint32_t out[8]) {
+  constexpr uint32_t MASK = (1u << 29) - 1u;
   uint32_t A[9], B[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
-    A[k] = limb29(a.v, k);
-    B[k] = limb29(b.v, k);
+    A[k] = limb29(a, k);
+    B[k] = limb29(b, k);
   }
   uint64_t t[10];
 #pragma unroll
@@ -257,16 +274,15 @@ HD Fp<FqmCfg> mul<FqmCfg>(const Fp<FqmCfg>& a, const Fp<FqmCfg>& b) {
   for (int i = 0; i < 9; ++i) {
 #pragma unroll
     for (int j = 0; j < 9; ++j) t[j] += (uint64_t)A[j] * B[i];
-    const uint32_t m = (uint32_t)((uint64_t)((uint32_t)t[0] & MASK) * N29) & MASK;
+    const uint32_t m = (uint32_t)((uint64_t)((uint32_t)t[0] & MASK) * M29::N29) & MASK;
 #pragma unroll
-    for (int j = 0; j < 9; ++j) t[j] += (uint64_t)m * P29[j];
+    for (int j = 0; j < 9; ++j) t[j] += (uint64_t)m * M29::p29(j);
     t[1] += t[0] >> 29;  // t[0] is now a multiple of 2^29
 #pragma unroll
     for (int j = 0; j < 9; ++j) t[j] = t[j + 1];
     t[9] = 0;
   }
-  // Carry the columns into 29-bit limbs and repack as 8 x 32 bits (< 2q < 2^256).
-  Fp<FqmCfg> r;
+  // Carry the columns into 29-bit limbs and repack as 8 x 32 bits (< 2 modulus < 2^256).
   uint64_t c = 0, acc = 0;
   int bits = 0, w = 0;
 #pragma unroll
@@ -276,14 +292,29 @@ HD Fp<FqmCfg> mul<FqmCfg>(const Fp<FqmCfg>& a, const Fp<FqmCfg>& b) {
     acc |= (uint64_t)((uint32_t)x & MASK) << bits;
     bits += 29;
     if (bits >= 32) {
-      if (w < 8) r.v[w] = (uint32_t)acc;
+      if (w < 8) out[w] = (uint32_t)acc;
       ++w;
       acc >>= 32;
       bits -= 32;
     }
   }
-  if (w < 8) r.v[w] = (uint32_t)acc;
-  reduce_once<FqmCfg>(r.v);
+  if (w < 8) out[w] = (uint32_t)acc;
+  reduce_once<typename M29::Cfg>(out);
+}
+
+template <>
+HD Fp<FqmCfg> mul<FqmCfg>(const Fp<FqmCfg>& a, const Fp<FqmCfg>& b) {
+  Fp<FqmCfg> r;
+  mont29<Fq29>(a.v, b.v, r.v);
+  return r;
+}
+
+// x * w for x in the usual R = 2^256 form and w stored in the R' = 2^261 form (w 2^261 mod r):
+// the 29-bit product divides by 2^261, so the result is x w in the R form. The NTT keeps its
+// twiddle and coset tables in the R' form and multiplies data by them this way.
+HD Fr mul_rp(const Fr& x, const Fr& w_rp) {
+  Fr r;
+  mont29<Fr29>(x.v, w_rp.v, r.v);
   return r;
 }
 

@@ -31,19 +31,25 @@ void launch_valu_peak(int kind, const void* in, void* out, unsigned blocks, int 
                       hipStream_t s);
 
 // ---- NTT kernels (kernels_ntt.hip) ----
-// Twiddle tables: fwd[i] = w^i, inv[i] = w^-i for i < n/2 (Montgomery Fr).
+// out[i] = base^i for i < n (Montgomery Fr).
 void launch_powers(void* out, const void* base, size_t n, hipStream_t s);
+// Per-pass twiddle tables from powers w^i (i < n/2): entry 2^s - 1 + j = w^(j 2^(L-1-s)),
+// in the R' = 2^261 form the NTT multiplies with (bn254.cuh mul_rp).
+void launch_stage_tables(void* out, const void* powers, int log_n, hipStream_t s);
 void launch_coset_table(void* out, const void* shift, const void* scale, int log_n,
                         hipStream_t s);
-// Transforms of `nvec` vectors stored back to back (stride n elements), in place.
+void launch_fr_to_rp(void* data, size_t n, hipStream_t s);  // R form -> R' form, in place
+// Transforms of `nvec` vectors stored back to back (stride n elements), in place, with
+// per-pass twiddle tables (launch_stage_tables).
 // dif_inverse: natural order in -> bit-reversed out, unscaled (w^-1 twiddles).
 // dit_forward: bit-reversed in -> natural out; if `prescale` is given, element p is first
-// multiplied by prescale[p] (the coset table in bit-reversed order).
+// multiplied by prescale[p] (the coset table in bit-reversed order, R' form).
 void ntt_dif_inverse(void* data, int nvec, int log_n, const void* tw_inv, hipStream_t s);
 void ntt_dit_forward(void* data, int nvec, int log_n, const void* tw_fwd, const void* prescale,
                      hipStream_t s);
 void launch_bitrev_permute(void* dst, const void* src, int log_n, hipStream_t s);
 void launch_scale(void* data, size_t n, const void* factor, hipStream_t s);
+void launch_scale_rp(void* data, size_t n, const void* factor_rp, hipStream_t s);
 
 // ---- prove stages (kernels_prove.hip) ----
 void launch_gather(const uint32_t* row_ptr, const uint32_t* signal, const uint32_t* value_id,
@@ -92,5 +98,7 @@ void msm_reduce_shape(const MsmPlan& p, int* rows, int* lg);
 void msm_to_msm_form(const void* src, void* dst, size_t n_fq, hipStream_t s);
 size_t msm_out_points(const MsmPlan& p);
 int msm_default_c(uint32_t n);
+// Width for scalars sorted once for g1 G1 MSMs and g2 G2 MSMs.
+int msm_choose_c(uint32_t n, int g1, int g2);
 
 }  // namespace g16

@@ -210,15 +210,19 @@ __global__ void k_scan_fix(uint32_t* offset, const uint32_t* bsum, uint32_t K,
   }
 }
 
-int msm_default_c(uint32_t n) {
-  // The reference's cost model (crates/msm/src/lib.rs:160-172) weighs a bucket twice a
-  // point; on the GPU the bucket reduction is latency-bound, so buckets are priced higher
-  // (4 per point) and the window is capped at 16.
+int msm_choose_c(uint32_t n, int g1, int g2) {
+  // The reference's cost model (crates/msm/src/lib.rs:160-172) prices a window at
+  // n + 2 * buckets points. Here the weights are fitted to this build's measurements on gfx950
+  // (sweeps of c at 2^20 and 2^21 points and of the prove's two widths): a G2 entry takes
+  // about 3 G1 entries' time, and a bucket's share of the merges and the window reduction
+  // about 6 entries of its group (G1) or 8 (G2). Scalars sorted once for g1 G1 and g2 G2 MSMs
+  // share one width; it is capped at 16 (the LDS counting sort).
   int best = 3;
   double best_cost = 1e300;
   for (int c = 3; c <= 16; ++c) {
     double windows = (RECODE_BITS + c - 1) / c;
-    double cost = windows * ((double)n + 4.0 * (double)(1u << (c - 1)));
+    double nb = (double)(1u << (c - 1));
+    double cost = windows * (g1 * ((double)n + 6.0 * nb) + 3.0 * g2 * ((double)n + 8.0 * nb));
     if (cost < best_cost) {
       best_cost = cost;
       best = c;
@@ -226,6 +230,8 @@ int msm_default_c(uint32_t n) {
   }
   return best;
 }
+
+int msm_default_c(uint32_t n) { return msm_choose_c(n, 1, 0); }
 
 static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 

@@ -292,7 +292,10 @@ __global__ void __launch_bounds__(BLOCK) k_reduce_rows(const uint4* T, const uin
   }
   sh[threadIdx.x] = acc;
   block_tree<F, BLOCK>(sh);
-  if (threadIdx.x == 0) st_xyzz(out, blockIdx.x, sh[0]);
+  if (threadIdx.x == 0) {
+    st_xyzz(out, blockIdx.x, sh[0]);
+    __threadfence_system();  // `out` may be pinned host memory the host reads next
+  }
 }
 
 inline uint32_t reduce_group_size(const MsmPlan& p) { return p.nb < 8 ? p.nb : 8; }

@@ -28,11 +28,14 @@ struct Slot {
   hipStream_t side[2] = {nullptr, nullptr};
   hipEvent_t ev[8] = {};
   hipEvent_t sync_ev[4] = {};  // witness ready, witness sorted, side[0] done, side[1] done
-  hipEvent_t done_ev[5] = {};  // window sums of A, B1, L, B2, H copied to host_wout
+  hipEvent_t done_ev[5] = {};  // reduction rows of A, B1, L, B2, H written to host_wout
   hipEvent_t mev[5][4] = {};   // per MSM: before accumulation, after it, after merges, after reduction
-  DevBuf witness, abc, hbuf, sort_w, sort_h, accum_w1, accum_w2, accum_h, wout;
+  DevBuf witness, abc, hbuf, sort_w, sort_h, accum_w1, accum_w2, accum_h;
   MsmPlan plan_w, plan_h;
+  // Pinned, device-mapped host memory: the reduction kernels write each MSM's rows straight
+  // here (no copy on the stream, which would wait for CUs behind other streams' kernels).
   uint8_t* host_wout = nullptr;
+  uint8_t* host_wout_dev = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   size_t bytes = 0;
@@ -156,8 +159,12 @@ static void upload_key(g16hip_circuit* C, const g16hip_key* k, hipStream_t s) {
   msm_to_msm_form(C->h_q.p, C->h_q.p, 2 * (size_t)n, s);
   msm_to_msm_form(C->b2_q.p, C->b2_q.p, 4 * nv, s);
   build_ntt_tables(C->tables, C->log_n, s);
-  C->cw = msm_default_c((uint32_t)nv);
+  // The witness digits feed A, B1, L (G1) and B2 (G2); H feeds one G1 MSM.
+  // G16HIP_C_WITNESS / G16HIP_C_H override the widths (2..16).
+  C->cw = msm_choose_c((uint32_t)nv, 3, 1);
   C->ch = msm_default_c(n);
+  if (const char* e = std::getenv("G16HIP_C_WITNESS")) C->cw = std::max(2, std::min(16, std::atoi(e)));
+  if (const char* e = std::getenv("G16HIP_C_H")) C->ch = std::max(2, std::min(16, std::atoi(e)));
   G16_CHECK(hipStreamSynchronize(s));
   DevBuf* bufs[] = {&C->row_ptr[0], &C->row_ptr[1], &C->signal[0], &C->signal[1],
                     &C->value_id[0], &C->value_id[1], &C->values, &C->a_q, &C->b1_q, &C->b2_q,
@@ -206,10 +213,11 @@ static void size_slot(const g16hip_circuit* C, Slot* S) {
   S->accum_w2.alloc(msm_accum_scratch_bytes(S->plan_w, 2));
   S->accum_h.alloc(msm_accum_scratch_bytes(S->plan_h, 1));
   size_t wbytes = 256 * (4 * msm_out_points(S->plan_w) + msm_out_points(S->plan_h));
-  S->wout.alloc(wbytes);
-  G16_CHECK(hipHostMalloc((void**)&S->host_wout, wbytes + 64, hipHostMallocDefault));
+  G16_CHECK(hipHostMalloc((void**)&S->host_wout, wbytes + 64,
+                          hipHostMallocMapped | hipHostMallocCoherent));
+  G16_CHECK(hipHostGetDevicePointer((void**)&S->host_wout_dev, S->host_wout, 0));
   DevBuf* bufs[] = {&S->witness, &S->abc, &S->hbuf, &S->sort_w, &S->sort_h, &S->accum_w1,
-                    &S->accum_w2, &S->accum_h, &S->wout};
+                    &S->accum_w2, &S->accum_h};
   S->bytes = 0;
   for (auto b : bufs) S->bytes += b->bytes;
 }
@@ -340,19 +348,18 @@ static WinLayout win_layout(const Slot* S) {
 
 // Enqueues stages (0-4 if `with_h`) 5-9 on the slot's three streams: side[0] sorts the
 // witness digits once and accumulates A, B1 and L; side[1] accumulates B2 on the same
-// entries; the main stream runs stages 0-4 and then the H sort and MSM. Each MSM's window
-// sums go to pinned host memory on its own stream. With `timed`, events bracket every phase
-// (and done_ev[k] marks MSM k's sums on the host); a graph capture passes timed = false.
+// entries; the main stream runs stages 0-4 and then the H sort and MSM. Each MSM's reduction
+// rows are written by its kernel into pinned host memory. With `timed`, events bracket every
+// phase (done_ev[k] marks MSM k's rows on the host) and the entry counts are copied out at
+// the end for the kernel statistics; a graph capture passes timed = false.
 static void enqueue_stages(const g16hip_circuit* C, Slot* S, bool with_h, void* h, bool timed) {
   hipStream_t s0 = S->stream, s1 = S->side[0], s2 = S->side[1];
   MsmPlan& pw = S->plan_w;
   MsmPlan& ph = S->plan_h;
-  char* wout = S->wout.as<char>();
-  uint8_t* hw = S->host_wout;
+  uint8_t* wout = S->host_wout_dev;
   WinLayout L = win_layout(S);
-  uint32_t* totals = reinterpret_cast<uint32_t*>(hw + L.totals);
-  auto sums_to_host = [&](int k, hipStream_t st) {
-    G16_CHECK(hipMemcpyAsync(hw + L.off[k], wout + L.off[k], L.len[k], hipMemcpyDeviceToHost, st));
+  uint32_t* totals = reinterpret_cast<uint32_t*>(S->host_wout + L.totals);
+  auto rows_done = [&](int k, hipStream_t st) {
     if (timed) G16_CHECK(hipEventRecord(S->done_ev[k], st));
   };
   hipEvent_t* mev0 = timed ? S->mev[0] : nullptr;
@@ -369,24 +376,26 @@ static void enqueue_stages(const g16hip_circuit* C, Slot* S, bool with_h, void* 
   // end up sharing a hardware queue.
   G16_CHECK(hipStreamWaitEvent(s2, S->sync_ev[1], 0));
   msm_accumulate(pw, 2, C->b2_q.p, S->accum_w2.p, wout + L.off[3], s2, mev3);
-  sums_to_host(3, s2);
+  rows_done(3, s2);
   G16_CHECK(hipEventRecord(S->sync_ev[3], s2));
-  G16_CHECK(hipMemcpyAsync(&totals[0], pw.total, 4, hipMemcpyDeviceToHost, s1));
   msm_accumulate(pw, 1, C->a_q.p, S->accum_w1.p, wout + L.off[0], s1, mev0);
-  sums_to_host(0, s1);
+  rows_done(0, s1);
   msm_accumulate(pw, 1, C->b1_q.p, S->accum_w1.p, wout + L.off[1], s1, mev1);
-  sums_to_host(1, s1);
+  rows_done(1, s1);
   msm_accumulate(pw, 1, C->l_q.p, S->accum_w1.p, wout + L.off[2], s1, mev2);
-  sums_to_host(2, s1);
+  rows_done(2, s1);
   G16_CHECK(hipEventRecord(S->sync_ev[2], s1));
   if (with_h) stages_0_4(C, S, h, timed);
   msm_sort(ph, h, s0);
-  G16_CHECK(hipMemcpyAsync(&totals[1], ph.total, 4, hipMemcpyDeviceToHost, s0));
   msm_accumulate(ph, 1, C->h_q.p, S->accum_h.p, wout + L.off[4], s0, mev4);
-  sums_to_host(4, s0);
+  rows_done(4, s0);
   G16_CHECK(hipStreamWaitEvent(s0, S->sync_ev[2], 0));
   G16_CHECK(hipStreamWaitEvent(s0, S->sync_ev[3], 0));
-  if (timed) G16_CHECK(hipEventRecord(S->ev[5], s0));
+  if (timed) {
+    G16_CHECK(hipEventRecord(S->ev[5], s0));
+    G16_CHECK(hipMemcpyAsync(&totals[0], pw.total, 4, hipMemcpyDeviceToHost, s0));
+    G16_CHECK(hipMemcpyAsync(&totals[1], ph.total, 4, hipMemcpyDeviceToHost, s0));
+  }
 }
 
 // The stage-11 products that need an MSM output: s*A and r*B1 (prove.rs:391-430 groups the

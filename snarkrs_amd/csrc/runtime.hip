@@ -29,13 +29,19 @@ void build_ntt_tables(NttTables& t, int L, hipStream_t s) {
   host[4] = fp_one<FrCfg>();
   host[5] = fp_zero<FrCfg>();
   upload(t.scalars, host, sizeof host, s);
-  t.fwd.alloc(32 * half);
-  t.inv.alloc(32 * half);
+  t.fwd.alloc(32 * (n > 1 ? n - 1 : 1));
+  t.inv.alloc(32 * (n > 1 ? n - 1 : 1));
   t.coset.alloc(32 * n);
   Fr* sc = t.scalars.as<Fr>();
-  launch_powers(t.fwd.p, sc + 0, half, s);
-  launch_powers(t.inv.p, sc + 1, half, s);
+  DevBuf powers;
+  powers.alloc(32 * half);
+  launch_powers(powers.p, sc + 0, half, s);
+  launch_stage_tables(t.fwd.p, powers.p, L, s);
+  launch_powers(powers.p, sc + 1, half, s);
+  launch_stage_tables(t.inv.p, powers.p, L, s);
   launch_coset_table(t.coset.p, sc + 2, sc + 3, L, s);
+  launch_fr_to_rp(t.coset.p, n, s);
+  G16_CHECK(hipStreamSynchronize(s));  // before `powers` is freed
 }
 
 NttTables& ntt_tables_for(g16hip_device* d, int L) {

@@ -205,7 +205,9 @@ struct HostTask {
 
 // ---- devices ---------------------------------------------------------------------------
 struct NttTables {
-  DevBuf fwd, inv, coset, scalars;  // scalars: [w, w^-1, shift, 1/n, 1, 0]
+  // fwd, inv: per-pass twiddle tables (2^L - 1 entries); coset: shift^bitrev(p) / n; both in
+  // the R' = 2^261 form. scalars (R form): [w, w^-1, shift, 1/n, 1, 0].
+  DevBuf fwd, inv, coset, scalars;
 };
 void build_ntt_tables(NttTables& t, int L, hipStream_t s);
 
