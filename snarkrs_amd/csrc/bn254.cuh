@@ -471,21 +471,40 @@ HD Xyzz<F> xyzz_from_aff(const Aff<F>& a) {
   return {a.x, a.y, f_one<F>(), f_one<F>()};
 }
 
+// The point formulas below are ordered to keep few values live (each product consumed as
+// early as the formula allows) and, on the device, fenced with scheduling barriers (SB) so
+// the compiler does not run several products side by side: a G2 addition otherwise holds more
+// than 256 registers per lane and takes a whole SIMD.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define SB() __builtin_amdgcn_sched_barrier(0)
+#else
+#define SB() ((void)0)
+#endif
+
 // dbl-2008-s-1 (a = 0).
 template <class F>
 HD Xyzz<F> xyzz_dbl(const Xyzz<F>& p) {
   if (xyzz_is_inf(p)) return p;
   F U = dbl(p.Y);
   F V = sqr(U);
+  SB();
   F W = mul(U, V);
+  SB();
   F S = mul(p.X, V);
-  F X2 = sqr(p.X);
-  F M = add(dbl(X2), X2);
+  SB();
   Xyzz<F> r;
-  r.X = sub(sqr(M), dbl(S));
-  r.Y = sub(mul(M, sub(S, r.X)), mul(W, p.Y));
   r.ZZ = mul(V, p.ZZ);
+  SB();
+  F X2 = sqr(p.X);
+  SB();
+  F M = add(dbl(X2), X2);
+  F WY = mul(W, p.Y);
+  SB();
   r.ZZZ = mul(W, p.ZZZ);
+  SB();
+  r.X = sub(sqr(M), dbl(S));
+  SB();
+  r.Y = sub(mul(M, sub(S, r.X)), WY);
   return r;
 }
 
@@ -494,13 +513,20 @@ template <class F>
 HD Xyzz<F> xyzz_dbl_aff(const Aff<F>& a) {
   F U = dbl(a.y);
   F V = sqr(U);
+  SB();
   F W = mul(U, V);
+  SB();
   F S = mul(a.x, V);
+  SB();
   F X2 = sqr(a.x);
+  SB();
   F M = add(dbl(X2), X2);
+  F WY = mul(W, a.y);
+  SB();
   Xyzz<F> r;
   r.X = sub(sqr(M), dbl(S));
-  r.Y = sub(mul(M, sub(S, r.X)), mul(W, a.y));
+  SB();
+  r.Y = sub(mul(M, sub(S, r.X)), WY);
   r.ZZ = V;
   r.ZZZ = W;
   return r;
@@ -511,48 +537,71 @@ template <class F>
 HD Xyzz<F> xyzz_add_aff(const Xyzz<F>& p, const Aff<F>& a) {
   if (aff_is_inf(a)) return p;
   if (xyzz_is_inf(p)) return {a.x, a.y, f_one<F>(), f_one<F>()};
-  F U2 = mul(a.x, p.ZZ);
-  F S2 = mul(a.y, p.ZZZ);
-  F P = sub(U2, p.X);
-  F R = sub(S2, p.Y);
+  F P = sub(mul(a.x, p.ZZ), p.X);
+  SB();
+  F R = sub(mul(a.y, p.ZZZ), p.Y);
+  SB();
   if (is_zero(P)) {
     if (is_zero(R)) return xyzz_dbl_aff(a);
     return xyzz_inf<F>();
   }
   F PP = sqr(P);
+  SB();
   F PPP = mul(P, PP);
-  F Q = mul(p.X, PP);
+  SB();
   Xyzz<F> r;
-  r.X = sub(sub(sqr(R), PPP), dbl(Q));
-  r.Y = sub(mul(R, sub(Q, r.X)), mul(p.Y, PPP));
   r.ZZ = mul(p.ZZ, PP);
+  SB();
+  F Q = mul(p.X, PP);
+  SB();
   r.ZZZ = mul(p.ZZZ, PPP);
+  SB();
+  F YP = mul(p.Y, PPP);
+  SB();
+  r.X = sub(sub(sqr(R), PPP), dbl(Q));
+  SB();
+  r.Y = sub(mul(R, sub(Q, r.X)), YP);
   return r;
 }
 
-// add-2008-s: p + q, general.
+// add-2008-s: p + q, general. The inputs die after the first six products: the doubling case
+// (p = q) doubles (U1, S1, ZZ1 ZZ2, ZZZ1 ZZZ2), which is p in other coordinates.
 template <class F>
 HD Xyzz<F> xyzz_add(const Xyzz<F>& p, const Xyzz<F>& q) {
   if (xyzz_is_inf(p)) return q;
   if (xyzz_is_inf(q)) return p;
   F U1 = mul(p.X, q.ZZ);
-  F U2 = mul(q.X, p.ZZ);
+  SB();
+  F P = sub(mul(q.X, p.ZZ), U1);
+  SB();
   F S1 = mul(p.Y, q.ZZZ);
-  F S2 = mul(q.Y, p.ZZZ);
-  F P = sub(U2, U1);
-  F R = sub(S2, S1);
+  SB();
+  F R = sub(mul(q.Y, p.ZZZ), S1);
+  SB();
+  F ZZ12 = mul(p.ZZ, q.ZZ);
+  SB();
+  F ZZZ12 = mul(p.ZZZ, q.ZZZ);
+  SB();
   if (is_zero(P)) {
-    if (is_zero(R)) return xyzz_dbl(p);
+    if (is_zero(R)) return xyzz_dbl(Xyzz<F>{U1, S1, ZZ12, ZZZ12});
     return xyzz_inf<F>();
   }
   F PP = sqr(P);
+  SB();
   F PPP = mul(P, PP);
-  F Q = mul(U1, PP);
+  SB();
   Xyzz<F> r;
+  r.ZZ = mul(ZZ12, PP);
+  SB();
+  F Q = mul(U1, PP);
+  SB();
+  r.ZZZ = mul(ZZZ12, PPP);
+  SB();
+  F SP = mul(S1, PPP);
+  SB();
   r.X = sub(sub(sqr(R), PPP), dbl(Q));
-  r.Y = sub(mul(R, sub(Q, r.X)), mul(S1, PPP));
-  r.ZZ = mul(mul(p.ZZ, q.ZZ), PP);
-  r.ZZZ = mul(mul(p.ZZZ, q.ZZZ), PPP);
+  SB();
+  r.Y = sub(mul(R, sub(Q, r.X)), SP);
   return r;
 }
 

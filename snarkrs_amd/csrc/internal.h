@@ -71,8 +71,8 @@ struct MsmPlan {
   uint32_t* count = nullptr;    // [windows*nb + 1] entries per bucket
   uint32_t* offset = nullptr;   // [windows*nb + 1] first entry of each bucket; [K] = total
   uint32_t* block_sums = nullptr;
-  uint32_t* keys = nullptr;     // [max_entries]
-  uint32_t* vals = nullptr;     // [max_entries]  (index << 1 | sign)
+  uint2* ent = nullptr;         // [max_entries] sorted entries: {key = window * nb + bucket,
+                                //                 value = index << 1 | sign}
   uint32_t* digits = nullptr;   // [windows * n] encoded signed digits, window-major
   uint32_t* chunk_count = nullptr;  // [windows][chunks][nb] per-chunk counts, then prefixes
   uint32_t* total = nullptr;    // [4] entry count

@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(256) k_chunk_scan(uint32_t* chunk_count, uint3
 // cursor.
 __global__ void __launch_bounds__(SORT_THREADS) k_chunk_scatter(
     const uint32_t* digits, uint32_t n, uint32_t nb, uint32_t chunks, uint32_t span,
-    const uint32_t* chunk_count, const uint32_t* offset, uint32_t* keys, uint32_t* vals) {
+    const uint32_t* chunk_count, const uint32_t* offset, uint2* ent) {
   extern __shared__ uint32_t cur[];  // span words
   uint32_t p = blockIdx.x, w = blockIdx.y, b0 = blockIdx.z * span;
   const uint32_t* cc = chunk_count + ((size_t)w * chunks + p) * nb + b0;
@@ -150,8 +150,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_chunk_scatter(
     uint32_t b = (e & 0xFFFFu) - b0;
     if (b >= span) continue;
     uint32_t pos = atomicAdd(&cur[b], 1u);
-    keys[pos] = w * nb + b0 + b;
-    vals[pos] = ((uint32_t)i << 1) | ((e >> 16) & 1u);
+    ent[pos] = make_uint2(w * nb + b0 + b, ((uint32_t)i << 1) | ((e >> 16) & 1u));
   }
 }
 
@@ -242,7 +241,7 @@ size_t msm_sort_scratch_bytes(uint32_t n, int c) {
   size_t nb = (size_t)1 << (c - 1);
   size_t K = (size_t)W * nb;
   size_t E = (size_t)n * W;
-  return align_up(4 * (K + 1)) * 2 + align_up(4 * 1025) + align_up(4 * E) * 3 +
+  return align_up(4 * (K + 1)) * 2 + align_up(4 * 1025) + align_up(8 * E) + align_up(4 * E) +
          align_up(4 * K * sort_chunks(n)) + align_up(16);
 }
 
@@ -268,8 +267,7 @@ void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c) {
   p.count = reinterpret_cast<uint32_t*>(take(4 * (K + 1)));
   p.offset = reinterpret_cast<uint32_t*>(take(4 * (K + 1)));
   p.block_sums = reinterpret_cast<uint32_t*>(take(4 * 1025));
-  p.keys = reinterpret_cast<uint32_t*>(take(4 * p.max_entries));
-  p.vals = reinterpret_cast<uint32_t*>(take(4 * p.max_entries));
+  p.ent = reinterpret_cast<uint2*>(take(8 * p.max_entries));
   p.digits = reinterpret_cast<uint32_t*>(take(4 * p.max_entries));
   p.chunk_count = reinterpret_cast<uint32_t*>(take(4 * K * p.chunks));
   p.total = reinterpret_cast<uint32_t*>(take(16));
@@ -301,7 +299,7 @@ void msm_sort(MsmPlan& p, const void* scalars, hipStream_t s) {
     uint32_t span = p.nb < SCATTER_SPAN ? p.nb : SCATTER_SPAN;
     hipLaunchKernelGGL(k_chunk_scatter, dim3(p.chunks, p.windows, p.nb / span),
                        dim3(SORT_THREADS), 4 * span, s, p.digits, p.n, p.nb, p.chunks, span,
-                       p.chunk_count, p.offset, p.keys, p.vals);
+                       p.chunk_count, p.offset, p.ent);
     G16_LAUNCH_CHECK("msm chunk scatter");
   }
 }

@@ -1,18 +1,13 @@
 // Number-theoretic transforms over Fr for stages 1-3 (crates/ntt/src/lib.rs:429-550).
 //
 // Layout: each vector is n = 2^L Fr elements, 32 bytes each (8 Montgomery u32 limbs),
-// several vectors back to back. A launch performs up to R = 4 radix-2 passes on
-// independent 2^R-point sub-transforms held in registers: for passes s0 .. s0+R-1 the
-// element indices i = hi * 2^(s0+R) + k * 2^s0 + lo (k < 2^R) interact only among
-// themselves, so thread (hi, lo) loads its 2^R elements, runs the butterflies and stores
-// them back. Consecutive threads take consecutive lo, so each of the 2^R loads of a wave is
-// a contiguous run whenever 2^s0 >= 64. Twiddles come from per-pass tables in HBM: pass s
-// reads T_s[j] = w^(j 2^(L-1-s)) for j < 2^s, stored contiguously at offset 2^s - 1, so the
-// twiddle loads of a wave are as contiguous as its data loads.
+// several vectors back to back, transformed in place by LDS-tiled passes of several radix-2
+// stages each (k_ntt_pass below). Stage s reads twiddles T_s[j] = w^(j 2^(L-1-s)), j < 2^s,
+// from per-pass tables stored contiguously at offset 2^s - 1.
 //
-//   DIF (natural -> bit-reversed), passes from s = L-1 down to 0:
+//   DIF (natural -> bit-reversed), stages from s = L-1 down to 0:
 //       (u, v) -> (u + v, (u - v) * T_s[j])
-//   DIT (bit-reversed -> natural), passes from s = 0 up to L-1:
+//   DIT (bit-reversed -> natural), stages from s = 0 up to L-1:
 //       (u, v) -> (u + v * T_s[j], u - v * T_s[j])
 // The tables (twiddles and the coset table) are kept in the Montgomery form R' = 2^261, and
 // data are multiplied by them with the 29-bit product (mul_rp): data stay in the R = 2^256
@@ -20,7 +15,7 @@
 // With T = inverse twiddles the DIF is the unscaled iNTT; with forward twiddles the DIT is
 // the forward NTT. The prover's pipeline iNTT -> coset shift -> NTT needs no permutation:
 // the coset table is stored in bit-reversed order and multiplied in as the forward DIT loads
-// its first group (crates/ntt/src/lib.rs:419-470 uses the same cancellation).
+// first pass (crates/ntt/src/lib.rs:419-470 uses the same cancellation).
 #include "bn254.cuh"
 #include "internal.h"
 
@@ -36,107 +31,101 @@ __device__ __forceinline__ void stfr(uint4* p, const Fr& x) {
   p[1] = make_uint4(x.v[4], x.v[5], x.v[6], x.v[7]);
 }
 
-// The 2^R elements of a thread stay in registers only while every index into x[] is a
-// compile-time constant; full unrolling of loops over field products exceeds the compiler's
-// unroll budget (the array then lands in scratch memory), so the butterflies, loads and stores
-// below are expanded by template recursion instead of loops.
-template <int R, int M, int K, bool DIF>
-__device__ __forceinline__ void ntt_pass(Fr (&x)[1 << R], const uint4* tw, int s0, uint32_t lo) {
-  if constexpr (K < (1 << R)) {
-    if constexpr ((K & (1 << M)) == 0) {
-      constexpr int K1 = K | (1 << M);
-      const int s = s0 + M;
-      const uint32_t j = ((uint32_t)(K & ((1 << M) - 1)) << s0) + lo;
-      const Fr w = ldfr(tw + 2 * (((size_t)1 << s) - 1 + j));
-      if constexpr (DIF) {
-        const Fr u = x[K], d = x[K1];
-        x[K] = add(u, d);
-        x[K1] = mul_rp(sub(u, d), w);
-      } else {
-        const Fr u = x[K], d = mul_rp(x[K1], w);
-        x[K] = add(u, d);
-        x[K1] = sub(u, d);
-      }
-    }
-    ntt_pass<R, M, K + 1, DIF>(x, tw, s0, lo);
-  }
-}
+// One pass = B consecutive radix-2 stages s0 .. s0+B-1, done in LDS. For those stages the
+// elements i = hi 2^(s0+B) + k 2^s0 + lo (k < 2^B) interact only among themselves; a
+// workgroup takes one hi and TLO consecutive lo (2^B TLO <= 2048 elements, 64 KB of LDS held
+// as eight 32-bit planes, so neighbouring lanes hit neighbouring banks), loads them with
+// consecutive lanes on consecutive lo, runs the B stages with a barrier between stages, and
+// stores them back. The pass over the lowest stages (s0 = 0) uses TLO = 1 and reads 2^11
+// contiguous elements; higher passes use B <= 6, so every load is a run of >= 32 elements.
+static constexpr int NTT_TILE = 2048;  // elements per workgroup
+static constexpr int NTT_THREADS = 256;
 
-// DIF runs passes M = R-1 down to 0, DIT 0 up to R-1.
-template <int R, int M, bool DIF>
-__device__ __forceinline__ void ntt_passes(Fr (&x)[1 << R], const uint4* tw, int s0,
-                                           uint32_t lo) {
-  if constexpr (M >= 0 && M < R) {
-    ntt_pass<R, M, 0, DIF>(x, tw, s0, lo);
-    ntt_passes<R, DIF ? M - 1 : M + 1, DIF>(x, tw, s0, lo);
-  }
+__device__ __forceinline__ void lds_put(uint32_t* lds, int e, const Fr& x) {
+#pragma unroll
+  for (int l = 0; l < 8; ++l) lds[l * NTT_TILE + e] = x.v[l];
 }
-
-template <int R, int K>
-__device__ __forceinline__ void ntt_load(Fr (&x)[1 << R], const uint4* v, size_t base, int s0,
-                                         const uint4* prescale) {
-  if constexpr (K < (1 << R)) {
-    const size_t idx = base + ((size_t)K << s0);
-    x[K] = ldfr(v + 2 * idx);
-    if (prescale) x[K] = mul_rp(x[K], ldfr(prescale + 2 * idx));
-    ntt_load<R, K + 1>(x, v, base, s0, prescale);
-  }
-}
-
-template <int R, int K>
-__device__ __forceinline__ void ntt_store(const Fr (&x)[1 << R], uint4* v, size_t base, int s0) {
-  if constexpr (K < (1 << R)) {
-    stfr(v + 2 * (base + ((size_t)K << s0)), x[K]);
-    ntt_store<R, K + 1>(x, v, base, s0);
-  }
-}
-
-template <int R, bool DIF>
-__global__ void __launch_bounds__(256) k_ntt_group(uint4* data, int L, int s0,
-                                                   const uint4* tw, const uint4* prescale,
-                                                   uint32_t per_vec) {
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t vec = blockIdx.y;
-  if (t >= per_vec) return;
-  uint32_t lo = t & ((1u << s0) - 1);
-  uint32_t hi = t >> s0;
-  size_t base = ((size_t)hi << (s0 + R)) + lo;
-  uint4* v = data + (((size_t)vec << L) * 2);
-  Fr x[1 << R];
-  ntt_load<R, 0>(x, v, base, s0, DIF ? nullptr : prescale);
-  ntt_passes<R, DIF ? R - 1 : 0, DIF>(x, tw, s0, lo);
-  ntt_store<R, 0>(x, v, base, s0);
+__device__ __forceinline__ Fr lds_get(const uint32_t* lds, int e) {
+  Fr x;
+#pragma unroll
+  for (int l = 0; l < 8; ++l) x.v[l] = lds[l * NTT_TILE + e];
+  return x;
 }
 
 template <bool DIF>
-static void launch_group(void* data, int nvec, int L, int s0, int R, const void* tw,
-                         const void* prescale, hipStream_t s) {
-  uint32_t per_vec = 1u << (L - R);
-  dim3 grid((per_vec + 255) / 256, nvec);
-  auto d = static_cast<uint4*>(data);
-  auto t = static_cast<const uint4*>(tw);
-  auto p = static_cast<const uint4*>(prescale);
-  switch (R) {
-    case 1: hipLaunchKernelGGL((k_ntt_group<1, DIF>), grid, dim3(256), 0, s, d, L, s0, t, p, per_vec); break;
-    case 2: hipLaunchKernelGGL((k_ntt_group<2, DIF>), grid, dim3(256), 0, s, d, L, s0, t, p, per_vec); break;
-    case 3: hipLaunchKernelGGL((k_ntt_group<3, DIF>), grid, dim3(256), 0, s, d, L, s0, t, p, per_vec); break;
-    default: hipLaunchKernelGGL((k_ntt_group<4, DIF>), grid, dim3(256), 0, s, d, L, s0, t, p, per_vec); break;
+__global__ void __launch_bounds__(NTT_THREADS) k_ntt_pass(uint4* data, int L, int s0, int B,
+                                                          int tlo, const uint4* tw,
+                                                          const uint4* prescale) {
+  __shared__ uint32_t lds[8 * NTT_TILE];
+  const int E = tlo << B;  // elements in this tile
+  const uint32_t tiles_lo = (1u << s0) / (uint32_t)tlo;
+  const uint32_t hi = blockIdx.x / tiles_lo;
+  const uint32_t lo0 = (blockIdx.x % tiles_lo) * (uint32_t)tlo;
+  uint4* v = data + (((size_t)blockIdx.y << L) * 2);
+  const size_t base = ((size_t)hi << (s0 + B)) + lo0;
+  for (int e = threadIdx.x; e < E; e += NTT_THREADS) {
+    const int k = e / tlo, l = e % tlo;
+    const size_t idx = base + ((size_t)k << s0) + l;
+    Fr x = ldfr(v + 2 * idx);
+    if (!DIF && prescale) x = mul_rp(x, ldfr(prescale + 2 * idx));
+    lds_put(lds, e, x);
   }
-  G16_LAUNCH_CHECK("ntt group");
+  __syncthreads();
+  for (int step = 0; step < B; ++step) {
+    const int m = DIF ? B - 1 - step : step;  // stage s0 + m
+    const int s = s0 + m;
+    const uint4* T = tw + 2 * (((size_t)1 << s) - 1);
+    for (int u = threadIdx.x; u < E / 2; u += NTT_THREADS) {
+      const int q = u / tlo, l = u % tlo;
+      const int k = ((q >> m) << (m + 1)) | (q & ((1 << m) - 1));  // bit m of k clear
+      const int e0 = k * tlo + l, e1 = e0 + (tlo << m);
+      const uint32_t j = ((uint32_t)(k & ((1 << m) - 1)) << s0) + lo0 + (uint32_t)l;
+      const Fr w = ldfr(T + 2 * (size_t)j);
+      const Fr a = lds_get(lds, e0), c = lds_get(lds, e1);
+      if (DIF) {
+        lds_put(lds, e0, add(a, c));
+        lds_put(lds, e1, mul_rp(sub(a, c), w));
+      } else {
+        const Fr d = mul_rp(c, w);
+        lds_put(lds, e0, add(a, d));
+        lds_put(lds, e1, sub(a, d));
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < E; e += NTT_THREADS) {
+    const int k = e / tlo, l = e % tlo;
+    stfr(v + 2 * (base + ((size_t)k << s0) + l), lds_get(lds, e));
+  }
 }
 
-static constexpr int RMAX = 4;
+template <bool DIF>
+static void launch_pass(void* data, int nvec, int L, int s0, int B, const void* tw,
+                        const void* prescale, hipStream_t s) {
+  int tlo = NTT_TILE >> B;
+  if (tlo > (1 << s0)) tlo = 1 << s0;
+  const uint32_t tiles = (1u << (L - B)) / (uint32_t)tlo;
+  hipLaunchKernelGGL(k_ntt_pass<DIF>, dim3(tiles, nvec), dim3(NTT_THREADS), 0, s,
+                     static_cast<uint4*>(data), L, s0, B, tlo, static_cast<const uint4*>(tw),
+                     static_cast<const uint4*>(prescale));
+  G16_LAUNCH_CHECK("ntt pass");
+}
+
+// Pass plan: the lowest 11 stages in one pass (contiguous tiles), the rest in passes of at
+// most 6 stages.
+static constexpr int LOW_PASS = 11, HIGH_PASS = 6;
 
 void ntt_dif_inverse(void* data, int nvec, int L, const void* tw, hipStream_t s) {
   if (L == 0) return;
-  // Groups from the top: the last (lowest) group takes the remainder.
+  const int low = L < LOW_PASS ? L : LOW_PASS;
   int top = L;
-  while (top > 0) {
-    int R = top >= RMAX ? RMAX : top;
-    int s0 = top - R;
-    launch_group<true>(data, nvec, L, s0, R, tw, nullptr, s);
-    top = s0;
+  while (top > low) {
+    const int rest = top - low;
+    const int B = rest < HIGH_PASS ? rest : HIGH_PASS;
+    launch_pass<true>(data, nvec, L, top - B, B, tw, nullptr, s);
+    top -= B;
   }
+  launch_pass<true>(data, nvec, L, 0, low, tw, nullptr, s);
 }
 
 void ntt_dit_forward(void* data, int nvec, int L, const void* tw, const void* prescale,
@@ -145,11 +134,13 @@ void ntt_dit_forward(void* data, int nvec, int L, const void* tw, const void* pr
     if (prescale) launch_scale_rp(data, (size_t)nvec, prescale, s);  // n == 1: one factor
     return;
   }
-  int s0 = 0;
+  const int low = L < LOW_PASS ? L : LOW_PASS;
+  launch_pass<false>(data, nvec, L, 0, low, tw, prescale, s);
+  int s0 = low;
   while (s0 < L) {
-    int R = (L - s0) >= RMAX ? RMAX : (L - s0);
-    launch_group<false>(data, nvec, L, s0, R, tw, s0 == 0 ? prescale : nullptr, s);
-    s0 += R;
+    const int B = (L - s0) < HIGH_PASS ? (L - s0) : HIGH_PASS;
+    launch_pass<false>(data, nvec, L, s0, B, tw, nullptr, s);
+    s0 += B;
   }
 }
 

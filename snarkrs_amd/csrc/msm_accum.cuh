@@ -94,8 +94,7 @@ __device__ __forceinline__ void st_xyzz(uint4* base, size_t i, const Xyzz<F>& r)
 // bucket accumulation
 // ---------------------------------------------------------------------------------------
 template <class F>
-__global__ void __launch_bounds__(128) k_accumulate(const uint32_t* __restrict__ keys,
-                                                    const uint32_t* __restrict__ vals,
+__global__ void __launch_bounds__(128) k_accumulate(const uint2* __restrict__ ent,
                                                     const uint32_t* total, size_t nseg,
                                                     const uint4* __restrict__ bases,
                                                     uint4* buckets, uint4* pfirst,
@@ -106,17 +105,17 @@ __global__ void __launch_bounds__(128) k_accumulate(const uint32_t* __restrict__
   size_t b = t * SEG;
   if (b >= T) return;
   size_t e = b + SEG < T ? b + SEG : T;
-  uint32_t cur = keys[b];
-  bool starts = (b == 0) || keys[b - 1] != cur;
+  uint32_t cur = ent[b].x;
+  bool starts = (b == 0) || ent[b - 1].x != cur;
   size_t rs = b;
   Xyzz<F> acc = xyzz_inf<F>();
   // G1: the next entry's point is loaded before this one is added, so the load latency of a
   // random base hides behind a mixed addition (the G2 loop has no registers to spare).
   constexpr bool PREFETCH = sizeof(F) == 32;
-  uint32_t v = vals[b];
+  uint32_t v = ent[b].y;
   Aff<F> a = ld_aff<F>(bases, v >> 1);
   for (size_t k = b; k < e; ++k) {
-    uint32_t key = keys[k];
+    uint32_t key = ent[k].x;
     if (key != cur) {
       // Run [rs, k) of bucket `cur` ends inside the segment.
       if (starts) st_xyzz(buckets, cur, acc);
@@ -127,13 +126,13 @@ __global__ void __launch_bounds__(128) k_accumulate(const uint32_t* __restrict__
       rs = k;
     }
     if (!PREFETCH && k != b) {
-      v = vals[k];
+      v = ent[k].y;
       a = ld_aff<F>(bases, v >> 1);
     }
     uint32_t v_next = 0;
     Aff<F> a_next;
     if (PREFETCH && k + 1 < e) {
-      v_next = vals[k + 1];
+      v_next = ent[k + 1].y;
       a_next = ld_aff<F>(bases, v_next >> 1);
     }
     acc = xyzz_add_aff(acc, (v & 1) ? aff_neg(a) : a);
@@ -142,7 +141,7 @@ __global__ void __launch_bounds__(128) k_accumulate(const uint32_t* __restrict__
       a = a_next;
     }
   }
-  bool ends = (e == T) || keys[e] != cur;
+  bool ends = (e == T) || ent[e].x != cur;
   if (starts && ends) {
     st_xyzz(buckets, cur, acc);
   } else {
@@ -415,7 +414,7 @@ inline void accumulate_impl(const MsmPlan& p, const void* bases, void* scratch, 
   if (ev) G16_CHECK(hipEventRecord(ev[0], s));
   if (nseg) {
     hipLaunchKernelGGL(k_accumulate<F>, dim3((unsigned)((nseg + 127) / 128)), dim3(128), 0, s,
-                       p.keys, p.vals, p.total, nseg, bp, a.buckets, a.pfirst, a.plast);
+                       p.ent, p.total, nseg, bp, a.buckets, a.pfirst, a.plast);
     G16_LAUNCH_CHECK("msm accumulate");
   }
   if (ev) G16_CHECK(hipEventRecord(ev[1], s));
