@@ -67,6 +67,9 @@ struct MsmPlan {
   uint32_t seg = 0;       // entries per accumulation segment
   size_t max_entries = 0; // n * windows
   uint32_t chunks = 0;    // scalar chunks of the counting sort
+  // Merged windows: the bases come as window tables (entry w n + i = 2^(c w) P_i), so the
+  // digits of every window fall into one set of nb buckets and the window sum is the MSM.
+  bool merged = false;
   // device scratch (owned by the caller)
   uint32_t* count = nullptr;    // [windows*nb + 1] entries per bucket
   uint32_t* offset = nullptr;   // [windows*nb + 1] first entry of each bucket; [K] = total
@@ -75,10 +78,17 @@ struct MsmPlan {
                                 //                 value = index << 1 | sign}
   uint32_t* digits = nullptr;   // [windows * n] encoded signed digits, window-major
   uint32_t* chunk_count = nullptr;  // [windows][chunks][nb] per-chunk counts, then prefixes
+  uint32_t* btotal = nullptr;   // [nb + 1] merged: entries per bucket over all windows
   uint32_t* total = nullptr;    // [4] entry count
 };
+// Bucket sets the accumulation and reduction work on: one per window, or one when merged.
+inline int msm_bucket_sets(const MsmPlan& p) { return p.merged ? 1 : p.windows; }
 size_t msm_sort_scratch_bytes(uint32_t n, int c);
-void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c);
+void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c, bool merged = false);
+// Window tables for merged MSMs: `table` holds n affine bases (MSM form) at [0, n); fills
+// entries w n + i = 2^(c w) P_i for 1 <= w < windows. group 1 = G1, 2 = G2.
+void msm_window_tables(int group, void* table, size_t n, int c, int windows, hipStream_t s);
+void msm_window_tables_g2(void* table, size_t n, int c, int windows, hipStream_t s);
 // Digits of `n` Montgomery scalars into (window, bucket)-sorted entries.
 void msm_sort(MsmPlan& p, const void* scalars_mont, hipStream_t s);
 size_t msm_accum_scratch_bytes(const MsmPlan& p, int group);

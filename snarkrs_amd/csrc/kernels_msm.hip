@@ -129,17 +129,36 @@ __global__ void __launch_bounds__(256) k_chunk_scan(uint32_t* chunk_count, uint3
   count[t] = run;
 }
 
+// Merged windows: per bucket b, count[w * nb + b] becomes the entries of windows before w and
+// btotal[b] the bucket's total over all windows.
+__global__ void __launch_bounds__(256) k_window_scan(uint32_t* count, uint32_t nb, int W,
+                                                     uint32_t* btotal) {
+  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  uint32_t run = 0;
+  for (int w = 0; w < W; ++w) {
+    uint32_t c = count[(size_t)w * nb + b];
+    count[(size_t)w * nb + b] = run;
+    run += c;
+  }
+  btotal[b] = run;
+}
+
 // Entries of chunk blockIdx.x of window blockIdx.y whose bucket lies in
 // [blockIdx.z * span, + span): positions from the bucket offset, the chunk prefix and an LDS
-// cursor.
+// cursor. Merged (wpre given): buckets are shared by every window, offsets are per bucket,
+// wpre[w * nb + b] places window w after the earlier windows, and an entry's value indexes the
+// window table (w n + i).
 __global__ void __launch_bounds__(SORT_THREADS) k_chunk_scatter(
     const uint32_t* digits, uint32_t n, uint32_t nb, uint32_t chunks, uint32_t span,
-    const uint32_t* chunk_count, const uint32_t* offset, uint2* ent) {
+    const uint32_t* chunk_count, const uint32_t* offset, const uint32_t* wpre, uint2* ent) {
   extern __shared__ uint32_t cur[];  // span words
   uint32_t p = blockIdx.x, w = blockIdx.y, b0 = blockIdx.z * span;
   const uint32_t* cc = chunk_count + ((size_t)w * chunks + p) * nb + b0;
-  const uint32_t* off = offset + (size_t)w * nb + b0;
-  for (uint32_t j = threadIdx.x; j < span; j += SORT_THREADS) cur[j] = off[j] + cc[j];
+  const uint32_t* off = offset + (wpre ? 0 : (size_t)w * nb) + b0;
+  const uint32_t* wp = wpre ? wpre + (size_t)w * nb + b0 : nullptr;
+  for (uint32_t j = threadIdx.x; j < span; j += SORT_THREADS)
+    cur[j] = off[j] + cc[j] + (wp ? wp[j] : 0u);
   __syncthreads();
   size_t lo = (size_t)p * SORT_CHUNK;
   size_t hi = lo + SORT_CHUNK < n ? lo + SORT_CHUNK : n;
@@ -150,7 +169,9 @@ __global__ void __launch_bounds__(SORT_THREADS) k_chunk_scatter(
     uint32_t b = (e & 0xFFFFu) - b0;
     if (b >= span) continue;
     uint32_t pos = atomicAdd(&cur[b], 1u);
-    ent[pos] = make_uint2(w * nb + b0 + b, ((uint32_t)i << 1) | ((e >> 16) & 1u));
+    const uint32_t key = (wpre ? 0u : w * nb) + b0 + b;
+    const uint32_t idx = wpre ? w * n + (uint32_t)i : (uint32_t)i;
+    ent[pos] = make_uint2(key, (idx << 1) | ((e >> 16) & 1u));
   }
 }
 
@@ -242,11 +263,12 @@ size_t msm_sort_scratch_bytes(uint32_t n, int c) {
   size_t K = (size_t)W * nb;
   size_t E = (size_t)n * W;
   return align_up(4 * (K + 1)) * 2 + align_up(4 * 1025) + align_up(8 * E) + align_up(4 * E) +
-         align_up(4 * K * sort_chunks(n)) + align_up(16);
+         align_up(4 * K * sort_chunks(n)) + align_up(4 * (nb + 1)) + align_up(16);
 }
 
-void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c) {
+void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c, bool merged) {
   if (c < 2 || c > 16) throw ArgError{"MSM window width must be in [2, 16]"};
+  p.merged = merged;
   p.c = c;
   p.windows = (RECODE_BITS + c - 1) / c;
   p.n = n;
@@ -257,7 +279,7 @@ void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c) {
   size_t K = (size_t)p.windows * p.nb;
   if ((K + 1023) / 1024 > 1024) throw ArgError{"MSM window too wide for the scan"};
   if ((size_t)n >= (1u << 31)) throw ArgError{"MSM larger than 2^31 points"};
-  if (p.max_entries >= (1ull << 32)) throw ArgError{"MSM entry count exceeds 2^32"};
+  if (p.max_entries >= (1ull << 31)) throw ArgError{"MSM entry count exceeds 2^31"};
   char* q = static_cast<char*>(scratch);
   auto take = [&](size_t bytes) {
     char* r = q;
@@ -270,6 +292,7 @@ void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c) {
   p.ent = reinterpret_cast<uint2*>(take(8 * p.max_entries));
   p.digits = reinterpret_cast<uint32_t*>(take(4 * p.max_entries));
   p.chunk_count = reinterpret_cast<uint32_t*>(take(4 * K * p.chunks));
+  p.btotal = reinterpret_cast<uint32_t*>(take(4 * ((size_t)p.nb + 1)));
   p.total = reinterpret_cast<uint32_t*>(take(16));
 }
 
@@ -286,20 +309,30 @@ void msm_sort(MsmPlan& p, const void* scalars, hipStream_t s) {
   hipLaunchKernelGGL(k_chunk_scan, dim3((K + 255) / 256), dim3(256), 0, s, p.chunk_count, p.nb,
                      p.chunks, K, p.count);
   G16_LAUNCH_CHECK("msm chunk scan");
-  uint32_t nblk = (K + 1023) / 1024;
-  hipLaunchKernelGGL(k_scan_blocks, dim3(nblk), dim3(256), 0, s, p.count, K, p.offset,
+  // The bucket totals to scan: per (window, bucket), or per bucket over all windows.
+  const uint32_t* totals = p.count;
+  uint32_t KS = K;
+  if (p.merged) {
+    hipLaunchKernelGGL(k_window_scan, dim3((p.nb + 255) / 256), dim3(256), 0, s, p.count, p.nb,
+                       p.windows, p.btotal);
+    G16_LAUNCH_CHECK("msm window scan");
+    totals = p.btotal;
+    KS = p.nb;
+  }
+  uint32_t nblk = (KS + 1023) / 1024;
+  hipLaunchKernelGGL(k_scan_blocks, dim3(nblk), dim3(256), 0, s, totals, KS, p.offset,
                      p.block_sums);
   G16_LAUNCH_CHECK("msm scan");
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, p.block_sums, nblk);
   G16_LAUNCH_CHECK("msm scan top");
-  hipLaunchKernelGGL(k_scan_fix, dim3((K + 255) / 256), dim3(256), 0, s, p.offset,
-                     p.block_sums, K, p.total);
+  hipLaunchKernelGGL(k_scan_fix, dim3((KS + 255) / 256), dim3(256), 0, s, p.offset,
+                     p.block_sums, KS, p.total);
   G16_LAUNCH_CHECK("msm scan fix");
   if (p.n) {
     uint32_t span = p.nb < SCATTER_SPAN ? p.nb : SCATTER_SPAN;
     hipLaunchKernelGGL(k_chunk_scatter, dim3(p.chunks, p.windows, p.nb / span),
                        dim3(SORT_THREADS), 4 * span, s, p.digits, p.n, p.nb, p.chunks, span,
-                       p.chunk_count, p.offset, p.ent);
+                       p.chunk_count, p.offset, p.merged ? p.count : nullptr, p.ent);
     G16_LAUNCH_CHECK("msm chunk scatter");
   }
 }
@@ -313,7 +346,7 @@ void msm_reduce_shape(const MsmPlan& p, int* rows, int* lg) {
 size_t msm_out_points(const MsmPlan& p) {
   int rows, lg;
   msm_reduce_shape(p, &rows, &lg);
-  return (size_t)p.windows * rows;
+  return (size_t)msm_bucket_sets(p) * rows;
 }
 
 size_t msm_accum_scratch_bytes(const MsmPlan& p, int group) {
@@ -329,6 +362,18 @@ template <>
 void Tail<Fm>::reduce(const MsmPlan& p, const AccumPtrs& a, uint32_t B, void* window_out,
                       hipStream_t s) {
   tail_reduce_impl<Fm>(p, a, B, window_out, s);
+}
+
+// Window tables: entry w n + i = 2^(c w) P_i from P_i at entry i, one thread per point.
+void msm_window_tables(int group, void* table, size_t n, int c, int windows, hipStream_t s) {
+  if (group == 2) {
+    msm_window_tables_g2(table, n, c, windows, s);
+    return;
+  }
+  if (!n || windows < 2) return;
+  hipLaunchKernelGGL(k_window_tables<Fm>, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s,
+                     static_cast<uint4*>(table), n, c, windows);
+  G16_LAUNCH_CHECK("msm window tables");
 }
 
 // x -> 32 x mod q (five modular doublings): the R = 2^256 Montgomery form to R' = 2^261.

@@ -69,6 +69,14 @@ __device__ __forceinline__ Aff<F> ld_aff(const uint4* base, uint32_t i) {
 }
 
 template <class F>
+__device__ __forceinline__ void st_aff(uint4* base, size_t i, const Aff<F>& a) {
+  constexpr int W = Words<F>::N;
+  uint4* p = base + i * 2 * W;
+  st_f(p, a.x);
+  st_f(p + W, a.y);
+}
+
+template <class F>
 __device__ __forceinline__ Xyzz<F> ld_xyzz(const uint4* base, size_t i) {
   constexpr int W = Words<F>::N;
   const uint4* p = base + i * 4 * W;
@@ -88,6 +96,23 @@ __device__ __forceinline__ void st_xyzz(uint4* base, size_t i, const Xyzz<F>& r)
   st_f(p + W, r.Y);
   st_f(p + 2 * W, r.ZZ);
   st_f(p + 3 * W, r.ZZZ);
+}
+
+// Window tables for merged MSMs (msm_window_tables): entry w n + i = 2^(c w) P_i, affine, from
+// the previous window's point by c doublings and one inversion; infinity stays (0, 0).
+template <class F>
+__global__ void __launch_bounds__(128) k_window_tables(uint4* table, size_t n, int c, int W) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Aff<F> P = ld_aff<F>(table, (uint32_t)i);
+  for (int w = 1; w < W; ++w) {
+    if (!aff_is_inf(P)) {
+      Xyzz<F> X = xyzz_dbl_aff(P);
+      for (int k = 1; k < c; ++k) X = xyzz_dbl(X);
+      P = xyzz_to_aff(X);
+    }
+    st_aff(table, (size_t)w * n + i, P);
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -113,7 +138,7 @@ __global__ void __launch_bounds__(128) k_accumulate(const uint2* __restrict__ en
   // random base hides behind a mixed addition (the G2 loop has no registers to spare).
   constexpr bool PREFETCH = sizeof(F) == 32;
   uint32_t v = ent[b].y;
-  Aff<F> a = ld_aff<F>(bases, v >> 1);
+  Aff<F> a = ld_aff<F>(bases, v >> 1);  // merged: v >> 1 = w n + i, a window-table entry
   for (size_t k = b; k < e; ++k) {
     uint32_t key = ent[k].x;
     if (key != cur) {
@@ -189,6 +214,65 @@ __global__ void __launch_bounds__(256) k_merge(const uint32_t* offset, uint32_t 
   for (uint32_t t = sa + 1; t < sb; ++t) acc = xyzz_add(acc, ld_xyzz<F>(pfirst, t));
   acc = xyzz_add(acc, ld_xyzz<F>(pfirst, sb));
   st_xyzz(buckets, k, acc);
+}
+
+// Merged windows: every bucket holds many entries and spans many segments (about 40 at the
+// 1.1M-constraint size), so a bucket gets MERGE_LANES lanes: each sums every MERGE_LANES-th of
+// the bucket's segment pieces (plast of its first segment, pfirst of the rest), then the lanes
+// of a bucket add up through LDS. 256-lane workgroups take 256 / MERGE_LANES buckets. A bucket
+// past WIDE_SPAN segments (the ones of a boolean witness pile into one) is handed to the heavy
+// path instead, as k_merge does.
+static constexpr int MERGE_LANES = 8;
+static constexpr uint32_t WIDE_SPAN = 512;
+
+template <class F>
+__global__ void __launch_bounds__(256) k_merge_wide(const uint32_t* offset, uint32_t K,
+                                                    uint4* buckets, const uint4* pfirst,
+                                                    const uint4* plast, uint32_t* cnt,
+                                                    uint32_t* heavy, uint32_t* tasks) {
+  __shared__ Xyzz<F> sh[256];
+  const uint32_t lane = threadIdx.x % MERGE_LANES;
+  const uint32_t k = blockIdx.x * (256 / MERGE_LANES) + threadIdx.x / MERGE_LANES;
+  Xyzz<F> acc = xyzz_inf<F>();
+  bool store = false;  // the bucket's sum is merged here
+  uint32_t s = 0, e = 0;
+  if (k < K) {
+    s = offset[k];
+    e = offset[k + 1];
+    if (s != e) {
+      const uint32_t sa = s / SEG, sb = (e - 1) / SEG;
+      if (sb - sa > WIDE_SPAN) {
+        if (lane == 0) {
+          uint32_t interior = sb - sa - 1;
+          uint32_t ntask = (interior + HEAVY_TASK - 1) / HEAVY_TASK;
+          uint32_t slot = atomicAdd(&cnt[0], 1u);
+          uint32_t tb = atomicAdd(&cnt[1], ntask);
+          heavy[3 * slot] = k;
+          heavy[3 * slot + 1] = tb;
+          heavy[3 * slot + 2] = ntask;
+          for (uint32_t t = 0; t < ntask; ++t) {
+            uint32_t lo = sa + 1 + t * HEAVY_TASK;
+            tasks[2 * (tb + t)] = lo;
+            tasks[2 * (tb + t) + 1] = lo + HEAVY_TASK < sb ? lo + HEAVY_TASK : sb;
+          }
+        }
+      } else if (sa != sb) {
+        store = true;
+        for (uint32_t t = sa + lane; t <= sb; t += MERGE_LANES)
+          acc = xyzz_add(acc, t == sa ? ld_xyzz<F>(plast, t) : ld_xyzz<F>(pfirst, t));
+      }
+    }
+  }
+  sh[threadIdx.x] = acc;
+  for (int off = MERGE_LANES / 2; off > 0; off >>= 1) {
+    __syncthreads();
+    if (lane < (uint32_t)off) sh[threadIdx.x] = xyzz_add(sh[threadIdx.x], sh[threadIdx.x + off]);
+  }
+  __syncthreads();
+  if (k < K && lane == 0) {
+    if (s == e) st_xyzz(buckets, k, xyzz_inf<F>());
+    else if (store) st_xyzz(buckets, k, sh[threadIdx.x]);
+  }
 }
 
 // Sum of BLOCK per-thread values in LDS, left in sh[0].
@@ -312,7 +396,7 @@ struct AccumLayout {
 };
 
 inline AccumLayout accum_layout(const MsmPlan& p, size_t pt) {
-  size_t K = (size_t)p.windows * p.nb;
+  size_t K = (size_t)msm_bucket_sets(p) * p.nb;
   size_t nseg = (p.max_entries + SEG - 1) / SEG;
   size_t max_heavy = nseg / HEAVY_SPAN + 2;
   size_t max_tasks = nseg / HEAVY_TASK + max_heavy + 2;
@@ -331,8 +415,8 @@ inline AccumLayout accum_layout(const MsmPlan& p, size_t pt) {
   L.heavy = take(12 * max_heavy);
   L.tasks = take(8 * max_tasks);
   L.task_sum = take(pt * max_tasks);
-  L.gT = take(pt * G * p.windows);
-  L.gU = take(pt * G * p.windows);
+  L.gT = take(pt * G * msm_bucket_sets(p));
+  L.gU = take(pt * G * msm_bucket_sets(p));
   L.cnt = take(16);
   L.total = off;
   return L;
@@ -361,12 +445,20 @@ struct Tail {
 
 template <class F>
 inline void tail_merge_impl(const MsmPlan& p, const AccumPtrs& a, hipStream_t s) {
-  size_t K = (size_t)p.windows * p.nb;
+  size_t K = (size_t)msm_bucket_sets(p) * p.nb;
   hipLaunchKernelGGL(k_clear_cnt<0>, dim3(1), dim3(64), 0, s, a.cnt);
   G16_LAUNCH_CHECK("msm clear counters");
-  hipLaunchKernelGGL(k_merge<F>, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, s, p.offset,
-                     (uint32_t)K, a.buckets, a.pfirst, a.plast, a.cnt, a.heavy, a.tasks);
-  G16_LAUNCH_CHECK("msm merge");
+  if (p.merged) {
+    constexpr unsigned per_block = 256 / MERGE_LANES;
+    hipLaunchKernelGGL(k_merge_wide<F>, dim3((unsigned)((K + per_block - 1) / per_block)),
+                       dim3(256), 0, s, p.offset, (uint32_t)K, a.buckets, a.pfirst, a.plast,
+                       a.cnt, a.heavy, a.tasks);
+    G16_LAUNCH_CHECK("msm merge (wide)");
+  } else {
+    hipLaunchKernelGGL(k_merge<F>, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, s, p.offset,
+                       (uint32_t)K, a.buckets, a.pfirst, a.plast, a.cnt, a.heavy, a.tasks);
+    G16_LAUNCH_CHECK("msm merge");
+  }
   hipLaunchKernelGGL((k_heavy_tasks<F, HEAVY_TASK>), dim3(1024), dim3(HEAVY_TASK), 0, s, a.cnt,
                      a.tasks, a.pfirst, a.task_sum);
   G16_LAUNCH_CHECK("msm heavy tasks");
@@ -380,11 +472,12 @@ inline void tail_reduce_impl(const MsmPlan& p, const AccumPtrs& a, uint32_t B, v
                              hipStream_t s) {
   uint32_t G = p.nb / B;
   uint32_t rows = 1 + (uint32_t)ilog2_u32(G);
-  uint32_t nthreads = (uint32_t)p.windows * G;
+  const int sets = msm_bucket_sets(p);
+  uint32_t nthreads = (uint32_t)sets * G;
   hipLaunchKernelGGL(k_reduce_groups<F>, dim3((nthreads + 127) / 128), dim3(128), 0, s,
-                     a.buckets, p.nb, p.windows, B, a.gT, a.gU);
+                     a.buckets, p.nb, sets, B, a.gT, a.gU);
   G16_LAUNCH_CHECK("msm reduce groups");
-  hipLaunchKernelGGL((k_reduce_rows<F, 256>), dim3((unsigned)p.windows * rows), dim3(256), 0, s,
+  hipLaunchKernelGGL((k_reduce_rows<F, 256>), dim3((unsigned)sets * rows), dim3(256), 0, s,
                      a.gT, a.gU, G, rows, static_cast<uint4*>(window_out));
   G16_LAUNCH_CHECK("msm reduce rows");
 }

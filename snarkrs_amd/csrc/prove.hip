@@ -68,6 +68,7 @@ struct g16hip_circuit {
   uint32_t n_vars = 0, n_public = 0, domain = 0;
   int log_n = 0;
   int cw = 0, ch = 0;  // window widths of the witness and H MSMs
+  bool merged = false; // bases stored as window tables (one bucket set per MSM)
   G1Aff alpha_g1, beta_g1, delta_g1;
   G2Aff beta_g2, delta_g2;
   DevBuf row_ptr[2], signal[2], value_id[2], values;
@@ -140,31 +141,54 @@ static void upload_key(g16hip_circuit* C, const g16hip_key* k, hipStream_t s) {
   }
   upload(C->values, k->values, 32 * (size_t)k->n_values, s);
   size_t nv = k->n_vars;
-  upload(C->a_q, k->a_query, 64 * nv, s);
-  upload(C->b1_q, k->b_g1_query, 64 * nv, s);
-  upload(C->b2_q, k->b_g2_query, 128 * nv, s);
-  // L covers the private wires only; it is stored behind n_public + 1 points at infinity so
-  // that it multiplies the full witness and shares the witness sort with A, B1 and B2.
-  size_t pub = (size_t)k->n_public + 1;
-  C->l_q.alloc(64 * nv);
-  G16_CHECK(hipMemsetAsync(C->l_q.p, 0, 64 * pub, s));
-  if (nv > pub)
-    G16_CHECK(hipMemcpyAsync(static_cast<char*>(C->l_q.p) + 64 * pub, k->l_query, 64 * (nv - pub),
-                             hipMemcpyHostToDevice, s));
-  upload(C->h_q, k->h_query, 64 * (size_t)n, s);
-  // The bases serve only the MSMs: keep them in the MSM kernels' R' = 2^261 form.
-  msm_to_msm_form(C->a_q.p, C->a_q.p, 2 * nv, s);
-  msm_to_msm_form(C->b1_q.p, C->b1_q.p, 2 * nv, s);
-  msm_to_msm_form(C->l_q.p, C->l_q.p, 2 * nv, s);
-  msm_to_msm_form(C->h_q.p, C->h_q.p, 2 * (size_t)n, s);
-  msm_to_msm_form(C->b2_q.p, C->b2_q.p, 4 * nv, s);
-  build_ntt_tables(C->tables, C->log_n, s);
   // The witness digits feed A, B1, L (G1) and B2 (G2); H feeds one G1 MSM.
   // G16HIP_C_WITNESS / G16HIP_C_H override the widths (2..16).
   C->cw = msm_choose_c((uint32_t)nv, 3, 1);
   C->ch = msm_default_c(n);
   if (const char* e = std::getenv("G16HIP_C_WITNESS")) C->cw = std::max(2, std::min(16, std::atoi(e)));
   if (const char* e = std::getenv("G16HIP_C_H")) C->ch = std::max(2, std::min(16, std::atoi(e)));
+  const int Ww = (255 + C->cw - 1) / C->cw, Wh = (255 + C->ch - 1) / C->ch;
+  // Window tables: with entry w n + i = 2^(c w) P_i precomputed (the key's bases are fixed),
+  // an MSM accumulates the digits of every window into one set of buckets, so the merges and
+  // the window reduction run once instead of once per window. They take W times the bases'
+  // memory and are used when the device keeps 8 GiB free after them (G16HIP_WINDOW_TABLES=0
+  // turns them off).
+  const size_t table_bytes = (3 * 64 + 128) * nv * Ww + 64 * (size_t)n * Wh;
+  size_t free_b = 0, total_b = 0;
+  G16_CHECK(hipMemGetInfo(&free_b, &total_b));
+  const char* wt = std::getenv("G16HIP_WINDOW_TABLES");
+  C->merged = !(wt && wt[0] == '0') && free_b > table_bytes + (8ull << 30);
+  const size_t tw = C->merged ? (size_t)Ww : 1, th = C->merged ? (size_t)Wh : 1;
+  auto put = [&](DevBuf& b, const void* src, size_t npts, size_t pt, size_t windows) {
+    b.alloc(pt * npts * windows);
+    if (npts) G16_CHECK(hipMemcpyAsync(b.p, src, pt * npts, hipMemcpyHostToDevice, s));
+  };
+  put(C->a_q, k->a_query, nv, 64, tw);
+  put(C->b1_q, k->b_g1_query, nv, 64, tw);
+  put(C->b2_q, k->b_g2_query, nv, 128, tw);
+  // L covers the private wires only; it is stored behind n_public + 1 points at infinity so
+  // that it multiplies the full witness and shares the witness sort with A, B1 and B2.
+  size_t pub = (size_t)k->n_public + 1;
+  C->l_q.alloc(64 * nv * tw);
+  G16_CHECK(hipMemsetAsync(C->l_q.p, 0, 64 * pub, s));
+  if (nv > pub)
+    G16_CHECK(hipMemcpyAsync(static_cast<char*>(C->l_q.p) + 64 * pub, k->l_query, 64 * (nv - pub),
+                             hipMemcpyHostToDevice, s));
+  put(C->h_q, k->h_query, n, 64, th);
+  // The bases serve only the MSMs: keep them in the MSM kernels' R' = 2^261 form.
+  msm_to_msm_form(C->a_q.p, C->a_q.p, 2 * nv, s);
+  msm_to_msm_form(C->b1_q.p, C->b1_q.p, 2 * nv, s);
+  msm_to_msm_form(C->l_q.p, C->l_q.p, 2 * nv, s);
+  msm_to_msm_form(C->h_q.p, C->h_q.p, 2 * (size_t)n, s);
+  msm_to_msm_form(C->b2_q.p, C->b2_q.p, 4 * nv, s);
+  if (C->merged) {
+    msm_window_tables(1, C->a_q.p, nv, C->cw, Ww, s);
+    msm_window_tables(1, C->b1_q.p, nv, C->cw, Ww, s);
+    msm_window_tables(1, C->l_q.p, nv, C->cw, Ww, s);
+    msm_window_tables(2, C->b2_q.p, nv, C->cw, Ww, s);
+    msm_window_tables(1, C->h_q.p, n, C->ch, Wh, s);
+  }
+  build_ntt_tables(C->tables, C->log_n, s);
   G16_CHECK(hipStreamSynchronize(s));
   DevBuf* bufs[] = {&C->row_ptr[0], &C->row_ptr[1], &C->signal[0], &C->signal[1],
                     &C->value_id[0], &C->value_id[1], &C->values, &C->a_q, &C->b1_q, &C->b2_q,
@@ -207,8 +231,8 @@ static void size_slot(const g16hip_circuit* C, Slot* S) {
   S->hbuf.alloc(32 * n);
   S->sort_w.alloc(msm_sort_scratch_bytes((uint32_t)nv, C->cw));
   S->sort_h.alloc(msm_sort_scratch_bytes((uint32_t)n, C->ch));
-  msm_plan_bind(S->plan_w, S->sort_w.p, (uint32_t)nv, C->cw);
-  msm_plan_bind(S->plan_h, S->sort_h.p, (uint32_t)n, C->ch);
+  msm_plan_bind(S->plan_w, S->sort_w.p, (uint32_t)nv, C->cw, C->merged);
+  msm_plan_bind(S->plan_h, S->sort_h.p, (uint32_t)n, C->ch, C->merged);
   S->accum_w1.alloc(msm_accum_scratch_bytes(S->plan_w, 1));
   S->accum_w2.alloc(msm_accum_scratch_bytes(S->plan_w, 2));
   S->accum_h.alloc(msm_accum_scratch_bytes(S->plan_h, 1));
@@ -236,11 +260,13 @@ static size_t slot_bytes_estimate(const g16hip_circuit* C) {
   pw.n = (uint32_t)nv;
   pw.nb = 1u << (C->cw - 1);
   pw.max_entries = nv * pw.windows;
+  pw.merged = C->merged;
   ph.c = C->ch;
   ph.windows = (255 + C->ch - 1) / C->ch;
   ph.n = (uint32_t)n;
   ph.nb = 1u << (C->ch - 1);
   ph.max_entries = n * ph.windows;
+  ph.merged = C->merged;
   return 32 * nv + 32 * 4 * n + msm_sort_scratch_bytes((uint32_t)nv, C->cw) +
          msm_sort_scratch_bytes((uint32_t)n, C->ch) + msm_accum_scratch_bytes(pw, 1) +
          msm_accum_scratch_bytes(pw, 2) + msm_accum_scratch_bytes(ph, 1) +

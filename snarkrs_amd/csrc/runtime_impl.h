@@ -155,7 +155,7 @@ template <class F>
 Xyzz<F> msm_combine(const Xyzz<F>* P, const MsmPlan& p) {
   int rows, lg;
   msm_reduce_shape(p, &rows, &lg);
-  const int c = p.c, W = p.windows;
+  const int c = p.c, W = msm_bucket_sets(p);  // one set when the windows are merged
   int emax = c * (W - 1) + (rows > 1 ? lg + rows - 2 : 0);
   std::vector<Xyzz<F>> E((size_t)emax + 1, xyzz_inf<F>());
   for (int w = 0; w < W; ++w)
