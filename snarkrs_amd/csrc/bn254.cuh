@@ -258,7 +258,9 @@ struct Fr29 {
   static constexpr uint32_t N29 = 0x0fffffffu;
 };
 
-template <class M29>
+// With FULL = false the result is left in [0, 2 modulus) (any 256-bit inputs): the MSM form
+// keeps its values there (see the FqmCfg operations below).
+template <class M29, bool FULL = true>
 HD void mont29(const uint32_t a[8], const uint32_t b[8], uint32_t out[8]) {
   constexpr uint32_t MASK = (1u << 29) - 1u;
   uint32_t A[9], B[9];
@@ -299,14 +301,81 @@ HD void mont29(const uint32_t a[8], const uint32_t b[8], uint32_t out[8]) {
     }
   }
   if (w < 8) out[w] = (uint32_t)acc;
-  reduce_once<typename M29::Cfg>(out);
+  if (FULL) reduce_once<typename M29::Cfg>(out);
 }
 
+// The MSM form's values live in [0, 2q), not reduced below q: the product needs no final
+// subtraction (its result is below 2q for any 256-bit inputs), additions subtract 2q and
+// subtractions add 2q, and zero tests accept 0 and q. Values leave the MSM kernels only
+// through the host, which reduces them fully.
 template <>
 HD Fp<FqmCfg> mul<FqmCfg>(const Fp<FqmCfg>& a, const Fp<FqmCfg>& b) {
   Fp<FqmCfg> r;
-  mont29<Fq29>(a.v, b.v, r.v);
+  mont29<Fq29, false>(a.v, b.v, r.v);
   return r;
+}
+
+HD uint32_t fqm_2q(int i) {
+  constexpr uint32_t Q2[8] = {0xb0f9fa8eu, 0x7841182du, 0xd0e3951au, 0x2f02d522u,
+                              0x0302b0bbu, 0x70a08b6du, 0xc2634053u, 0x60c89ce5u};
+  return Q2[i];
+}
+
+template <>
+HD Fp<FqmCfg> add<FqmCfg>(const Fp<FqmCfg>& a, const Fp<FqmCfg>& b) {
+  Fp<FqmCfg> r, d;
+  uint32_t carry = 0, borrow = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {  // a + b < 4q < 2^256
+    uint64_t t = (uint64_t)a.v[i] + b.v[i] + carry;
+    r.v[i] = (uint32_t)t;
+    carry = (uint32_t)(t >> 32);
+  }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    uint64_t t = (uint64_t)r.v[i] - fqm_2q(i) - borrow;
+    d.v[i] = (uint32_t)t;
+    borrow = (uint32_t)(t >> 63);
+  }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) r.v[i] = borrow ? r.v[i] : d.v[i];
+  return r;
+}
+
+template <>
+HD Fp<FqmCfg> sub<FqmCfg>(const Fp<FqmCfg>& a, const Fp<FqmCfg>& b) {
+  Fp<FqmCfg> r;
+  uint32_t borrow = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    uint64_t t = (uint64_t)a.v[i] - b.v[i] - borrow;
+    r.v[i] = (uint32_t)t;
+    borrow = (uint32_t)(t >> 63);
+  }
+  uint32_t mask = 0u - borrow, carry = 0;  // negative: add 2q
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    uint64_t t = (uint64_t)r.v[i] + (fqm_2q(i) & mask) + carry;
+    r.v[i] = (uint32_t)t;
+    carry = (uint32_t)(t >> 32);
+  }
+  return r;
+}
+
+template <>
+HD bool is_zero<FqmCfg>(const Fp<FqmCfg>& a) {
+  uint32_t z = 0, q = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    z |= a.v[i];
+    q |= a.v[i] ^ FqCfg::p(i);
+  }
+  return z == 0 || q == 0;
+}
+
+template <>
+HD Fp<FqmCfg> neg<FqmCfg>(const Fp<FqmCfg>& a) {
+  return sub(fp_zero<FqmCfg>(), a);
 }
 
 // x * w for x in the usual R = 2^256 form and w stored in the R' = 2^261 form (w 2^261 mod r):

@@ -1,5 +1,7 @@
 // Field probe (device arithmetic checked element by element against the oracle) and the
 // batched fixed-base multiplication the synthetic setup uses.
+#include <type_traits>
+
 #include "bn254.cuh"
 #include "internal.h"
 
@@ -36,6 +38,8 @@ __global__ void k_probe_fp(int op, const uint8_t* a, const uint8_t* b, uint8_t* 
     case 6: r = from_std(x); break;
     default: r = to_std(x); break;
   }
+  // The MSM form keeps values in [0, 2q); the probe reports them fully reduced.
+  if constexpr (std::is_same<C, FqmCfg>::value) reduce_once<FqCfg>(r.v);
   st<C>(out, i, r);
 }
 
