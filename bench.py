@@ -162,7 +162,7 @@ def run_batch(args) -> int:
             "slots_per_circuit": args.slots,
             "witness": "host memory (PCIe copy inside the timed region, both paths)",
             "gpu_max_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
-            "graph_replay": os.environ.get("G16HIP_NO_GRAPH") is None,
+            "graph_replay": os.environ.get("G16HIP_GRAPHS") == "1",
         },
         "batch_ms": t_batch * 1e3,
         "plain_path": {"value": len(items) / t_plain, "ms": t_plain * 1e3},
@@ -280,7 +280,7 @@ def run_serve(args) -> int:
             "requests": len(reqs),
             "lanes": args.lanes,
             "gpu_max_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
-            "graph_replay": os.environ.get("G16HIP_NO_GRAPH") is None,
+            "graph_replay": os.environ.get("G16HIP_GRAPHS") == "1",
         },
         "by_load": runs,
         "bit_identical_to_plain": same,

@@ -178,18 +178,23 @@ typedef struct {
   const uint8_t* s;
 } g16hip_batch_item;
 
-/* Proves every item (any mix of circuits) with all slots of each circuit busy at once; each
- * slot replays its proof pipeline, captured once as a hipGraph. proofs[i] is bit-identical to
- * g16hip_prove_with_blinders on item i. statuses (optional) receives each item's status; the
- * return value is the first failure's. */
+/* Batches and the server run each proof on a slot's three streams, kernel by kernel (the
+ * default), or, with graph replay on, replay the slot's proof pipeline captured once as a
+ * hipGraph (stages 0-9 with their fork/join; no memset nodes). Process-wide; the default can
+ * also be set with G16HIP_GRAPHS=1. Results are bit-identical either way. */
+void g16hip_set_graph_replay(int on);
+
+/* Proves every item (any mix of circuits) with all slots of each circuit busy at once.
+ * proofs[i] is bit-identical to g16hip_prove_with_blinders on item i. statuses (optional)
+ * receives each item's status; the return value is the first failure's. */
 g16hip_status g16hip_prove_batch(const g16hip_batch_item* items, size_t n, g16hip_proof* proofs,
                                  g16hip_status* statuses);
 
 /* A serving loop with continuous batching. `lanes` proofs are in flight at once (a host
  * thread each; 1..64): a free lane takes the oldest request of any circuit, leases a slot of
  * that circuit — adding one, up to `lanes` per circuit, while the device keeps
- * `reserve_bytes` free (admission by free device memory), else waiting for one — and replays
- * the slot's captured proof graph. submit returns at once; the witness stays owned by the
+ * `reserve_bytes` free (admission by free device memory), else waiting for one — and proves on
+ * it. submit returns at once; the witness stays owned by the
  * caller and must stay valid until wait returns for its ticket. wait blocks for a ticket's
  * proof and reports its latency (submit to completion). Slots' streams spread over the
  * process's hardware queues (GPU_MAX_HW_QUEUES, 4 by default) and streams that share a queue

@@ -168,6 +168,7 @@ EXPORTS = {
                                                          ctypes.POINTER(Timings)]),
     "g16hip_circuit_add_slots": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_size_t,
                                                 ctypes.POINTER(ctypes.c_uint32)]),
+    "g16hip_set_graph_replay": (None, [ctypes.c_int]),
     "g16hip_prove_batch": (ctypes.c_int, [ctypes.POINTER(BatchItem), ctypes.c_size_t,
                                           ctypes.POINTER(ProofOut), ctypes.POINTER(ctypes.c_int)]),
     "g16hip_server_open": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_uint32,
@@ -414,6 +415,11 @@ class Circuit:
                                                           ctypes.byref(out),
                                                           ctypes.byref(t or Timings())))
         return out.as_array()
+
+
+def set_graph_replay(on: bool) -> None:
+    """Batches and the server: replay captured graphs (True) or launch eagerly (False)."""
+    load().g16hip_set_graph_replay(1 if on else 0)
 
 
 def prove_batch(items) -> list:

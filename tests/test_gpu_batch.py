@@ -29,7 +29,14 @@ def _blind(i):
     return fr_array([1000 + i]), fr_array([2000 + 3 * i])
 
 
-def test_batch_matches_single_proofs_and_oracle(dev, orc, circuits):
+@pytest.fixture(params=[False, True], ids=["eager", "graphs"])
+def replay(request):
+    native.set_graph_replay(request.param)
+    yield request.param
+    native.set_graph_replay(False)
+
+
+def test_batch_matches_single_proofs_and_oracle(dev, orc, circuits, replay):
     for _, _, _, circ in circuits:
         circ.add_slots(2)
     # independent witnesses of the same circuits
@@ -52,6 +59,7 @@ def test_batch_matches_single_proofs_and_oracle(dev, orc, circuits):
 
 
 def test_graph_replays_interleaved_with_eager_proofs(dev, circuits):
+    native.set_graph_replay(True)
     syn, key, wit, circ = circuits[2]
     r, s = _blind(7)
     eager = circ.prove_with_blinders(wit, r, s)
@@ -59,9 +67,10 @@ def test_graph_replays_interleaved_with_eager_proofs(dev, circuits):
         (g,) = native.prove_batch([(circ, wit, r, s)])
         assert (g == eager).all(), f"replay {k}"
         assert (circ.prove_with_blinders(wit, r, s) == eager).all()
+    native.set_graph_replay(False)
 
 
-def test_server_results_are_the_same_alone_and_in_the_mix(dev, circuits):
+def test_server_results_are_the_same_alone_and_in_the_mix(dev, circuits, replay):
     alone = []
     for i in range(9):
         _, _, wit, circ = circuits[i % 3]
