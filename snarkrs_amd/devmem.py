@@ -47,6 +47,13 @@ class DeviceArray:
             raise RuntimeError("hipMemcpy to host failed")
         return out
 
+    def copy_from(self, other: "DeviceArray") -> None:
+        """Device-to-device copy of min(sizes) bytes (the runtime's copy kernel)."""
+        n = min(self.nbytes, other.nbytes)
+        if n and _lib().hipMemcpy(self.ptr, other.ptr, n, 3) != 0:
+            raise RuntimeError("hipMemcpy device to device failed")
+        synchronize()
+
     def free(self):
         if self.ptr:
             _lib().hipFree(self.ptr)
