@@ -150,6 +150,8 @@ def run_batch(args) -> int:
     log(f"batch: {len(items)} proofs in {t_batch * 1e3:.1f} ms -> {len(items) / t_batch:.1f} "
         f"proofs/s; plain path {t_plain * 1e3:.1f} ms -> {len(items) / t_plain:.1f} proofs/s; "
         f"bit-identical {same}")
+    verified = _verify_all(items, got)
+    log(f"every batch proof verifies: {verified}")
     line = {
         "metric": "Groth16 batch proofs/s (64 proofs, 4 resident circuits of 2^14..2^20 constraints)",
         "value": len(items) / t_batch,
@@ -172,9 +174,27 @@ def run_batch(args) -> int:
         "plain_path": {"value": len(items) / t_plain, "ms": t_plain * 1e3},
         "speedup_vs_plain": t_plain / t_batch,
         "bit_identical_to_plain": same,
+        "every_proof_verifies": verified,
     }
     print(json.dumps(line), flush=True)
-    return 0 if same else 1
+    return 0 if same and verified else 1
+
+
+def _verify_all(items, proofs) -> bool:
+    """Each proof checked against its circuit's verifying key (host pairing verifier)."""
+    from snarkrs_amd import native
+    from snarkrs_amd.verify import VerifyingKey, verify
+
+    vks = {}
+    for (circ, w, _, _), proof in zip(items, proofs):
+        if id(circ) not in vks:
+            vks[id(circ)] = VerifyingKey.from_bytes(circ.vk_bytes())
+        vk = vks[id(circ)]
+        try:
+            verify(vk, np.asarray(w).reshape(-1, 32)[1:1 + vk.n_public], proof)
+        except native.VerifyError:
+            return False
+    return True
 
 
 def run_serve(args) -> int:
@@ -266,7 +286,8 @@ def run_serve(args) -> int:
         log(f"load {load}: served {sv}; plain path {pl}")
         runs.append({"load_of_plain_path": load, "rate_per_s": rate, "served": sv,
                      "plain_path": pl})
-    log(f"bit-identical {same}")
+    verified = _verify_all([(circ, w, r, s) for _, circ, w, r, s in reqs], out_s)
+    log(f"bit-identical {same}; every served proof verifies: {verified}")
     line = {
         "metric": "Groth16 serving proofs/s + p50/p99 latency (Poisson arrivals, 8 resident "
                   "circuits of 2^16..2^22 constraints)",
@@ -288,9 +309,10 @@ def run_serve(args) -> int:
         },
         "by_load": runs,
         "bit_identical_to_plain": same,
+        "every_proof_verifies": verified,
     }
     print(json.dumps(line), flush=True)
-    return 0 if same else 1
+    return 0 if same and verified else 1
 
 
 def main() -> int:
@@ -369,6 +391,15 @@ def main() -> int:
     proofs_per_s = args.steps / total
     log(f"timed: {args.steps} proofs in {total:.3f} s -> {proofs_per_s:.2f} proofs/s, "
         f"p50 {percentile(lat, 50) * 1e3:.1f} ms")
+
+    # Every timed proof is the same (same witness, r and s): check it once against the
+    # circuit's verifying key with the host pairing verifier, outside the timed region.
+    from snarkrs_amd.verify import VerifyingKey, verify
+    tv = time.perf_counter()
+    verify(VerifyingKey.from_bytes(circ.vk_bytes()),
+           np.asarray(wit).reshape(-1, 32)[1:1 + key.n_public], proof)
+    verify_ms = (time.perf_counter() - tv) * 1e3
+    log(f"the proof verifies (host pairing check, {verify_ms:.1f} ms)")
 
     # Roofline of the dominant kernel, the G1 bucket accumulation, over the timed proofs.
     launches, entries, acc_ms = ks["launches"][0], ks["entries"][0], ks["accumulate_ms"][0]
@@ -453,6 +484,8 @@ def main() -> int:
         "latency_ms": {"p50": percentile(lat, 50) * 1e3, "p99": percentile(lat, 99) * 1e3,
                        "min": min(lat) * 1e3, "max": max(lat) * 1e3},
         "stage_us_per_proof": {k: v / args.steps for k, v in stage.as_dict().items()},
+        "proof_verifies": True,
+        "verify_ms": verify_ms,
         "msm_g1": msm,
         "roofline": roofline,
         "cpu_baseline": cpu,

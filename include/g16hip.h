@@ -42,7 +42,15 @@ typedef enum {
   G16HIP_DEVICE = 3,         /* ProveError::Device: a device fault, may succeed on retry */
   G16HIP_DEVICE_MEMORY = 4,  /* ProveError::DeviceMemory: refused before allocating */
   G16HIP_CONSTANT_WIRE = 5,  /* ProveError::ConstantWire: witness[0] != 1 */
-  G16HIP_ZERO_BLINDER = 6    /* ProveError::ZeroBlinder: r or s is zero */
+  G16HIP_ZERO_BLINDER = 6,   /* ProveError::ZeroBlinder: r or s is zero */
+  G16HIP_SELF_VERIFY = 7,    /* ProveError::SelfVerify: the proof failed the check of its result */
+  /* VerifyError, crates/groth16/src/verify.rs:16-31 */
+  G16HIP_PUBLIC_COUNT = 8,   /* VerifyError::PublicInputCount { got, want } */
+  G16HIP_EMPTY_KEY = 9,      /* VerifyError::EmptyVerifyingKey */
+  G16HIP_PAIRING_FAILED = 10,/* VerifyError::PairingFailed */
+  G16HIP_INVALID_PROOF = 11, /* VerifyError::InvalidProof: off the curve or subgroup, or infinity */
+  G16HIP_UNSAFE_KEY = 12,    /* VerifyError::UnsafeKey (ZkeyError::UnsafeVerifyingKey) */
+  G16HIP_BAD_JSON = 13       /* JsonError (crates/groth16/src/json.rs:39-41), ZkeyError::BadJson */
 } g16hip_status;
 
 typedef struct g16hip_device g16hip_device;
@@ -100,12 +108,23 @@ typedef struct {
   uint8_t h_g1[64];
 } g16hip_msm_out;
 
-/* Proof, crates/groth16/src/lib.rs:175-180. */
+/* Proof, crates/groth16/src/lib.rs:188-192. */
 typedef struct {
   uint8_t a[64];
   uint8_t b[128];
   uint8_t c[64];
 } g16hip_proof;
+
+/* VerifyingKey (crates/formats/src/lib.rs:353-360): points in the wire encoding above,
+ * borrowed for the call. ic holds n_ic = n_public + 1 G1 points, the constant wire's first. */
+typedef struct {
+  const uint8_t* alpha_g1; /* G1 */
+  const uint8_t* beta_g2;  /* G2 */
+  const uint8_t* gamma_g2; /* G2 */
+  const uint8_t* delta_g2; /* G2 */
+  const uint8_t* ic;       /* G1[n_ic] */
+  uint32_t n_ic;
+} g16hip_vk;
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* g16hip_last_error(void);
@@ -159,6 +178,64 @@ g16hip_status g16hip_prove_with_blinders_device(g16hip_circuit* c, const void* d
                                                 size_t n_witness, const uint8_t* r,
                                                 const uint8_t* s, g16hip_proof* proof,
                                                 g16hip_timings* t);
+
+/* prove (crates/groth16/src/prove.rs:12-41, :79-91): g16hip_prove_with_blinders, then the
+ * check of the result against the circuit's own verifying key (g16hip_verify below); a proof
+ * that fails it is G16HIP_SELF_VERIFY with the verifier's message. r and s come from the
+ * caller's CSPRNG (the reference's sample_blinders, prove.rs:138-146); zero is refused. The
+ * check's time goes to t->verify_us. */
+g16hip_status g16hip_prove(g16hip_circuit* c, const uint8_t* witness, size_t n_witness,
+                           const uint8_t* r, const uint8_t* s, g16hip_proof* proof,
+                           g16hip_timings* t);
+/* PreparedCircuit::key().vk: alpha_g1 | beta_g2 | gamma_g2 | delta_g2 | ic[n_public + 1], in
+ * the wire encoding (64 + 3 * 128 + 64 * (n_public + 1) bytes). */
+g16hip_status g16hip_circuit_vk(const g16hip_circuit* c, uint8_t* out);
+
+/* ---- Verification and snarkjs files (host code; no device needed). ---- */
+
+/* verify (crates/groth16/src/verify.rs:49-52): the key's structure checked
+ * (crates/formats/src/lib.rs:1326-1350, gamma_g2 != delta_g2 included), A and C on the curve,
+ * B on the curve and in the prime-order subgroup, none of them infinity, then
+ * e(-A, B) e(alpha, beta) e(L_bar, gamma) e(C, delta) = 1 with
+ * L_bar = IC[0] + sum_i public[i] IC[i + 1]. public_inputs: Fr[n_public] Montgomery.
+ * G16HIP_OK, or the VerifyError status. */
+g16hip_status g16hip_verify(const g16hip_vk* vk, const uint8_t* public_inputs, size_t n_public,
+                            const g16hip_proof* proof);
+/* verify_allowing_unblinded (verify.rs:66-76): every check but gamma_g2 != delta_g2. */
+g16hip_status g16hip_verify_allowing_unblinded(const g16hip_vk* vk, const uint8_t* public_inputs,
+                                               size_t n_public, const g16hip_proof* proof);
+/* verify_unchecked (verify.rs:104-134): the bare pairing check. */
+g16hip_status g16hip_verify_unchecked(const g16hip_vk* vk, const uint8_t* public_inputs,
+                                      size_t n_public, const g16hip_proof* proof);
+
+/* The optimal ate pairing (ark-bn254's Bn254::pairing, the reference's verifier arithmetic):
+ * *is_one = whether prod_i e(g1[i], g2[i]) = 1 (Bn254::multi_pairing). Points must be valid. */
+g16hip_status g16hip_pairing_check(const uint8_t* g1, const uint8_t* g2, size_t n, int* is_one);
+/* e(g1, g2) in Fq12 = Fq6[w]/(w^2 - v), Fq6 = Fq2[v]/(v^3 - (9 + u)): 12 Fq Montgomery
+ * coordinates, c0.{a0,a1,a2} then c1.{a0,a1,a2}, each Fq2 c0 then c1 (384 bytes). */
+g16hip_status g16hip_pairing(const uint8_t* g1, const uint8_t* g2, uint8_t* out);
+/* *valid = the point decodes, is on the curve and (group 2) in the prime-order subgroup. */
+g16hip_status g16hip_point_check(int group, const uint8_t* point, int* valid);
+
+/* proof_to_string / public_to_string (crates/groth16/src/json.rs:97-114): snarkjs'
+ * proof.json and public.json text, NUL-terminated. With buf NULL only *len (the text's length
+ * without the NUL) is set; otherwise cap must exceed it. */
+g16hip_status g16hip_proof_to_json(const g16hip_proof* proof, char* buf, size_t cap, size_t* len);
+g16hip_status g16hip_public_to_json(const uint8_t* public_inputs, size_t n, char* buf, size_t cap,
+                                    size_t* len);
+/* proof_from_str / public_from_str (json.rs:150-243): strict readers; coordinates must be
+ * below their modulus, z exactly 1 (or ffjavascript's infinity), points on the curve and in
+ * the subgroup. public: *n receives the count; with out NULL nothing else is written, else out
+ * holds cap scalars (Fr Montgomery). G16HIP_BAD_JSON with the reason on a malformed file. */
+g16hip_status g16hip_proof_from_json(const char* text, size_t len, g16hip_proof* out);
+g16hip_status g16hip_public_from_json(const char* text, size_t len, uint8_t* out, size_t cap,
+                                      size_t* n);
+/* VerifyingKey::from_json_str (check_structure != 0) / from_json_str_unchecked
+ * (crates/formats/src/lib.rs:1371-1430): snarkjs' verification_key.json. *n_public receives
+ * nPublic; out (cap bytes, NULL to ask for n_public only) receives the g16hip_circuit_vk
+ * layout. */
+g16hip_status g16hip_vk_from_json(const char* text, size_t len, int check_structure, uint8_t* out,
+                                  size_t cap, uint32_t* n_public);
 
 /* ---- Batched proving and serving (no counterpart in the reference, whose provers run one
  * proof per call; these are for callers that keep many proofs in flight on one GPU). ---- */

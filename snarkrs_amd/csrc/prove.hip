@@ -18,6 +18,7 @@
 #include <unordered_map>
 
 #include "runtime_impl.h"
+#include "verify.h"
 
 using namespace g16;
 
@@ -71,6 +72,7 @@ struct g16hip_circuit {
   bool merged = false; // bases stored as window tables (one bucket set per MSM)
   G1Aff alpha_g1, beta_g1, delta_g1;
   G2Aff beta_g2, delta_g2;
+  VkBytes vk;  // the verifying key, for the check of a proof's result
   DevBuf row_ptr[2], signal[2], value_id[2], values;
   DevBuf a_q, b1_q, b2_q, l_q, h_q;
   NttTables tables;
@@ -119,6 +121,7 @@ static void validate_key(const g16hip_key* k) {
   }
   if (aff_is_inf(g2_load(k->beta_g2)) || aff_is_inf(g2_load(k->delta_g2)))
     throw ArgError{"a key point is the point at infinity, which no honest setup produces"};
+  if (!k->gamma_g2 || !k->ic) throw ArgError{"the key has no verifying-key points (gamma_g2, ic)"};
 }
 
 static void upload_key(g16hip_circuit* C, const g16hip_key* k, hipStream_t s) {
@@ -133,6 +136,11 @@ static void upload_key(g16hip_circuit* C, const g16hip_key* k, hipStream_t s) {
   C->delta_g1 = g1_load(k->delta_g1);
   C->beta_g2 = g2_load(k->beta_g2);
   C->delta_g2 = g2_load(k->delta_g2);
+  std::memcpy(C->vk.alpha_g1, k->alpha_g1, 64);
+  std::memcpy(C->vk.beta_g2, k->beta_g2, 128);
+  std::memcpy(C->vk.gamma_g2, k->gamma_g2, 128);
+  std::memcpy(C->vk.delta_g2, k->delta_g2, 128);
+  C->vk.ic.assign(k->ic, k->ic + 64 * ((size_t)k->n_public + 1));
   for (int m = 0; m < 2; ++m) {
     uint32_t nnz = k->row_ptr[m][n];
     upload(C->row_ptr[m], k->row_ptr[m], 4 * (size_t)(n + 1), s);
@@ -883,6 +891,32 @@ g16hip_status g16hip_prove_with_blinders(g16hip_circuit* C, const uint8_t* w, si
                                          const uint8_t* r, const uint8_t* s, g16hip_proof* proof,
                                          g16hip_timings* t) {
   return prove_impl(C, w, n, false, r, s, proof, t);
+}
+
+// prove then self_verify (crates/groth16/src/prove.rs:66-91): the public inputs are
+// witness[1..=n_public], checked against the circuit's own verifying key.
+g16hip_status g16hip_prove(g16hip_circuit* C, const uint8_t* w, size_t n, const uint8_t* r,
+                           const uint8_t* s, g16hip_proof* proof, g16hip_timings* t) {
+  g16hip_status st = prove_impl(C, w, n, false, r, s, proof, t);
+  if (st != G16HIP_OK) return st;
+  Timer tv;
+  std::string why;
+  st = groth16_check(C->vk, w + 32, C->n_public, *proof, VerifyMode::Checked, &why);
+  if (t) t->verify_us += tv.us();
+  if (st != G16HIP_OK) return fail(G16HIP_SELF_VERIFY, "the proof failed its check: " + why);
+  return G16HIP_OK;
+}
+
+g16hip_status g16hip_circuit_vk(const g16hip_circuit* C, uint8_t* out) {
+  return guarded([&] {
+    if (!C || !out) throw ArgError{"null argument"};
+    std::memcpy(out, C->vk.alpha_g1, 64);
+    std::memcpy(out + 64, C->vk.beta_g2, 128);
+    std::memcpy(out + 192, C->vk.gamma_g2, 128);
+    std::memcpy(out + 320, C->vk.delta_g2, 128);
+    std::memcpy(out + 448, C->vk.ic.data(), C->vk.ic.size());
+    return G16HIP_OK;
+  });
 }
 
 g16hip_status g16hip_prove_with_blinders_device(g16hip_circuit* C, const void* w, size_t n,

@@ -1,0 +1,1635 @@
+// Groth16 verification over BN254 on the host, and snarkjs' proof.json / public.json /
+// verification_key.json encodings.
+//
+// The reference verifies with arkworks (ark-bn254 0.5.0) through
+// crates/groth16/src/verify.rs:49-175: the check e(A, B) = e(alpha, beta) e(L_bar, gamma)
+// e(C, delta) becomes one product of four Miller loops with A negated and a single final
+// exponentiation, and every point is validated first. This file computes the same check
+// with its own arithmetic:
+//
+//   * Fq, Fr: four 64-bit limbs, Montgomery form with R = 2^256 (so the boundary's 8 x u32
+//     limbs are these limbs unchanged), CIOS products with 128-bit intermediates;
+//   * the tower Fq2 = Fq[u]/(u^2 + 1), Fq6 = Fq2[v]/(v^3 - xi) with xi = 9 + u,
+//     Fq12 = Fq6[w]/(w^2 - v), so w^6 = xi;
+//   * G2 on the twist E': y^2 = x^3 + 3/xi, mapped into E(Fq12) by (x, y) -> (x w^2, y w^3);
+//   * the optimal ate Miller loop over s = 6x + 2 (x = 4965661367192848881, the BN
+//     parameter: q = 36x^4 + 36x^3 + 24x^2 + 6x + 1 and r = 36x^4 + 36x^3 + 18x^2 + 6x + 1,
+//     which start-up checks), with T in Jacobian coordinates and each line scaled by an
+//     Fq2 factor (the final exponentiation sends every element of Fq6 to 1), followed by the
+//     two Frobenius lines through pi(Q) and -pi^2(Q);
+//   * the final exponentiation f^((q^12 - 1)/r) as f^(q^6 - 1), then ^(q^2 + 1), then the
+//     hard part (q^4 - q^2 + 1)/r by the BN addition chain in x (Scott, Benger, Charlemagne,
+//     Dominguez Perez, Kachisa, "On the final exponentiation for calculating pairings on
+//     ordinary elliptic curves", 2009), which start-up checks against a plain
+//     exponentiation by the integer exponent.
+//
+// Frobenius constants xi^(k (q^j - 1)/6) are computed at start-up from their definition, not
+// tabulated.
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <thread>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "verify.h"
+
+namespace g16 {
+void set_last_error(const std::string& msg);
+}
+
+namespace {
+
+typedef unsigned __int128 u128;
+
+// ---------------------------------------------------------------------------------------
+// fixed-width unsigned integers (exponents and decimal conversion)
+// ---------------------------------------------------------------------------------------
+struct Big {
+  static constexpr int L = 16;  // 1024 bits
+  uint64_t v[L] = {0};
+  static Big of(const uint64_t* limbs, int n) {
+    Big b;
+    for (int i = 0; i < n; ++i) b.v[i] = limbs[i];
+    return b;
+  }
+  int bits() const {
+    for (int i = L - 1; i >= 0; --i)
+      if (v[i]) return 64 * i + 64 - __builtin_clzll(v[i]);
+    return 0;
+  }
+  bool bit(int i) const { return (v[i >> 6] >> (i & 63)) & 1; }
+  bool is_zero() const {
+    for (uint64_t x : v)
+      if (x) return false;
+    return true;
+  }
+};
+
+Big big_mul(const Big& a, const Big& b) {
+  Big out;
+  for (int i = 0; i < Big::L; ++i) {
+    if (!a.v[i]) continue;
+    uint64_t carry = 0;
+    for (int j = 0; i + j < Big::L; ++j) {
+      u128 t = (u128)a.v[i] * b.v[j] + out.v[i + j] + carry;
+      out.v[i + j] = (uint64_t)t;
+      carry = (uint64_t)(t >> 64);
+    }
+  }
+  return out;
+}
+
+Big big_add_small(const Big& a, uint64_t k) {
+  Big out = a;
+  for (int i = 0; i < Big::L && k; ++i) {
+    u128 t = (u128)out.v[i] + k;
+    out.v[i] = (uint64_t)t;
+    k = (uint64_t)(t >> 64);
+  }
+  return out;
+}
+
+Big big_add(const Big& a, const Big& b) {
+  Big out;
+  uint64_t carry = 0;
+  for (int i = 0; i < Big::L; ++i) {
+    u128 t = (u128)a.v[i] + b.v[i] + carry;
+    out.v[i] = (uint64_t)t;
+    carry = (uint64_t)(t >> 64);
+  }
+  return out;
+}
+
+Big big_sub(const Big& a, const Big& b) {
+  Big out;
+  uint64_t borrow = 0;
+  for (int i = 0; i < Big::L; ++i) {
+    u128 t = (u128)a.v[i] - b.v[i] - borrow;
+    out.v[i] = (uint64_t)t;
+    borrow = (uint64_t)(t >> 64) ? 1 : 0;
+  }
+  return out;
+}
+
+bool big_geq(const Big& a, const Big& b) {
+  for (int i = Big::L - 1; i >= 0; --i)
+    if (a.v[i] != b.v[i]) return a.v[i] > b.v[i];
+  return true;
+}
+
+// Schoolbook binary long division; `rem` receives a mod d.
+Big big_div(const Big& a, const Big& d, Big* rem) {
+  Big q, r;
+  for (int i = a.bits() - 1; i >= 0; --i) {
+    for (int k = Big::L - 1; k > 0; --k) r.v[k] = (r.v[k] << 1) | (r.v[k - 1] >> 63);
+    r.v[0] = (r.v[0] << 1) | (uint64_t)a.bit(i);
+    if (big_geq(r, d)) {
+      r = big_sub(r, d);
+      q.v[i >> 6] |= 1ull << (i & 63);
+    }
+  }
+  if (rem) *rem = r;
+  return q;
+}
+
+uint64_t big_divmod_small(Big& a, uint64_t d) {
+  u128 r = 0;
+  for (int i = Big::L - 1; i >= 0; --i) {
+    u128 cur = (r << 64) | a.v[i];
+    a.v[i] = (uint64_t)(cur / d);
+    r = cur % d;
+  }
+  return (uint64_t)r;
+}
+
+// ---------------------------------------------------------------------------------------
+// prime fields
+// ---------------------------------------------------------------------------------------
+struct QTag {  // base field q
+  static constexpr uint64_t P[4] = {0x3c208c16d87cfd47ull, 0x97816a916871ca8dull,
+                                    0xb85045b68181585dull, 0x30644e72e131a029ull};
+  static constexpr uint64_t INV = 0x87d20782e4866389ull;  // -q^-1 mod 2^64
+};
+struct RTag {  // scalar field r
+  static constexpr uint64_t P[4] = {0x43e1f593f0000001ull, 0x2833e84879b97091ull,
+                                    0xb85045b68181585dull, 0x30644e72e131a029ull};
+  static constexpr uint64_t INV = 0xc2e1f593efffffffull;
+};
+static_assert(QTag::P[0] * QTag::INV + 1 == 0, "q Montgomery constant");
+static_assert(RTag::P[0] * RTag::INV + 1 == 0, "r Montgomery constant");
+
+template <class T>
+struct Fp {
+  uint64_t v[4];
+};
+typedef Fp<QTag> Fq;
+typedef Fp<RTag> Fr;
+
+template <class T>
+inline bool lt_p(const uint64_t a[4]) {
+  for (int i = 3; i >= 0; --i)
+    if (a[i] != T::P[i]) return a[i] < T::P[i];
+  return false;
+}
+
+template <class T>
+inline void sub_p(uint64_t a[4]) {
+  uint64_t borrow = 0;
+  for (int i = 0; i < 4; ++i) {
+    u128 t = (u128)a[i] - T::P[i] - borrow;
+    a[i] = (uint64_t)t;
+    borrow = (uint64_t)(t >> 64) ? 1 : 0;
+  }
+}
+
+template <class T>
+inline Fp<T> f_zero() {
+  return Fp<T>{{0, 0, 0, 0}};
+}
+
+template <class T>
+inline bool f_is_zero(const Fp<T>& a) {
+  return !(a.v[0] | a.v[1] | a.v[2] | a.v[3]);
+}
+
+template <class T>
+inline bool f_eq(const Fp<T>& a, const Fp<T>& b) {
+  return a.v[0] == b.v[0] && a.v[1] == b.v[1] && a.v[2] == b.v[2] && a.v[3] == b.v[3];
+}
+
+// a - p when a >= p, else a, without branches (a < 2p < 2^256).
+template <class T>
+inline void reduce_once(uint64_t a[4]) {
+  uint64_t d[4], borrow = 0;
+  for (int i = 0; i < 4; ++i) {
+    u128 t = (u128)a[i] - T::P[i] - borrow;
+    d[i] = (uint64_t)t;
+    borrow = (uint64_t)(t >> 64) & 1;
+  }
+  uint64_t keep = 0 - borrow;  // all ones when a < p
+  for (int i = 0; i < 4; ++i) a[i] = (a[i] & keep) | (d[i] & ~keep);
+}
+
+template <class T>
+inline Fp<T> f_add(const Fp<T>& a, const Fp<T>& b) {
+  Fp<T> o;
+  uint64_t carry = 0;
+  for (int i = 0; i < 4; ++i) {
+    u128 t = (u128)a.v[i] + b.v[i] + carry;
+    o.v[i] = (uint64_t)t;
+    carry = (uint64_t)(t >> 64);
+  }
+  reduce_once<T>(o.v);  // both moduli are below 2^254: no carry out
+  return o;
+}
+
+template <class T>
+inline Fp<T> f_sub(const Fp<T>& a, const Fp<T>& b) {
+  Fp<T> o;
+  uint64_t borrow = 0;
+  for (int i = 0; i < 4; ++i) {
+    u128 t = (u128)a.v[i] - b.v[i] - borrow;
+    o.v[i] = (uint64_t)t;
+    borrow = (uint64_t)(t >> 64) & 1;
+  }
+  uint64_t mask = 0 - borrow, carry = 0;
+  for (int i = 0; i < 4; ++i) {
+    u128 t = (u128)o.v[i] + (T::P[i] & mask) + carry;
+    o.v[i] = (uint64_t)t;
+    carry = (uint64_t)(t >> 64);
+  }
+  return o;
+}
+
+template <class T>
+inline Fp<T> f_neg(const Fp<T>& a) {
+  return f_is_zero(a) ? a : f_sub(f_zero<T>(), a);
+}
+
+// a b / 2^256 mod p: CIOS with the carry words dropped, which is exact when the modulus'
+// top limb is below 2^62 (true of both moduli here), then one conditional subtraction.
+template <class T>
+inline Fp<T> f_mul(const Fp<T>& a, const Fp<T>& b) {
+  static_assert(T::P[3] < (1ull << 62), "the carry-free product needs p < 2^254");
+  uint64_t t[4] = {0, 0, 0, 0};
+  for (int i = 0; i < 4; ++i) {
+    u128 s = (u128)a.v[0] * b.v[i] + t[0];
+    uint64_t A = (uint64_t)(s >> 64);
+    t[0] = (uint64_t)s;
+    uint64_t m = t[0] * T::INV;
+    u128 u = (u128)m * T::P[0] + t[0];
+    uint64_t C = (uint64_t)(u >> 64);
+    for (int j = 1; j < 4; ++j) {
+      s = (u128)a.v[j] * b.v[i] + t[j] + A;
+      A = (uint64_t)(s >> 64);
+      u = (u128)m * T::P[j] + (uint64_t)s + C;
+      C = (uint64_t)(u >> 64);
+      t[j - 1] = (uint64_t)u;
+    }
+    t[3] = C + A;
+  }
+  Fp<T> o{{t[0], t[1], t[2], t[3]}};
+  reduce_once<T>(o.v);
+  return o;
+}
+
+template <class T>
+inline Fp<T> f_sqr(const Fp<T>& a) {
+  return f_mul(a, a);
+}
+
+template <class T>
+inline Fp<T> f_dbl(const Fp<T>& a) {
+  return f_add(a, a);
+}
+
+// 2^256 mod p and 2^512 mod p by doubling, computed once.
+template <class T>
+struct Consts {
+  Fp<T> one, r2;
+  Consts() {
+    Fp<T> x{{1, 0, 0, 0}};
+    for (int i = 0; i < 256; ++i) x = f_add(x, x);
+    one = x;
+    for (int i = 0; i < 256; ++i) x = f_add(x, x);
+    r2 = x;
+  }
+};
+template <class T>
+const Consts<T>& consts() {
+  static const Consts<T> c;
+  return c;
+}
+
+template <class T>
+inline Fp<T> f_one() {
+  return consts<T>().one;
+}
+
+template <class T>
+inline bool f_is_one(const Fp<T>& a) {
+  return f_eq(a, f_one<T>());
+}
+
+template <class T>
+inline Fp<T> f_from_u64(uint64_t k) {
+  return f_mul(Fp<T>{{k, 0, 0, 0}}, consts<T>().r2);
+}
+
+// Montgomery form -> the integer, and back.
+template <class T>
+inline Fp<T> f_to_int(const Fp<T>& a) {
+  return f_mul(a, Fp<T>{{1, 0, 0, 0}});
+}
+template <class T>
+inline Fp<T> f_from_int(const Fp<T>& a) {
+  return f_mul(a, consts<T>().r2);
+}
+
+template <class T>
+Fp<T> f_pow(const Fp<T>& a, const Big& e) {
+  Fp<T> acc = f_one<T>();
+  for (int i = e.bits() - 1; i >= 0; --i) {
+    acc = f_sqr(acc);
+    if (e.bit(i)) acc = f_mul(acc, a);
+  }
+  return acc;
+}
+
+template <class T>
+Fp<T> f_inv(const Fp<T>& a) {  // a^(p - 2); 0 maps to 0
+  Big e = big_sub(Big::of(T::P, 4), Big::of((const uint64_t[]){2}, 1));
+  return f_pow(a, e);
+}
+
+template <class T>
+inline bool f_load(const uint8_t* p, Fp<T>* out) {
+  std::memcpy(out->v, p, 32);
+  return lt_p<T>(out->v);
+}
+
+template <class T>
+inline void f_store(uint8_t* p, const Fp<T>& a) {
+  std::memcpy(p, a.v, 32);
+}
+
+// ---------------------------------------------------------------------------------------
+// the tower
+// ---------------------------------------------------------------------------------------
+struct Fq2 {
+  Fq c0, c1;
+};
+struct Fq6 {
+  Fq2 a0, a1, a2;
+};
+struct Fq12 {
+  Fq6 c0, c1;
+};
+
+inline Fq2 e2_zero() { return {f_zero<QTag>(), f_zero<QTag>()}; }
+inline Fq2 e2_one() { return {f_one<QTag>(), f_zero<QTag>()}; }
+inline bool e2_is_zero(const Fq2& a) { return f_is_zero(a.c0) && f_is_zero(a.c1); }
+inline bool e2_eq(const Fq2& a, const Fq2& b) { return f_eq(a.c0, b.c0) && f_eq(a.c1, b.c1); }
+inline Fq2 e2_add(const Fq2& a, const Fq2& b) { return {f_add(a.c0, b.c0), f_add(a.c1, b.c1)}; }
+inline Fq2 e2_sub(const Fq2& a, const Fq2& b) { return {f_sub(a.c0, b.c0), f_sub(a.c1, b.c1)}; }
+inline Fq2 e2_neg(const Fq2& a) { return {f_neg(a.c0), f_neg(a.c1)}; }
+inline Fq2 e2_dbl(const Fq2& a) { return e2_add(a, a); }
+inline Fq2 e2_conj(const Fq2& a) { return {a.c0, f_neg(a.c1)}; }
+inline Fq2 e2_mul(const Fq2& a, const Fq2& b) {
+  Fq t0 = f_mul(a.c0, b.c0), t1 = f_mul(a.c1, b.c1);
+  Fq m = f_mul(f_add(a.c0, a.c1), f_add(b.c0, b.c1));
+  return {f_sub(t0, t1), f_sub(f_sub(m, t0), t1)};
+}
+inline Fq2 e2_sqr(const Fq2& a) {  // (a0 + a1)(a0 - a1) + 2 a0 a1 u
+  Fq m = f_mul(a.c0, a.c1);
+  return {f_mul(f_add(a.c0, a.c1), f_sub(a.c0, a.c1)), f_add(m, m)};
+}
+inline Fq2 e2_mul_fq(const Fq2& a, const Fq& k) { return {f_mul(a.c0, k), f_mul(a.c1, k)}; }
+// a (9 + u) = (9 a0 - a1) + (a0 + 9 a1) u
+inline Fq2 e2_mul_xi(const Fq2& a) {
+  Fq n0 = f_add(f_dbl(f_dbl(f_dbl(a.c0))), a.c0);
+  Fq n1 = f_add(f_dbl(f_dbl(f_dbl(a.c1))), a.c1);
+  return {f_sub(n0, a.c1), f_add(a.c0, n1)};
+}
+inline Fq2 e2_inv(const Fq2& a) {  // (a0 - a1 u) / (a0^2 + a1^2)
+  Fq d = f_inv(f_add(f_sqr(a.c0), f_sqr(a.c1)));
+  return {f_mul(a.c0, d), f_neg(f_mul(a.c1, d))};
+}
+Fq2 e2_pow(const Fq2& a, const Big& e) {
+  Fq2 acc = e2_one();
+  for (int i = e.bits() - 1; i >= 0; --i) {
+    acc = e2_sqr(acc);
+    if (e.bit(i)) acc = e2_mul(acc, a);
+  }
+  return acc;
+}
+
+inline Fq6 e6_zero() { return {e2_zero(), e2_zero(), e2_zero()}; }
+inline Fq6 e6_one() { return {e2_one(), e2_zero(), e2_zero()}; }
+inline bool e6_eq(const Fq6& a, const Fq6& b) {
+  return e2_eq(a.a0, b.a0) && e2_eq(a.a1, b.a1) && e2_eq(a.a2, b.a2);
+}
+inline Fq6 e6_add(const Fq6& a, const Fq6& b) {
+  return {e2_add(a.a0, b.a0), e2_add(a.a1, b.a1), e2_add(a.a2, b.a2)};
+}
+inline Fq6 e6_sub(const Fq6& a, const Fq6& b) {
+  return {e2_sub(a.a0, b.a0), e2_sub(a.a1, b.a1), e2_sub(a.a2, b.a2)};
+}
+inline Fq6 e6_neg(const Fq6& a) { return {e2_neg(a.a0), e2_neg(a.a1), e2_neg(a.a2)}; }
+// (a0 + a1 v + a2 v^2)(b0 + b1 v + b2 v^2) with v^3 = xi, three-way Karatsuba.
+inline Fq6 e6_mul(const Fq6& a, const Fq6& b) {
+  Fq2 v0 = e2_mul(a.a0, b.a0), v1 = e2_mul(a.a1, b.a1), v2 = e2_mul(a.a2, b.a2);
+  Fq2 t12 = e2_sub(e2_sub(e2_mul(e2_add(a.a1, a.a2), e2_add(b.a1, b.a2)), v1), v2);
+  Fq2 t01 = e2_sub(e2_sub(e2_mul(e2_add(a.a0, a.a1), e2_add(b.a0, b.a1)), v0), v1);
+  Fq2 t02 = e2_sub(e2_sub(e2_mul(e2_add(a.a0, a.a2), e2_add(b.a0, b.a2)), v0), v2);
+  return {e2_add(v0, e2_mul_xi(t12)), e2_add(t01, e2_mul_xi(v2)), e2_add(t02, v1)};
+}
+inline Fq6 e6_sqr(const Fq6& a) { return e6_mul(a, a); }
+// a * v = xi a2 + a0 v + a1 v^2
+inline Fq6 e6_mul_v(const Fq6& a) { return {e2_mul_xi(a.a2), a.a0, a.a1}; }
+inline Fq6 e6_mul_e2(const Fq6& a, const Fq2& k) {
+  return {e2_mul(a.a0, k), e2_mul(a.a1, k), e2_mul(a.a2, k)};
+}
+// a (b0 + b1 v): the sparse factor of a line.
+inline Fq6 e6_mul_01(const Fq6& a, const Fq2& b0, const Fq2& b1) {
+  Fq2 v0 = e2_mul(a.a0, b0), v1 = e2_mul(a.a1, b1);
+  Fq2 c0 = e2_add(v0, e2_mul_xi(e2_mul(a.a2, b1)));
+  Fq2 c1 = e2_sub(e2_sub(e2_mul(e2_add(a.a0, a.a1), e2_add(b0, b1)), v0), v1);
+  Fq2 c2 = e2_add(e2_mul(a.a2, b0), v1);
+  return {c0, c1, c2};
+}
+inline Fq6 e6_inv(const Fq6& a) {
+  Fq2 t0 = e2_sub(e2_sqr(a.a0), e2_mul_xi(e2_mul(a.a1, a.a2)));
+  Fq2 t1 = e2_sub(e2_mul_xi(e2_sqr(a.a2)), e2_mul(a.a0, a.a1));
+  Fq2 t2 = e2_sub(e2_sqr(a.a1), e2_mul(a.a0, a.a2));
+  Fq2 det = e2_add(e2_mul(a.a0, t0), e2_mul_xi(e2_add(e2_mul(a.a2, t1), e2_mul(a.a1, t2))));
+  Fq2 di = e2_inv(det);
+  return {e2_mul(t0, di), e2_mul(t1, di), e2_mul(t2, di)};
+}
+
+inline Fq12 e12_one() { return {e6_one(), e6_zero()}; }
+inline bool e12_eq(const Fq12& a, const Fq12& b) { return e6_eq(a.c0, b.c0) && e6_eq(a.c1, b.c1); }
+inline bool e12_is_one(const Fq12& a) { return e12_eq(a, e12_one()); }
+inline Fq12 e12_mul(const Fq12& a, const Fq12& b) {
+  Fq6 t0 = e6_mul(a.c0, b.c0), t1 = e6_mul(a.c1, b.c1);
+  Fq6 m = e6_mul(e6_add(a.c0, a.c1), e6_add(b.c0, b.c1));
+  return {e6_add(t0, e6_mul_v(t1)), e6_sub(e6_sub(m, t0), t1)};
+}
+inline Fq12 e12_sqr(const Fq12& a) {  // (c0 + c1 w)^2 = c0^2 + v c1^2 + 2 c0 c1 w
+  Fq6 t = e6_mul(a.c0, a.c1);
+  Fq6 m = e6_mul(e6_add(a.c0, a.c1), e6_add(a.c0, e6_mul_v(a.c1)));
+  return {e6_sub(e6_sub(m, t), e6_mul_v(t)), e6_add(t, t)};
+}
+inline Fq12 e12_conj(const Fq12& a) { return {a.c0, e6_neg(a.c1)}; }
+// Squaring in the cyclotomic subgroup (where the easy part of the final exponentiation lands),
+// by Granger and Scott, "Faster squaring in the cyclotomic subgroup of sixth degree extensions" (PKC 2010).
+// View Fq12 as Fq4[w]/(w^3 - s), Fq4 = Fq2[s]/(s^2 - xi), s = w^3: f = A + B w + C w^2 with
+// A = g0 + g3 s, B = g1 + g4 s, C = g2 + g5 s (g_k the coefficient of w^k). Then
+// f^2 = (3 A^2 - 2 conj(A)) + (3 s C^2 + 2 conj(B)) w + (3 B^2 - 2 conj(C)) w^2, conj: s -> -s.
+inline void fq4_sqr(const Fq2& x, const Fq2& y, Fq2* r0, Fq2* r1) {  // (x + y s)^2
+  Fq2 t0 = e2_sqr(x), t1 = e2_sqr(y);
+  *r0 = e2_add(t0, e2_mul_xi(t1));
+  *r1 = e2_sub(e2_sub(e2_sqr(e2_add(x, y)), t0), t1);
+}
+inline Fq2 e2_triple_minus_2(const Fq2& t, const Fq2& g) {  // 3t - 2g
+  Fq2 d = e2_sub(t, g);
+  return e2_add(e2_dbl(d), t);
+}
+inline Fq2 e2_triple_plus_2(const Fq2& t, const Fq2& g) {  // 3t + 2g
+  Fq2 d = e2_add(t, g);
+  return e2_add(e2_dbl(d), t);
+}
+inline Fq12 e12_cyc_sqr(const Fq12& f) {
+  const Fq2 &g0 = f.c0.a0, &g1 = f.c1.a0, &g2 = f.c0.a1, &g3 = f.c1.a1, &g4 = f.c0.a2,
+            &g5 = f.c1.a2;
+  Fq2 a, b, c, d, e, h;
+  fq4_sqr(g0, g3, &a, &b);  // A^2
+  fq4_sqr(g2, g5, &c, &d);  // C^2
+  fq4_sqr(g1, g4, &e, &h);  // B^2
+  Fq12 o;
+  o.c0.a0 = e2_triple_minus_2(a, g0);
+  o.c1.a1 = e2_triple_plus_2(b, g3);
+  o.c1.a0 = e2_triple_plus_2(e2_mul_xi(d), g1);
+  o.c0.a2 = e2_triple_minus_2(c, g4);
+  o.c0.a1 = e2_triple_minus_2(e, g2);
+  o.c1.a2 = e2_triple_plus_2(h, g5);
+  return o;
+}
+inline Fq12 e12_inv(const Fq12& a) {  // (c0 - c1 w) / (c0^2 - v c1^2)
+  Fq6 d = e6_inv(e6_sub(e6_sqr(a.c0), e6_mul_v(e6_sqr(a.c1))));
+  return {e6_mul(a.c0, d), e6_neg(e6_mul(a.c1, d))};
+}
+// The line l0 + (l1 + l2 v) w.
+inline Fq12 e12_mul_line(const Fq12& f, const Fq2& l0, const Fq2& l1, const Fq2& l2) {
+  Fq6 t0 = e6_mul_e2(f.c0, l0);
+  Fq6 t1 = e6_mul_01(f.c1, l1, l2);
+  Fq6 m = e6_mul_01(e6_add(f.c0, f.c1), e2_add(l0, l1), l2);
+  return {e6_add(t0, e6_mul_v(t1)), e6_sub(e6_sub(m, t0), t1)};
+}
+Fq12 e12_pow(const Fq12& a, const Big& e) {
+  Fq12 acc = e12_one();
+  for (int i = e.bits() - 1; i >= 0; --i) {
+    acc = e12_sqr(acc);
+    if (e.bit(i)) acc = e12_mul(acc, a);
+  }
+  return acc;
+}
+
+// ---------------------------------------------------------------------------------------
+// constants derived at start-up
+// ---------------------------------------------------------------------------------------
+constexpr uint64_t BN_X = 4965661367192848881ull;
+
+struct Tower {
+  Fq b1;                 // 3
+  Fq2 b2;                // 3 / xi
+  Fq2 gamma[4][6];       // gamma[j][k] = xi^(k (q^j - 1) / 6), j = 1..3
+  Big hard;              // (q^4 - q^2 + 1) / r
+  Big r;
+  bool ok = false;
+  std::string err;
+  Tower() {
+    Big q = Big::of(QTag::P, 4);
+    r = Big::of(RTag::P, 4);
+    Big x = Big::of(&BN_X, 1);
+    Big x2 = big_mul(x, x), x3 = big_mul(x2, x), x4 = big_mul(x3, x);
+    auto k = [](uint64_t c, const Big& b) { return big_mul(Big::of(&c, 1), b); };
+    // q and r from x: everything below relies on them being the BN polynomials.
+    Big base = big_add(big_add(k(36, x4), k(36, x3)), big_add_small(k(6, x), 1));
+    Big q_x = big_add(base, k(24, x2)), r_x = big_add(base, k(18, x2));
+    if (!big_sub(q_x, q).is_zero() || !big_sub(r_x, r).is_zero()) {
+      err = "the moduli are not the BN polynomials in x";
+      return;
+    }
+    b1 = f_from_u64<QTag>(3);
+    Fq2 xi{f_from_u64<QTag>(9), f_from_u64<QTag>(1)};
+    b2 = e2_mul(Fq2{b1, f_zero<QTag>()}, e2_inv(xi));
+    Big qj = q;
+    Big one = Big::of((const uint64_t[]){1}, 1);
+    for (int j = 1; j <= 3; ++j) {
+      Big e = big_sub(qj, one);
+      Big rem;
+      Big e6 = big_div(e, Big::of((const uint64_t[]){6}, 1), &rem);
+      if (!rem.is_zero()) {
+        err = "6 does not divide q^j - 1";
+        return;
+      }
+      Fq2 g = e2_pow(xi, e6);
+      gamma[j][0] = e2_one();
+      for (int kk = 1; kk < 6; ++kk) gamma[j][kk] = e2_mul(gamma[j][kk - 1], g);
+      qj = big_mul(qj, q);
+    }
+    Big q2 = big_mul(q, q);
+    Big q4 = big_mul(q2, q2);
+    Big rem;
+    hard = big_div(big_add_small(big_sub(q4, q2), 1), r, &rem);
+    if (!rem.is_zero()) {
+      err = "r does not divide q^4 - q^2 + 1";
+      return;
+    }
+    ok = true;
+  }
+};
+
+const Tower& tower() {
+  static const Tower t;
+  return t;
+}
+
+// f^(q^j): the coefficient of w^k (k = 0..5 over w^0 = c0.a0, w = c1.a0, w^2 = c0.a1,
+// w^3 = c1.a1, w^4 = c0.a2, w^5 = c1.a2) goes to frob^j(g_k) * xi^(k (q^j - 1)/6).
+Fq12 e12_frob(const Fq12& f, int j) {
+  const Tower& T = tower();
+  auto fr = [&](const Fq2& g, int k) {
+    Fq2 h = (j & 1) ? e2_conj(g) : g;
+    return e2_mul(h, T.gamma[j][k]);
+  };
+  return {{fr(f.c0.a0, 0), fr(f.c0.a1, 2), fr(f.c0.a2, 4)},
+          {fr(f.c1.a0, 1), fr(f.c1.a1, 3), fr(f.c1.a2, 5)}};
+}
+
+// ---------------------------------------------------------------------------------------
+// curve points
+// ---------------------------------------------------------------------------------------
+struct G1A {
+  Fq x, y;
+  bool inf;
+};
+struct G2A {
+  Fq2 x, y;
+  bool inf;
+};
+
+template <class F>
+struct Jac {
+  F X, Y, Z;  // x = X/Z^2, y = Y/Z^3; Z = 0 is infinity
+};
+
+// Field operations by type, so the Jacobian formulas serve both groups.
+struct OpsQ {
+  typedef Fq F;
+  static F zero() { return f_zero<QTag>(); }
+  static F one() { return f_one<QTag>(); }
+  static bool is_zero(const F& a) { return f_is_zero(a); }
+  static bool eq(const F& a, const F& b) { return f_eq(a, b); }
+  static F add(const F& a, const F& b) { return f_add(a, b); }
+  static F sub(const F& a, const F& b) { return f_sub(a, b); }
+  static F mul(const F& a, const F& b) { return f_mul(a, b); }
+  static F sqr(const F& a) { return f_sqr(a); }
+  static F neg(const F& a) { return f_neg(a); }
+  static F inv(const F& a) { return f_inv(a); }
+};
+struct OpsQ2 {
+  typedef Fq2 F;
+  static F zero() { return e2_zero(); }
+  static F one() { return e2_one(); }
+  static bool is_zero(const F& a) { return e2_is_zero(a); }
+  static bool eq(const F& a, const F& b) { return e2_eq(a, b); }
+  static F add(const F& a, const F& b) { return e2_add(a, b); }
+  static F sub(const F& a, const F& b) { return e2_sub(a, b); }
+  static F mul(const F& a, const F& b) { return e2_mul(a, b); }
+  static F sqr(const F& a) { return e2_sqr(a); }
+  static F neg(const F& a) { return e2_neg(a); }
+  static F inv(const F& a) { return e2_inv(a); }
+};
+
+template <class O>
+Jac<typename O::F> jac_dbl(const Jac<typename O::F>& p) {
+  typedef typename O::F F;
+  if (O::is_zero(p.Z)) return p;
+  F A = O::sqr(p.X), B = O::sqr(p.Y), C = O::sqr(B);
+  F D = O::sub(O::sub(O::sqr(O::add(p.X, B)), A), C);
+  D = O::add(D, D);
+  F E = O::add(O::add(A, A), A), Fv = O::sqr(E);
+  Jac<F> o;
+  o.X = O::sub(Fv, O::add(D, D));
+  F C8 = O::add(C, C);
+  C8 = O::add(C8, C8);
+  C8 = O::add(C8, C8);
+  o.Y = O::sub(O::mul(E, O::sub(D, o.X)), C8);
+  F YZ = O::mul(p.Y, p.Z);
+  o.Z = O::add(YZ, YZ);
+  return o;
+}
+
+// Complete over the cases a scalar multiplication meets: either input infinity, P = Q, P = -Q.
+template <class O>
+Jac<typename O::F> jac_add(const Jac<typename O::F>& p, const Jac<typename O::F>& q) {
+  typedef typename O::F F;
+  if (O::is_zero(p.Z)) return q;
+  if (O::is_zero(q.Z)) return p;
+  F Z1Z1 = O::sqr(p.Z), Z2Z2 = O::sqr(q.Z);
+  F U1 = O::mul(p.X, Z2Z2), U2 = O::mul(q.X, Z1Z1);
+  F S1 = O::mul(O::mul(p.Y, q.Z), Z2Z2), S2 = O::mul(O::mul(q.Y, p.Z), Z1Z1);
+  F H = O::sub(U2, U1), R = O::sub(S2, S1);
+  if (O::is_zero(H)) {
+    if (O::is_zero(R)) return jac_dbl<O>(p);
+    return Jac<F>{O::one(), O::one(), O::zero()};
+  }
+  F HH = O::sqr(H), HHH = O::mul(H, HH), V = O::mul(U1, HH);
+  Jac<F> o;
+  o.X = O::sub(O::sub(O::sqr(R), HHH), O::add(V, V));
+  o.Y = O::sub(O::mul(R, O::sub(V, o.X)), O::mul(S1, HHH));
+  o.Z = O::mul(O::mul(p.Z, q.Z), H);
+  return o;
+}
+
+template <class O, class A>
+Jac<typename O::F> jac_from(const A& a) {
+  if (a.inf) return {O::one(), O::one(), O::zero()};
+  return {a.x, a.y, O::one()};
+}
+
+template <class O, class A>
+A jac_to_aff(const Jac<typename O::F>& p) {
+  A a;
+  if (O::is_zero(p.Z)) {
+    a.x = O::zero();
+    a.y = O::zero();
+    a.inf = true;
+    return a;
+  }
+  typename O::F zi = O::inv(p.Z), zi2 = O::sqr(zi);
+  a.x = O::mul(p.X, zi2);
+  a.y = O::mul(O::mul(p.Y, zi2), zi);
+  a.inf = false;
+  return a;
+}
+
+template <class O>
+Jac<typename O::F> jac_mul(const Jac<typename O::F>& p, const Big& k) {
+  Jac<typename O::F> acc{O::one(), O::one(), O::zero()};
+  for (int i = k.bits() - 1; i >= 0; --i) {
+    acc = jac_dbl<O>(acc);
+    if (k.bit(i)) acc = jac_add<O>(acc, p);
+  }
+  return acc;
+}
+
+bool g1_on_curve(const G1A& p) {
+  if (p.inf) return true;
+  return f_eq(f_sqr(p.y), f_add(f_mul(f_sqr(p.x), p.x), tower().b1));
+}
+bool g2_on_curve(const G2A& p) {
+  if (p.inf) return true;
+  return e2_eq(e2_sqr(p.y), e2_add(e2_mul(e2_sqr(p.x), p.x), tower().b2));
+}
+// [r] P = infinity. G1 has cofactor 1, so on the curve is in the subgroup there.
+bool g2_in_subgroup(const G2A& p) {
+  if (p.inf) return true;
+  return OpsQ2::is_zero(jac_mul<OpsQ2>(jac_from<OpsQ2>(p), tower().r).Z);
+}
+
+// Wire encoding: 32-byte Montgomery coordinates, all zero = infinity. False when a coordinate
+// is not below q.
+bool load_g1(const uint8_t* b, G1A* p) {
+  bool zero = true;
+  for (int i = 0; i < 64; ++i) zero = zero && b[i] == 0;
+  p->inf = zero;
+  return f_load(b, &p->x) && f_load(b + 32, &p->y);
+}
+bool load_g2(const uint8_t* b, G2A* p) {
+  bool zero = true;
+  for (int i = 0; i < 128; ++i) zero = zero && b[i] == 0;
+  p->inf = zero;
+  return f_load(b, &p->x.c0) && f_load(b + 32, &p->x.c1) && f_load(b + 64, &p->y.c0) &&
+         f_load(b + 96, &p->y.c1);
+}
+void store_g1(uint8_t* b, const G1A& p) {
+  if (p.inf) {
+    std::memset(b, 0, 64);
+    return;
+  }
+  f_store(b, p.x);
+  f_store(b + 32, p.y);
+}
+
+// ---------------------------------------------------------------------------------------
+// the pairing
+// ---------------------------------------------------------------------------------------
+struct Line {
+  Fq2 l0, l1, l2;  // l0 + (l1 + l2 v) w
+};
+
+// T <- 2T and the tangent at T evaluated at P, scaled by 2 Y Z^3:
+// l0 = 2 Y Z^3 yP, l1 = -3 X^2 Z^2 xP, l2 = 3 X^3 - 2 Y^2.
+inline Line dbl_step(Jac<Fq2>& T, const G1A& P) {
+  Fq2 A = e2_sqr(T.X), B = e2_sqr(T.Y), C = e2_sqr(B), ZZ = e2_sqr(T.Z);
+  Fq2 D = e2_sub(e2_sub(e2_sqr(e2_add(T.X, B)), A), C);
+  D = e2_dbl(D);
+  Fq2 E = e2_add(e2_dbl(A), A);
+  Line l;
+  l.l2 = e2_sub(e2_mul(E, T.X), e2_dbl(B));
+  l.l1 = e2_neg(e2_mul_fq(e2_mul(E, ZZ), P.x));
+  Fq2 X3 = e2_sub(e2_sqr(E), e2_dbl(D));
+  Fq2 Y3 = e2_sub(e2_mul(E, e2_sub(D, X3)), e2_dbl(e2_dbl(e2_dbl(C))));
+  Fq2 Z3 = e2_dbl(e2_mul(T.Y, T.Z));
+  l.l0 = e2_mul_fq(e2_mul(Z3, ZZ), P.y);
+  T = {X3, Y3, Z3};
+  return l;
+}
+
+// T <- T + Q (Q affine, T != +-Q) and the chord through T and Q at P, scaled by Z3 = Z H:
+// l0 = Z3 yP, l1 = -R xP, l2 = R xQ - Z3 yQ.
+inline Line add_step(Jac<Fq2>& T, const Fq2& qx, const Fq2& qy, const G1A& P) {
+  Fq2 ZZ = e2_sqr(T.Z);
+  Fq2 U2 = e2_mul(qx, ZZ), S2 = e2_mul(e2_mul(qy, T.Z), ZZ);
+  Fq2 H = e2_sub(U2, T.X), R = e2_sub(S2, T.Y);
+  Fq2 HH = e2_sqr(H), HHH = e2_mul(H, HH), V = e2_mul(T.X, HH);
+  Fq2 X3 = e2_sub(e2_sub(e2_sqr(R), HHH), e2_dbl(V));
+  Fq2 Y3 = e2_sub(e2_mul(R, e2_sub(V, X3)), e2_mul(T.Y, HHH));
+  Fq2 Z3 = e2_mul(T.Z, H);
+  Line l;
+  l.l0 = e2_mul_fq(Z3, P.y);
+  l.l1 = e2_neg(e2_mul_fq(R, P.x));
+  l.l2 = e2_sub(e2_mul(R, qx), e2_mul(Z3, qy));
+  T = {X3, Y3, Z3};
+  return l;
+}
+
+// Product over the pairs of the optimal ate Miller loop f_{s,Q}(P) l_{sQ, pi(Q)}(P)
+// l_{sQ + pi(Q), -pi^2(Q)}(P), s = 6x + 2. Pairs with a point at infinity contribute 1.
+Fq12 miller_loop(const std::vector<std::pair<G1A, G2A>>& pairs) {
+  const Tower& TW = tower();
+  std::vector<std::pair<G1A, G2A>> live;
+  for (const auto& pr : pairs)
+    if (!pr.first.inf && !pr.second.inf) live.push_back(pr);
+  std::vector<Jac<Fq2>> T;
+  for (const auto& pr : live) T.push_back({pr.second.x, pr.second.y, e2_one()});
+  const u128 s = (u128)6 * BN_X + 2;
+  int top = 127;
+  while (!((s >> top) & 1)) --top;
+  Fq12 f = e12_one();
+  for (int i = top - 1; i >= 0; --i) {
+    f = e12_sqr(f);
+    for (size_t k = 0; k < live.size(); ++k) {
+      Line l = dbl_step(T[k], live[k].first);
+      f = e12_mul_line(f, l.l0, l.l1, l.l2);
+    }
+    if ((s >> i) & 1) {
+      for (size_t k = 0; k < live.size(); ++k) {
+        Line l = add_step(T[k], live[k].second.x, live[k].second.y, live[k].first);
+        f = e12_mul_line(f, l.l0, l.l1, l.l2);
+      }
+    }
+  }
+  for (size_t k = 0; k < live.size(); ++k) {
+    const G2A& Q = live[k].second;
+    // pi(Q) = (conj(x) xi^((q-1)/3), conj(y) xi^((q-1)/2)); pi^2(Q) likewise without conj.
+    Fq2 x1 = e2_mul(e2_conj(Q.x), TW.gamma[1][2]), y1 = e2_mul(e2_conj(Q.y), TW.gamma[1][3]);
+    Fq2 x2 = e2_mul(Q.x, TW.gamma[2][2]), y2 = e2_neg(e2_mul(Q.y, TW.gamma[2][3]));
+    Line l = add_step(T[k], x1, y1, live[k].first);
+    f = e12_mul_line(f, l.l0, l.l1, l.l2);
+    l = add_step(T[k], x2, y2, live[k].first);
+    f = e12_mul_line(f, l.l0, l.l1, l.l2);
+  }
+  return f;
+}
+
+// The product of the pairs' Miller loops, one thread per pair after the first (the loop's
+// output is multiplicative over pairs, so splitting it changes nothing but the time).
+Fq12 miller_loop_parallel(const std::vector<std::pair<G1A, G2A>>& pairs) {
+  if (pairs.size() < 2) return miller_loop(pairs);
+  std::vector<Fq12> part(pairs.size());
+  std::vector<std::thread> th;
+  for (size_t k = 1; k < pairs.size(); ++k)
+    th.emplace_back([&, k] { part[k] = miller_loop({pairs[k]}); });
+  part[0] = miller_loop({pairs[0]});
+  for (auto& t : th) t.join();
+  Fq12 f = part[0];
+  for (size_t k = 1; k < pairs.size(); ++k) f = e12_mul(f, part[k]);
+  return f;
+}
+
+// f^x for f in the cyclotomic subgroup (x > 0).
+Fq12 cyc_pow_x(const Fq12& f) {
+  Fq12 acc = f;
+  for (int i = 62 - __builtin_clzll(BN_X); i >= 0; --i) {
+    acc = e12_cyc_sqr(acc);
+    if ((BN_X >> i) & 1) acc = e12_mul(acc, f);
+  }
+  return acc;
+}
+
+Fq12 final_exp_easy(const Fq12& f) {
+  Fq12 t = e12_mul(e12_conj(f), e12_inv(f));  // f^(q^6 - 1)
+  return e12_mul(e12_frob(t, 2), t);          // ^(q^2 + 1)
+}
+
+// The hard part by the addition chain in x. In the cyclotomic subgroup conjugation inverts, and
+// the chain's exponent is l0 + l1 q + l2 q^2 + l3 q^3 with l0 = -36x^3 - 30x^2 - 18x - 2,
+// l1 = -36x^3 - 18x^2 - 12x + 1, l2 = 6x^2 + 1, l3 = 1, which is (q^4 - q^2 + 1)/r.
+Fq12 final_exp_hard_chain(const Fq12& f) {
+  Fq12 fx = cyc_pow_x(f), fx2 = cyc_pow_x(fx), fx3 = cyc_pow_x(fx2);
+  Fq12 y0 = e12_mul(e12_mul(e12_frob(f, 1), e12_frob(f, 2)), e12_frob(f, 3));
+  Fq12 y1 = e12_conj(f);
+  Fq12 y2 = e12_frob(fx2, 2);
+  Fq12 y3 = e12_conj(e12_frob(fx, 1));
+  Fq12 y4 = e12_conj(e12_mul(fx, e12_frob(fx2, 1)));
+  Fq12 y5 = e12_conj(fx2);
+  Fq12 y6 = e12_conj(e12_mul(fx3, e12_frob(fx3, 1)));
+  Fq12 t0 = e12_mul(e12_mul(e12_sqr(y6), y4), y5);
+  Fq12 t1 = e12_mul(e12_mul(y3, y5), t0);
+  t0 = e12_mul(t0, y2);
+  t1 = e12_sqr(e12_mul(e12_sqr(t1), t0));
+  t0 = e12_mul(t1, y1);
+  t1 = e12_mul(t1, y0);
+  t0 = e12_sqr(t0);
+  return e12_mul(t1, t0);
+}
+
+Fq12 final_exp_hard_plain(const Fq12& f) { return e12_pow(f, tower().hard); }
+
+// Start-up check of the chain: on a fixed element of the cyclotomic subgroup it must agree
+// with the plain exponentiation by (q^4 - q^2 + 1)/r.
+struct ChainCheck {
+  bool ok = false;
+  ChainCheck() {
+    if (!tower().ok) return;
+    Fq12 a = e12_one();
+    a.c0.a1 = {f_from_u64<QTag>(5), f_from_u64<QTag>(7)};
+    a.c1.a2 = {f_from_u64<QTag>(11), f_from_u64<QTag>(13)};
+    a.c1.a0 = {f_from_u64<QTag>(17), f_from_u64<QTag>(19)};
+    Fq12 e = final_exp_easy(a);
+    ok = e12_eq(final_exp_hard_chain(e), final_exp_hard_plain(e));
+  }
+};
+
+Fq12 final_exp(const Fq12& f) { return final_exp_hard_chain(final_exp_easy(f)); }
+
+void require_tower() {
+  const Tower& T = tower();
+  if (!T.ok) throw std::runtime_error("BN254 constants failed their start-up check: " + T.err);
+  static const ChainCheck chain;
+  if (!chain.ok) throw std::runtime_error("the final exponentiation's addition chain failed its start-up check");
+}
+
+// ---------------------------------------------------------------------------------------
+// Groth16
+// ---------------------------------------------------------------------------------------
+struct VerifyFail {
+  g16hip_status st;
+  std::string msg;
+};
+
+[[noreturn]] void vfail(g16hip_status st, const std::string& msg) { throw VerifyFail{st, msg}; }
+
+G1A must_g1(const uint8_t* b, const char* what, g16hip_status st) {
+  G1A p;
+  if (!load_g1(b, &p)) vfail(st, std::string(what) + ": coordinate is not below the base-field modulus");
+  return p;
+}
+G2A must_g2(const uint8_t* b, const char* what, g16hip_status st) {
+  G2A p;
+  if (!load_g2(b, &p)) vfail(st, std::string(what) + ": coordinate is not below the base-field modulus");
+  return p;
+}
+
+}  // namespace
+
+namespace g16 {
+
+g16hip_status groth16_check(const VkBytes& vk, const uint8_t* pub, size_t n, const g16hip_proof& pf,
+                            VerifyMode mode, std::string* why) {
+  try {
+    require_tower();
+    // VerifyingKey::check_structure (crates/formats/src/lib.rs:1326-1350).
+    const g16hip_status K = G16HIP_UNSAFE_KEY;
+    G1A alpha = must_g1(vk.alpha_g1, "alpha_g1", K);
+    G2A beta = must_g2(vk.beta_g2, "beta_g2", K), gamma = must_g2(vk.gamma_g2, "gamma_g2", K),
+        delta = must_g2(vk.delta_g2, "delta_g2", K);
+    size_t n_ic = vk.ic.size() / 64;
+    if (mode != VerifyMode::Unchecked) {
+      if (alpha.inf) vfail(K, "alpha_g1 is the point at infinity");
+      if (beta.inf) vfail(K, "beta_g2 is the point at infinity");
+      if (gamma.inf) vfail(K, "gamma_g2 is the point at infinity");
+      if (delta.inf) vfail(K, "delta_g2 is the point at infinity");
+      if (mode == VerifyMode::Checked && std::memcmp(vk.gamma_g2, vk.delta_g2, 128) == 0)
+        vfail(K,
+              "gamma_g2 equals delta_g2, so the key had no phase-2 contribution and anyone can "
+              "forge a proof against it");
+      if (n_ic == 0) vfail(K, "IC is empty; it must carry at least the constant-wire point");
+    }
+    // check_proof (crates/groth16/src/verify.rs:80-101).
+    const g16hip_status P = G16HIP_INVALID_PROOF;
+    G1A a = must_g1(pf.a, "A", P), c = must_g1(pf.c, "C", P);
+    G2A b = must_g2(pf.b, "B", P);
+    if (mode != VerifyMode::Unchecked) {
+      if (a.inf) vfail(P, "A is the point at infinity");
+      if (b.inf) vfail(P, "B is the point at infinity");
+      if (c.inf) vfail(P, "C is the point at infinity");
+      if (!g1_on_curve(a)) vfail(P, "A is not a valid G1 point");
+      if (!g1_on_curve(c)) vfail(P, "C is not a valid G1 point");
+      if (!g2_on_curve(b)) vfail(P, "B is not a valid G2 point");
+    }
+    // B's subgroup check runs beside the pairing (checked_pairing, verify.rs:56-63); a proof
+    // that fails it is refused with that error whatever the pairing says.
+    bool b_in_subgroup = true;
+    std::thread sub;
+    if (mode != VerifyMode::Unchecked) sub = std::thread([&] { b_in_subgroup = g2_in_subgroup(b); });
+    struct Join {
+      std::thread& t;
+      ~Join() {
+        if (t.joinable()) t.join();
+      }
+    } join{sub};
+    // aggregate_public (crates/groth16/src/verify.rs:149-176).
+    if (n_ic == 0) vfail(G16HIP_EMPTY_KEY,
+                         "verifying key has an empty IC; it must carry at least the constant-wire point");
+    if (n != n_ic - 1)
+      vfail(G16HIP_PUBLIC_COUNT,
+            "expected " + std::to_string(n_ic - 1) + " public inputs, got " + std::to_string(n));
+    Jac<Fq> acc = jac_from<OpsQ>(must_g1(vk.ic.data(), "IC[0]", K));
+    for (size_t i = 0; i < n; ++i) {
+      Fr w;
+      if (!f_load(pub + 32 * i, &w))
+        vfail(G16HIP_BACKEND, "public input " + std::to_string(i) + " is not below the scalar modulus");
+      Fr wi = f_to_int(w);
+      G1A ic = must_g1(vk.ic.data() + 64 * (i + 1), "IC", K);
+      acc = jac_add<OpsQ>(acc, jac_mul<OpsQ>(jac_from<OpsQ>(ic), Big::of(wi.v, 4)));
+    }
+    G1A lbar = jac_to_aff<OpsQ, G1A>(acc);
+    G1A na = a;
+    na.y = f_neg(a.y);
+    Fq12 m = miller_loop_parallel({{na, b}, {alpha, beta}, {lbar, gamma}, {c, delta}});
+    bool one = e12_is_one(final_exp(m));
+    if (sub.joinable()) sub.join();
+    if (!b_in_subgroup) vfail(P, "B is not a valid G2 point");
+    if (!one) vfail(G16HIP_PAIRING_FAILED, "pairing check failed");
+    if (why) why->clear();
+    return G16HIP_OK;
+  } catch (const VerifyFail& e) {
+    if (why) *why = e.msg;
+    return e.st;
+  } catch (const std::exception& e) {
+    if (why) *why = e.what();
+    return G16HIP_BACKEND;
+  }
+}
+
+}  // namespace g16
+
+// ---------------------------------------------------------------------------------------
+// decimal and JSON
+// ---------------------------------------------------------------------------------------
+namespace {
+
+// The integer of a 4-limb value, in decimal.
+std::string dec(const uint64_t v[4]) {
+  Big b = Big::of(v, 4);
+  if (b.is_zero()) return "0";
+  std::string s;
+  while (!b.is_zero()) {
+    uint64_t d = big_divmod_small(b, 10000000000000000000ull);
+    std::string part = std::to_string(d);
+    if (!b.is_zero()) part = std::string(19 - part.size(), '0') + part;
+    s = part + s;
+  }
+  return s;
+}
+
+template <class T>
+std::string dec_f(const Fp<T>& a) {
+  return dec(f_to_int(a).v);
+}
+
+struct JsonError {
+  std::string msg;
+};
+[[noreturn]] void jfail(const std::string& m) { throw JsonError{m}; }
+
+// A JSON document as values: enough of RFC 8259 for the three snarkjs files.
+struct JVal {
+  enum Kind { Null, Bool, Num, Str, Arr, Obj } k = Null;
+  std::string s;  // string contents, or the number's text
+  bool b = false;
+  std::vector<JVal> a;
+  std::vector<std::pair<std::string, JVal>> o;
+  const JVal* get(const std::string& key) const {  // the last duplicate wins
+    const JVal* hit = nullptr;
+    for (const auto& kv : o)
+      if (kv.first == key) hit = &kv.second;
+    return hit;
+  }
+  std::string show() const {
+    switch (k) {
+      case Null: return "null";
+      case Bool: return b ? "true" : "false";
+      case Num: return s;
+      case Str: return "\"" + s + "\"";
+      case Arr: return "an array";
+      default: return "an object";
+    }
+  }
+};
+
+struct JParser {
+  const char* p;
+  const char* end;
+  int depth = 0;
+  void ws() {
+    while (p < end && (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r')) ++p;
+  }
+  [[noreturn]] void bad(const char* what) { jfail(std::string("not valid JSON: ") + what); }
+  void lit(const char* w) {
+    size_t n = std::strlen(w);
+    if ((size_t)(end - p) < n || std::memcmp(p, w, n) != 0) bad("unexpected token");
+    p += n;
+  }
+  static void utf8(std::string& out, uint32_t c) {
+    if (c < 0x80) {
+      out += (char)c;
+    } else if (c < 0x800) {
+      out += (char)(0xC0 | (c >> 6));
+      out += (char)(0x80 | (c & 0x3F));
+    } else if (c < 0x10000) {
+      out += (char)(0xE0 | (c >> 12));
+      out += (char)(0x80 | ((c >> 6) & 0x3F));
+      out += (char)(0x80 | (c & 0x3F));
+    } else {
+      out += (char)(0xF0 | (c >> 18));
+      out += (char)(0x80 | ((c >> 12) & 0x3F));
+      out += (char)(0x80 | ((c >> 6) & 0x3F));
+      out += (char)(0x80 | (c & 0x3F));
+    }
+  }
+  uint32_t hex4() {
+    if (end - p < 4) bad("short \\u escape");
+    uint32_t v = 0;
+    for (int i = 0; i < 4; ++i) {
+      char ch = *p++;
+      v <<= 4;
+      if (ch >= '0' && ch <= '9') v |= ch - '0';
+      else if (ch >= 'a' && ch <= 'f') v |= ch - 'a' + 10;
+      else if (ch >= 'A' && ch <= 'F') v |= ch - 'A' + 10;
+      else bad("bad \\u escape");
+    }
+    return v;
+  }
+  std::string str() {
+    if (p >= end || *p != '"') bad("expected a string");
+    ++p;
+    std::string out;
+    while (true) {
+      if (p >= end) bad("unterminated string");
+      unsigned char ch = (unsigned char)*p++;
+      if (ch == '"') break;
+      if (ch < 0x20) bad("control character in string");
+      if (ch != '\\') {
+        out += (char)ch;
+        continue;
+      }
+      if (p >= end) bad("unterminated escape");
+      char e = *p++;
+      switch (e) {
+        case '"': out += '"'; break;
+        case '\\': out += '\\'; break;
+        case '/': out += '/'; break;
+        case 'b': out += '\b'; break;
+        case 'f': out += '\f'; break;
+        case 'n': out += '\n'; break;
+        case 'r': out += '\r'; break;
+        case 't': out += '\t'; break;
+        case 'u': {
+          uint32_t c = hex4();
+          if (c >= 0xD800 && c < 0xDC00) {
+            if (end - p < 6 || p[0] != '\\' || p[1] != 'u') bad("lone surrogate");
+            p += 2;
+            uint32_t lo = hex4();
+            if (lo < 0xDC00 || lo >= 0xE000) bad("lone surrogate");
+            c = 0x10000 + ((c - 0xD800) << 10) + (lo - 0xDC00);
+          } else if (c >= 0xDC00 && c < 0xE000) {
+            bad("lone surrogate");
+          }
+          utf8(out, c);
+          break;
+        }
+        default: bad("bad escape");
+      }
+    }
+    return out;
+  }
+  JVal value() {
+    if (++depth > 64) bad("nested too deeply");
+    ws();
+    if (p >= end) bad("unexpected end");
+    JVal v;
+    char ch = *p;
+    if (ch == '{') {
+      v.k = JVal::Obj;
+      ++p;
+      ws();
+      if (p < end && *p == '}') {
+        ++p;
+      } else {
+        while (true) {
+          ws();
+          std::string key = str();
+          ws();
+          if (p >= end || *p != ':') bad("expected ':'");
+          ++p;
+          v.o.emplace_back(key, value());
+          ws();
+          if (p < end && *p == ',') {
+            ++p;
+            continue;
+          }
+          if (p < end && *p == '}') {
+            ++p;
+            break;
+          }
+          bad("expected ',' or '}'");
+        }
+      }
+    } else if (ch == '[') {
+      v.k = JVal::Arr;
+      ++p;
+      ws();
+      if (p < end && *p == ']') {
+        ++p;
+      } else {
+        while (true) {
+          v.a.push_back(value());
+          ws();
+          if (p < end && *p == ',') {
+            ++p;
+            continue;
+          }
+          if (p < end && *p == ']') {
+            ++p;
+            break;
+          }
+          bad("expected ',' or ']'");
+        }
+      }
+    } else if (ch == '"') {
+      v.k = JVal::Str;
+      v.s = str();
+    } else if (ch == 't') {
+      lit("true");
+      v.k = JVal::Bool;
+      v.b = true;
+    } else if (ch == 'f') {
+      lit("false");
+      v.k = JVal::Bool;
+    } else if (ch == 'n') {
+      lit("null");
+    } else if (ch == '-' || (ch >= '0' && ch <= '9')) {
+      const char* s0 = p;
+      if (*p == '-') ++p;
+      if (p >= end) bad("bad number");
+      if (*p == '0') {
+        ++p;
+      } else if (*p >= '1' && *p <= '9') {
+        while (p < end && *p >= '0' && *p <= '9') ++p;
+      } else {
+        bad("bad number");
+      }
+      if (p < end && *p == '.') {
+        ++p;
+        if (p >= end || *p < '0' || *p > '9') bad("bad number");
+        while (p < end && *p >= '0' && *p <= '9') ++p;
+      }
+      if (p < end && (*p == 'e' || *p == 'E')) {
+        ++p;
+        if (p < end && (*p == '+' || *p == '-')) ++p;
+        if (p >= end || *p < '0' || *p > '9') bad("bad number");
+        while (p < end && *p >= '0' && *p <= '9') ++p;
+      }
+      v.k = JVal::Num;
+      v.s.assign(s0, p);
+    } else {
+      bad("unexpected character");
+    }
+    --depth;
+    return v;
+  }
+};
+
+JVal parse_json(const char* text, size_t len, const char* what) {
+  JParser ps{text, text + len};
+  try {
+    JVal v = ps.value();
+    ps.ws();
+    if (ps.p != ps.end) ps.bad("trailing characters");
+    return v;
+  } catch (const JsonError& e) {
+    jfail(std::string(what) + " is not valid JSON: " + e.msg.substr(std::strlen("not valid JSON: ")));
+  }
+}
+
+// A decimal string below the modulus of T, to Montgomery form (parse_field,
+// crates/groth16/src/json.rs:53-68): one optional '+', digits only, never reduced.
+template <class T>
+Fp<T> parse_field(const JVal& v, const std::string& what) {
+  if (v.k != JVal::Str) jfail(what + ": expected a decimal string, got " + v.show());
+  const std::string& s = v.s;
+  size_t i = (!s.empty() && s[0] == '+') ? 1 : 0;
+  if (i == s.size()) jfail(what + ": \"" + s + "\" is not a decimal integer");
+  Big acc;
+  for (; i < s.size(); ++i) {
+    if (s[i] < '0' || s[i] > '9') jfail(what + ": \"" + s + "\" is not a decimal integer");
+    acc = big_add_small(big_mul(acc, Big::of((const uint64_t[]){10}, 1)), (uint64_t)(s[i] - '0'));
+    if (acc.bits() > 256) jfail(what + ": " + s + " is not less than the field modulus");
+  }
+  Fp<T> x{{acc.v[0], acc.v[1], acc.v[2], acc.v[3]}};
+  if (!lt_p<T>(x.v)) jfail(what + ": " + s + " is not less than the field modulus");
+  return f_from_int(x);
+}
+
+const std::vector<JVal>& elements(const JVal& v, size_t n, const std::string& what) {
+  if (v.k != JVal::Arr) jfail(what + ": expected an array of " + std::to_string(n));
+  if (v.a.size() != n)
+    jfail(what + ": expected " + std::to_string(n) + " entries, got " + std::to_string(v.a.size()));
+  return v.a;
+}
+
+G1A checked_g1(const Fq& x, const Fq& y, const std::string& what) {
+  G1A p{x, y, false};
+  if (!g1_on_curve(p)) jfail(what + ": point is not on the curve");
+  return p;
+}
+
+G2A checked_g2(const Fq2& x, const Fq2& y, const std::string& what) {
+  G2A p{x, y, false};
+  if (!g2_on_curve(p)) jfail(what + ": point is not on the curve");
+  if (!g2_in_subgroup(p)) jfail(what + ": point is not in the prime-order subgroup");
+  return p;
+}
+
+// read_g1 (crates/groth16/src/json.rs:150-168): [x, y, "1"], or exactly ["0", "1", "0"].
+G1A read_g1(const JVal& v, const std::string& what) {
+  const auto& a = elements(v, 3, what);
+  Fq x = parse_field<QTag>(a[0], what), y = parse_field<QTag>(a[1], what),
+     z = parse_field<QTag>(a[2], what);
+  if (f_is_zero(z)) {
+    if (f_is_zero(x) && f_is_one(y)) return G1A{f_zero<QTag>(), f_zero<QTag>(), true};
+    jfail(what + ": z = 0 but (x, y) is not ffjavascript's (0, 1)");
+  }
+  if (!f_is_one(z)) jfail(what + ": z = " + dec_f(z) + ", expected 1");
+  return checked_g1(x, y, what);
+}
+
+Fq2 read_fq2(const JVal& v, const std::string& what) {
+  const auto& a = elements(v, 2, what);
+  return {parse_field<QTag>(a[0], what), parse_field<QTag>(a[1], what)};
+}
+
+// read_g2 (crates/groth16/src/json.rs:180-197).
+G2A read_g2(const JVal& v, const std::string& what) {
+  const auto& a = elements(v, 3, what);
+  Fq2 x = read_fq2(a[0], what), y = read_fq2(a[1], what), z = read_fq2(a[2], what);
+  if (e2_is_zero(z)) {
+    if (e2_is_zero(x) && e2_eq(y, e2_one())) return G2A{e2_zero(), e2_zero(), true};
+    jfail(what + ": z = 0 but (x, y) is not ffjavascript's (0, 1)");
+  }
+  if (!e2_eq(z, e2_one())) jfail(what + ": z is not [1, 0]");
+  return checked_g2(x, y, what);
+}
+
+void store_g2(uint8_t* b, const G2A& p) {
+  if (p.inf) {
+    std::memset(b, 0, 128);
+    return;
+  }
+  f_store(b, p.x.c0);
+  f_store(b + 32, p.x.c1);
+  f_store(b + 64, p.y.c0);
+  f_store(b + 96, p.y.c1);
+}
+
+std::string g1_json(const G1A& p) {
+  if (p.inf) return "[\"0\",\"1\",\"0\"]";
+  return "[\"" + dec_f(p.x) + "\",\"" + dec_f(p.y) + "\",\"1\"]";
+}
+
+std::string fq2_json(const Fq2& x) { return "[\"" + dec_f(x.c0) + "\",\"" + dec_f(x.c1) + "\"]"; }
+
+std::string g2_json(const G2A& p) {
+  if (p.inf) return "[[\"0\",\"0\"],[\"1\",\"0\"],[\"0\",\"0\"]]";
+  return "[" + fq2_json(p.x) + "," + fq2_json(p.y) + ",[\"1\",\"0\"]]";
+}
+
+// Writes `text` into the caller's buffer: a null buffer asks for the size.
+g16hip_status emit(const std::string& text, char* buf, size_t cap, size_t* len) {
+  if (len) *len = text.size();
+  if (!buf) return G16HIP_OK;
+  if (cap < text.size() + 1) {
+    g16::set_last_error("output buffer holds " + std::to_string(cap) + " bytes, the text needs " +
+                        std::to_string(text.size() + 1));
+    return G16HIP_BACKEND;
+  }
+  std::memcpy(buf, text.data(), text.size());
+  buf[text.size()] = 0;
+  return G16HIP_OK;
+}
+
+template <class F>
+g16hip_status run(F&& f) {
+  try {
+    g16::set_last_error("");
+    require_tower();
+    return f();
+  } catch (const JsonError& e) {
+    g16::set_last_error(e.msg);
+    return G16HIP_BAD_JSON;
+  } catch (const VerifyFail& e) {
+    g16::set_last_error(e.msg);
+    return e.st;
+  } catch (const std::bad_alloc&) {
+    g16::set_last_error("host allocation failed");
+    return G16HIP_BACKEND;
+  } catch (const std::exception& e) {
+    g16::set_last_error(e.what());
+    return G16HIP_BACKEND;
+  }
+}
+
+g16::VkBytes vk_bytes(const g16hip_vk* vk) {
+  if (!vk || !vk->alpha_g1 || !vk->beta_g2 || !vk->gamma_g2 || !vk->delta_g2 ||
+      (!vk->ic && vk->n_ic))
+    vfail(G16HIP_BACKEND, "null verifying key field");
+  g16::VkBytes v;
+  std::memcpy(v.alpha_g1, vk->alpha_g1, 64);
+  std::memcpy(v.beta_g2, vk->beta_g2, 128);
+  std::memcpy(v.gamma_g2, vk->gamma_g2, 128);
+  std::memcpy(v.delta_g2, vk->delta_g2, 128);
+  v.ic.assign(vk->ic, vk->ic + 64 * (size_t)vk->n_ic);
+  return v;
+}
+
+g16hip_status verify_with(const g16hip_vk* vk, const uint8_t* pub, size_t n, const g16hip_proof* proof,
+                          g16::VerifyMode mode) {
+  return run([&] {
+    if (!proof || (!pub && n)) vfail(G16HIP_BACKEND, "null argument");
+    std::string why;
+    g16hip_status st = g16::groth16_check(vk_bytes(vk), pub, n, *proof, mode, &why);
+    if (st != G16HIP_OK) g16::set_last_error(why);
+    return st;
+  });
+}
+
+}  // namespace
+
+extern "C" {
+
+g16hip_status g16hip_verify(const g16hip_vk* vk, const uint8_t* public_inputs, size_t n_public,
+                            const g16hip_proof* proof) {
+  return verify_with(vk, public_inputs, n_public, proof, g16::VerifyMode::Checked);
+}
+
+g16hip_status g16hip_verify_allowing_unblinded(const g16hip_vk* vk, const uint8_t* public_inputs,
+                                               size_t n_public, const g16hip_proof* proof) {
+  return verify_with(vk, public_inputs, n_public, proof, g16::VerifyMode::AllowingUnblinded);
+}
+
+g16hip_status g16hip_verify_unchecked(const g16hip_vk* vk, const uint8_t* public_inputs,
+                                      size_t n_public, const g16hip_proof* proof) {
+  return verify_with(vk, public_inputs, n_public, proof, g16::VerifyMode::Unchecked);
+}
+
+g16hip_status g16hip_pairing_check(const uint8_t* g1, const uint8_t* g2, size_t n, int* is_one) {
+  return run([&] {
+    if ((!g1 || !g2) && n) vfail(G16HIP_BACKEND, "null argument");
+    if (!is_one) vfail(G16HIP_BACKEND, "null argument");
+    std::vector<std::pair<G1A, G2A>> pairs;
+    for (size_t i = 0; i < n; ++i) {
+      G1A p = must_g1(g1 + 64 * i, "G1 point", G16HIP_BACKEND);
+      G2A q = must_g2(g2 + 128 * i, "G2 point", G16HIP_BACKEND);
+      if (!g1_on_curve(p)) vfail(G16HIP_BACKEND, "G1 point " + std::to_string(i) + " is not on the curve");
+      if (!g2_on_curve(q) || !g2_in_subgroup(q))
+        vfail(G16HIP_BACKEND, "G2 point " + std::to_string(i) + " is not in the prime-order subgroup");
+      pairs.push_back({p, q});
+    }
+    *is_one = e12_is_one(final_exp(miller_loop(pairs))) ? 1 : 0;
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_pairing(const uint8_t* g1, const uint8_t* g2, uint8_t* out) {
+  return run([&] {
+    if (!g1 || !g2 || !out) vfail(G16HIP_BACKEND, "null argument");
+    G1A p = must_g1(g1, "G1 point", G16HIP_BACKEND);
+    G2A q = must_g2(g2, "G2 point", G16HIP_BACKEND);
+    if (!g1_on_curve(p)) vfail(G16HIP_BACKEND, "G1 point is not on the curve");
+    if (!g2_on_curve(q) || !g2_in_subgroup(q))
+      vfail(G16HIP_BACKEND, "G2 point is not in the prime-order subgroup");
+    Fq12 f = final_exp(miller_loop({{p, q}}));
+    const Fq2* parts[6] = {&f.c0.a0, &f.c0.a1, &f.c0.a2, &f.c1.a0, &f.c1.a1, &f.c1.a2};
+    for (int i = 0; i < 6; ++i) {
+      f_store(out + 64 * i, parts[i]->c0);
+      f_store(out + 64 * i + 32, parts[i]->c1);
+    }
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_point_check(int group, const uint8_t* point, int* valid) {
+  return run([&] {
+    if (!point || !valid || (group != 1 && group != 2)) vfail(G16HIP_BACKEND, "bad argument");
+    if (group == 1) {
+      G1A p;
+      *valid = load_g1(point, &p) && g1_on_curve(p);
+    } else {
+      G2A q;
+      *valid = load_g2(point, &q) && g2_on_curve(q) && g2_in_subgroup(q);
+    }
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_proof_to_json(const g16hip_proof* proof, char* buf, size_t cap, size_t* len) {
+  return run([&] {
+    if (!proof) vfail(G16HIP_BACKEND, "null proof");
+    G1A a = must_g1(proof->a, "A", G16HIP_BACKEND), c = must_g1(proof->c, "C", G16HIP_BACKEND);
+    G2A b = must_g2(proof->b, "B", G16HIP_BACKEND);
+    std::string text = "{\n \"pi_a\": " + g1_json(a) + ",\n \"pi_b\": " + g2_json(b) +
+                       ",\n \"pi_c\": " + g1_json(c) +
+                       ",\n \"protocol\": \"groth16\",\n \"curve\": \"bn128\"\n}\n";
+    return emit(text, buf, cap, len);
+  });
+}
+
+g16hip_status g16hip_public_to_json(const uint8_t* public_inputs, size_t n, char* buf, size_t cap,
+                                    size_t* len) {
+  return run([&] {
+    if (!public_inputs && n) vfail(G16HIP_BACKEND, "null public inputs");
+    std::string items;
+    for (size_t i = 0; i < n; ++i) {
+      Fr w;
+      if (!f_load(public_inputs + 32 * i, &w))
+        vfail(G16HIP_BACKEND, "public input " + std::to_string(i) + " is not below the scalar modulus");
+      if (i) items += ",\n ";
+      items += "\"" + dec_f(w) + "\"";
+    }
+    return emit("[\n " + items + "\n]\n", buf, cap, len);
+  });
+}
+
+g16hip_status g16hip_proof_from_json(const char* text, size_t len, g16hip_proof* out) {
+  return run([&] {
+    if (!text || !out) vfail(G16HIP_BACKEND, "null argument");
+    JVal v = parse_json(text, len, "proof");
+    if (v.k != JVal::Obj) jfail("proof has no pi_a");
+    if (const JVal* p = v.get("protocol"))
+      if (p->k == JVal::Str && p->s != "groth16")
+        jfail("proof protocol is \"" + p->s + "\", expected groth16");
+    if (const JVal* c = v.get("curve"))
+      if (c->k == JVal::Str && c->s != "bn128") jfail("proof curve is \"" + c->s + "\", expected bn128");
+    auto field = [&](const char* k) -> const JVal& {
+      const JVal* f = v.get(k);
+      if (!f) jfail(std::string("proof has no ") + k);
+      return *f;
+    };
+    G1A a = read_g1(field("pi_a"), "pi_a");
+    G2A b = read_g2(field("pi_b"), "pi_b");
+    G1A c = read_g1(field("pi_c"), "pi_c");
+    store_g1(out->a, a);
+    store_g2(out->b, b);
+    store_g1(out->c, c);
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_public_from_json(const char* text, size_t len, uint8_t* out, size_t cap,
+                                      size_t* n) {
+  return run([&] {
+    if (!text || !n) vfail(G16HIP_BACKEND, "null argument");
+    JVal v = parse_json(text, len, "public signals");
+    if (v.k != JVal::Arr) jfail("expected a flat JSON array of public signals");
+    std::vector<Fr> vals;
+    for (size_t i = 0; i < v.a.size(); ++i)
+      vals.push_back(parse_field<RTag>(v.a[i], "public signal " + std::to_string(i)));
+    *n = vals.size();
+    if (!out) return G16HIP_OK;
+    if (cap < vals.size()) vfail(G16HIP_BACKEND, "output holds " + std::to_string(cap) +
+                                                     " scalars, the file has " + std::to_string(vals.size()));
+    for (size_t i = 0; i < vals.size(); ++i) f_store(out + 32 * i, vals[i]);
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_vk_from_json(const char* text, size_t len, int check_structure, uint8_t* out,
+                                  size_t cap, uint32_t* n_public) {
+  return run([&] {
+    if (!text || !n_public) vfail(G16HIP_BACKEND, "null argument");
+    JVal v = parse_json(text, len, "verifying key");
+    auto str_field = [&](const char* k) -> std::string {
+      const JVal* f = v.k == JVal::Obj ? v.get(k) : nullptr;
+      return f && f->k == JVal::Str ? f->s : std::string();
+    };
+    const JVal* pr = v.k == JVal::Obj ? v.get("protocol") : nullptr;
+    if (!(pr && pr->k == JVal::Str && pr->s == "groth16"))
+      jfail("protocol is " + (pr && pr->k == JVal::Str ? "Some(\"" + pr->s + "\")" : std::string("None")) +
+            ", expected groth16");
+    const JVal* cu = v.get("curve");
+    if (!(cu && cu->k == JVal::Str && cu->s == "bn128"))
+      jfail("curve is " + (cu && cu->k == JVal::Str ? "Some(\"" + cu->s + "\")" : std::string("None")) +
+            ", expected bn128");
+    (void)str_field;
+    const JVal* ic = v.get("IC");
+    if (!ic || ic->k != JVal::Arr) jfail("IC is missing or not an array");
+    const JVal* np = v.get("nPublic");
+    uint64_t npub = 0;
+    bool ok = np && np->k == JVal::Num && !np->s.empty() && np->s[0] != '-' &&
+              np->s.find_first_of(".eE") == std::string::npos && np->s.size() <= 20;
+    if (ok) {
+      u128 acc = 0;
+      for (char ch : np->s) acc = acc * 10 + (uint64_t)(ch - '0');
+      ok = acc <= (u128)UINT64_MAX;
+      npub = (uint64_t)acc;
+    }
+    if (!ok) jfail("nPublic is missing");
+    if (npub == UINT64_MAX) jfail("nPublic " + np->s + " is impossibly large");
+    if (ic->a.size() != npub + 1)
+      jfail("IC has " + std::to_string(ic->a.size()) + " entries, nPublic " + np->s + " implies " +
+            std::to_string(npub + 1));
+    if (npub > UINT32_MAX - 1) jfail("nPublic " + np->s + " is impossibly large");
+    auto field = [&](const char* k) -> const JVal& {
+      const JVal* f = v.get(k);
+      if (!f) jfail(std::string(k) + " is missing");
+      return *f;
+    };
+    // json_g1 / json_g2 read the same [x, y, z] forms as the proof reader.
+    G1A alpha = read_g1(field("vk_alpha_1"), "vk_alpha_1");
+    G2A beta = read_g2(field("vk_beta_2"), "vk_beta_2");
+    G2A gamma = read_g2(field("vk_gamma_2"), "vk_gamma_2");
+    G2A delta = read_g2(field("vk_delta_2"), "vk_delta_2");
+    std::vector<G1A> pts;
+    for (size_t i = 0; i < ic->a.size(); ++i) pts.push_back(read_g1(ic->a[i], "IC[" + std::to_string(i) + "]"));
+    *n_public = (uint32_t)npub;
+    size_t need = 64 + 3 * 128 + 64 * pts.size();
+    if (check_structure) {
+      g16::VkBytes vb;
+      store_g1(vb.alpha_g1, alpha);
+      store_g2(vb.beta_g2, beta);
+      store_g2(vb.gamma_g2, gamma);
+      store_g2(vb.delta_g2, delta);
+      vb.ic.resize(64 * pts.size());
+      for (size_t i = 0; i < pts.size(); ++i) store_g1(vb.ic.data() + 64 * i, pts[i]);
+      const g16hip_status K = G16HIP_UNSAFE_KEY;
+      if (alpha.inf) vfail(K, "alpha_g1 is the point at infinity");
+      if (beta.inf) vfail(K, "beta_g2 is the point at infinity");
+      if (gamma.inf) vfail(K, "gamma_g2 is the point at infinity");
+      if (delta.inf) vfail(K, "delta_g2 is the point at infinity");
+      if (std::memcmp(vb.gamma_g2, vb.delta_g2, 128) == 0)
+        vfail(K,
+              "gamma_g2 equals delta_g2, so the key had no phase-2 contribution and anyone can "
+              "forge a proof against it");
+    }
+    if (!out) return G16HIP_OK;
+    if (cap < need) vfail(G16HIP_BACKEND, "output holds " + std::to_string(cap) + " bytes, the key needs " +
+                                              std::to_string(need));
+    store_g1(out, alpha);
+    store_g2(out + 64, beta);
+    store_g2(out + 192, gamma);
+    store_g2(out + 320, delta);
+    for (size_t i = 0; i < pts.size(); ++i) store_g1(out + 448 + 64 * i, pts[i]);
+    return G16HIP_OK;
+  });
+}
+
+}  // extern "C"

@@ -20,7 +20,8 @@ _u8p = ctypes.POINTER(ctypes.c_uint8)
 _vp = ctypes.c_void_p
 
 # g16hip_status
-OK, WITNESS_LENGTH, BACKEND, DEVICE, DEVICE_MEMORY, CONSTANT_WIRE, ZERO_BLINDER = range(7)
+(OK, WITNESS_LENGTH, BACKEND, DEVICE, DEVICE_MEMORY, CONSTANT_WIRE, ZERO_BLINDER, SELF_VERIFY,
+ PUBLIC_COUNT, EMPTY_KEY, PAIRING_FAILED, INVALID_PROOF, UNSAFE_KEY, BAD_JSON) = range(14)
 
 
 class NativeUnavailable(RuntimeError):
@@ -62,8 +63,44 @@ class ZeroBlinder(ProveError):
     pass
 
 
+class SelfVerify(ProveError):
+    """The proof failed the check of its result (``ProveError::SelfVerify``)."""
+
+
+class VerifyError(ProveError):
+    """``VerifyError`` (crates/groth16/src/verify.rs:16-31)."""
+
+
+class PublicInputCount(VerifyError):
+    pass
+
+
+class EmptyVerifyingKey(VerifyError):
+    pass
+
+
+class PairingFailed(VerifyError):
+    pass
+
+
+class InvalidProof(VerifyError):
+    pass
+
+
+class UnsafeKey(VerifyError):
+    pass
+
+
+class JsonError(ProveError):
+    """A malformed proof.json / public.json / verification_key.json
+    (crates/groth16/src/json.rs:39-41, ``ZkeyError::BadJson``)."""
+
+
 _ERRORS = {WITNESS_LENGTH: WitnessLength, BACKEND: BackendError, DEVICE: DeviceFault,
-           DEVICE_MEMORY: DeviceMemory, CONSTANT_WIRE: ConstantWire, ZERO_BLINDER: ZeroBlinder}
+           DEVICE_MEMORY: DeviceMemory, CONSTANT_WIRE: ConstantWire, ZERO_BLINDER: ZeroBlinder,
+           SELF_VERIFY: SelfVerify, PUBLIC_COUNT: PublicInputCount, EMPTY_KEY: EmptyVerifyingKey,
+           PAIRING_FAILED: PairingFailed, INVALID_PROOF: InvalidProof, UNSAFE_KEY: UnsafeKey,
+           BAD_JSON: JsonError}
 
 
 class Timings(ctypes.Structure):
@@ -120,6 +157,12 @@ class ProofOut(ctypes.Structure):
 
 _lib = None
 _lib_lock = threading.Lock()
+
+
+class VkStruct(ctypes.Structure):
+    """``g16hip_vk`` = VerifyingKey (crates/formats/src/lib.rs:353-360)."""
+    _fields_ = [("alpha_g1", _u8p), ("beta_g2", _u8p), ("gamma_g2", _u8p), ("delta_g2", _u8p),
+                ("ic", _u8p), ("n_ic", ctypes.c_uint32)]
 
 
 def load() -> ctypes.CDLL:
@@ -191,6 +234,29 @@ EXPORTS = {
                                           ctypes.c_size_t]),
     "g16hip_valu_probe": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "g16hip_fixed_base": (ctypes.c_int, [_vp, ctypes.c_int, _u8p, ctypes.c_size_t, _u8p]),
+    "g16hip_prove": (ctypes.c_int, [_vp, _u8p, ctypes.c_size_t, _u8p, _u8p,
+                                    ctypes.POINTER(ProofOut), ctypes.POINTER(Timings)]),
+    "g16hip_circuit_vk": (ctypes.c_int, [_vp, _u8p]),
+    "g16hip_verify": (ctypes.c_int, [ctypes.POINTER(VkStruct), _u8p, ctypes.c_size_t,
+                                     ctypes.POINTER(ProofOut)]),
+    "g16hip_verify_allowing_unblinded": (ctypes.c_int, [ctypes.POINTER(VkStruct), _u8p,
+                                                        ctypes.c_size_t, ctypes.POINTER(ProofOut)]),
+    "g16hip_verify_unchecked": (ctypes.c_int, [ctypes.POINTER(VkStruct), _u8p, ctypes.c_size_t,
+                                               ctypes.POINTER(ProofOut)]),
+    "g16hip_pairing_check": (ctypes.c_int, [_u8p, _u8p, ctypes.c_size_t,
+                                            ctypes.POINTER(ctypes.c_int)]),
+    "g16hip_pairing": (ctypes.c_int, [_u8p, _u8p, _u8p]),
+    "g16hip_point_check": (ctypes.c_int, [ctypes.c_int, _u8p, ctypes.POINTER(ctypes.c_int)]),
+    "g16hip_proof_to_json": (ctypes.c_int, [ctypes.POINTER(ProofOut), ctypes.c_char_p,
+                                            ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "g16hip_public_to_json": (ctypes.c_int, [_u8p, ctypes.c_size_t, ctypes.c_char_p,
+                                             ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "g16hip_proof_from_json": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t,
+                                              ctypes.POINTER(ProofOut)]),
+    "g16hip_public_from_json": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, _u8p,
+                                               ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "g16hip_vk_from_json": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int, _u8p,
+                                           ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint32)]),
 }
 
 
@@ -406,6 +472,26 @@ class Circuit:
                                                    _p8(_c8(s)), ctypes.byref(out),
                                                    ctypes.byref(t or Timings())))
         return out.as_array()
+
+    def prove(self, witness: np.ndarray, r: np.ndarray, s: np.ndarray,
+              t: Timings | None = None) -> np.ndarray:
+        """``prove`` (crates/groth16/src/prove.rs:12-41): the proof, checked against this
+        circuit's verifying key before it is returned (``SelfVerify`` if it fails). r and s
+        are the caller's random blinders (Fr Montgomery)."""
+        w = _c8(witness).reshape(-1, 32)
+        out = ProofOut()
+        _check(self.lib.g16hip_prove(self.h, _p8(w), len(w), _p8(_c8(r)), _p8(_c8(s)),
+                                     ctypes.byref(out), ctypes.byref(t or Timings())))
+        return out.as_array()
+
+    def vk_bytes(self) -> np.ndarray:
+        """``key().vk`` as alpha_g1 | beta_g2 | gamma_g2 | delta_g2 | ic (wire encoding)."""
+        nv, npub, dom = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        _check(self.lib.g16hip_circuit_dims(self.h, ctypes.byref(nv), ctypes.byref(npub),
+                                            ctypes.byref(dom)))
+        out = np.zeros(64 + 3 * 128 + 64 * (npub.value + 1), np.uint8)
+        _check(self.lib.g16hip_circuit_vk(self.h, _p8(out)))
+        return out
 
     def prove_with_blinders_device(self, d_witness: int, n: int, r: np.ndarray, s: np.ndarray,
                                    t: Timings | None = None) -> np.ndarray:
