@@ -86,7 +86,12 @@ typedef struct {
   const uint8_t* b_g1_query; /* section 6, G1[n_vars] */
   const uint8_t* b_g2_query; /* section 7, G2[n_vars] */
   const uint8_t* l_query;    /* section 8, G1[n_vars - n_public - 1] */
-  const uint8_t* h_query;    /* section 9, G1[domain_size] */
+  const uint8_t* h_query;    /* section 9, G1[domain_size]; gnark: G1[domain_size - 1] */
+  /* HBasis (crates/formats/src/lib.rs:66-89): 0 = snarkjs' odd coset (every zkey), 1 = gnark's
+   * pk.G1.Z, [tau^bitrev(j) t(tau) / delta]_1 for j < domain_size - 1. A key may leave the
+   * matrices (row_ptr, signal, value_id, values) NULL: it then proves from A.w and B.w rows
+   * only (g16hip_h_and_msms_from_rows), as a gnark key, which carries no matrices, must. */
+  uint32_t h_basis;
 } g16hip_key;
 
 /* StageTimings, crates/groth16/src/lib.rs:322-368 (microseconds; calls add to them). */
@@ -167,6 +172,22 @@ g16hip_status g16hip_h_and_msms(g16hip_circuit* c, const uint8_t* witness, size_
 /* The same with the witness already in device memory (Fr[n_witness] Montgomery). */
 g16hip_status g16hip_h_and_msms_device(g16hip_circuit* c, const void* d_witness,
                                        size_t n_witness, g16hip_msm_out* out, g16hip_timings* t);
+/* PreparedCircuit::h_and_msms_from_rows (crates/groth16/src/lib.rs:425-444): stages 1-9 with
+ * stage 0 done by the caller. a_rows[i], b_rows[i] are constraint i's A.w and B.w (Fr
+ * Montgomery, host memory), n_rows <= domain_size of them, every later row zero; C = A*B on
+ * the rows. A gnark-basis key takes H's coefficients in bit-reversed order (the conversion
+ * runs on the device); a nonzero coefficient n - 1 is refused as the reference refuses it
+ * (crates/groth16/src/cpu.rs:265-286). */
+g16hip_status g16hip_h_and_msms_from_rows(g16hip_circuit* c, const uint8_t* witness,
+                                          size_t n_witness, const uint8_t* a_rows,
+                                          const uint8_t* b_rows, size_t n_rows,
+                                          g16hip_msm_out* out, g16hip_timings* t);
+/* prove_from_rows (crates/groth16/src/prove.rs:93-115): the proof from rows with caller r, s,
+ * checked against the circuit's verifying key like g16hip_prove (G16HIP_SELF_VERIFY). */
+g16hip_status g16hip_prove_from_rows(g16hip_circuit* c, const uint8_t* witness, size_t n_witness,
+                                     const uint8_t* a_rows, const uint8_t* b_rows, size_t n_rows,
+                                     const uint8_t* r, const uint8_t* s, g16hip_proof* proof,
+                                     g16hip_timings* t);
 
 /* prove_with_blinders (crates/groth16/src/prove.rs:175-229): stages 0-11 with caller r, s
  * (Fr Montgomery), checking the witness shape first (prove.rs:272-283). Assembly (stage 11)
@@ -325,6 +346,33 @@ typedef struct {
 } g16hip_kernel_stats;
 g16hip_status g16hip_kernel_stats_get(g16hip_device* dev, g16hip_kernel_stats* out);
 g16hip_status g16hip_kernel_stats_reset(g16hip_device* dev);
+
+/* The ceremony's MsmBackend (crates/cuda/src/ceremony.rs:65-312, the CudaMsmBackend behind
+ * `groth16 setup` and `zkey verify`'s recompute): MSMs over host slices (bases affine,
+ * scalars Fr Montgomery), each call cut into runs of at most max_run points (0 = 2^21) whose
+ * sums are added on the host. A run refused for device memory is halved, down to 2^10
+ * points; with vram_limit != 0 a run whose scratch would exceed it is refused before anything
+ * is allocated (the reference's with_vram_limit). A run whose sum is not on the curve is
+ * re-run, 8 attempts in all, then G16HIP_DEVICE. Calls on one handle run one at a time; every
+ * call goes to the device (the reference sends calls under 2^12 points to its CPU MSM; this
+ * library has no CPU path). */
+typedef struct g16hip_ceremony g16hip_ceremony;
+typedef struct {
+  uint64_t calls, runs, points, retries;
+  double ms;
+} g16hip_ceremony_stats;
+g16hip_status g16hip_ceremony_open(g16hip_device* dev, size_t max_run, size_t vram_limit,
+                                   g16hip_ceremony** out);
+void g16hip_ceremony_close(g16hip_ceremony* c);
+/* group 1 or 2; out: the affine sum (64 or 128 bytes). Empty input is infinity. */
+g16hip_status g16hip_ceremony_msm(g16hip_ceremony* c, int group, const uint8_t* bases,
+                                  const uint8_t* scalars, size_t n, uint8_t* out);
+/* Device bytes a standalone MSM of n points takes (uploaded inputs, the bases' MSM form,
+ * sort and accumulation scratch): what vram_limit is compared with. */
+g16hip_status g16hip_msm_scratch_bytes(int group, size_t n, size_t* bytes);
+/* Runs the last call took (CudaMsmBackend::last_runs), and the running tally. */
+g16hip_status g16hip_ceremony_last_runs(g16hip_ceremony* c, uint64_t* runs);
+g16hip_status g16hip_ceremony_stats_get(g16hip_ceremony* c, g16hip_ceremony_stats* out);
 
 /* Standalone NTT over Fr, in place on host memory (Fr[2^log_n] Montgomery, natural order):
  * kind 0 = forward, 1 = inverse (with 1/n), 2 = the prover's coset pipeline

@@ -231,6 +231,7 @@ typedef struct {
   const uint32_t* value_id[2];
   const uint8_t* values;
   const uint8_t *a_query, *b_g1_query, *b_g2_query, *l_query, *h_query;
+  uint32_t h_basis;  // 0 snarkjs' odd coset (domain_size H bases), 1 gnark's (domain_size - 1)
 } orc_key;
 }
 
@@ -259,6 +260,16 @@ static void gather(const orc_key* k, int m, const Fr* w, Fr* out) {
   });
 }
 
+// Stages 1-4 on stage 0's vectors (crates/groth16/src/cpu.rs:191-245): H = A*B - C on the
+// odd coset.
+static void coset_h(std::vector<Fr>& a, std::vector<Fr>& b, std::vector<Fr>& c, int log_n, Fr* h) {
+  size_t n = a.size();
+  coset_pipeline(a.data(), log_n);
+  coset_pipeline(b.data(), log_n);
+  coset_pipeline(c.data(), log_n);
+  for (size_t i = 0; i < n; ++i) h[i] = a[i] * b[i] - c[i];
+}
+
 // Stages 0-4 (crates/groth16/src/cpu.rs:316-341, :191-245).
 static void compute_h(const orc_key* k, const Fr* w, Fr* h) {
   size_t n = k->domain_size;
@@ -267,10 +278,40 @@ static void compute_h(const orc_key* k, const Fr* w, Fr* h) {
   gather(k, 0, w, a.data());
   gather(k, 1, w, b.data());
   for (size_t i = 0; i < n; ++i) c[i] = a[i] * b[i];
-  coset_pipeline(a.data(), log_n);
-  coset_pipeline(b.data(), log_n);
-  coset_pipeline(c.data(), log_n);
-  for (size_t i = 0; i < n; ++i) h[i] = a[i] * b[i] - c[i];
+  coset_h(a, b, c, log_n, h);
+}
+
+// Stage 0 from the caller's rows (crates/groth16/src/cpu.rs:387-424): A.w and B.w padded
+// with zero rows to the domain, C = A*B on the rows.
+static void rows_h(size_t n, const Fr* a_rows, const Fr* b_rows, size_t rows, Fr* h) {
+  int log_n = log2_exact((uint32_t)n);
+  std::vector<Fr> a(n, Fr::zero()), b(n, Fr::zero()), c(n, Fr::zero());
+  for (size_t i = 0; i < rows; ++i) {
+    a[i] = a_rows[i];
+    b[i] = b_rows[i];
+    c[i] = a[i] * b[i];
+  }
+  coset_h(a, b, c, log_n, h);
+}
+
+// H on the odd coset to gnark's basis (crates/groth16/src/cpu.rs:250-286): the coefficients
+// of V(shift x) by the inverse transform, V's coefficient j = g_j shift^-j, placed at
+// bitrev(j). These are H's times -2 (t = x^n - 1 = -2 on the odd coset); the caller scales the
+// MSM by -1/2, as the reference does. Returns false if coefficient n - 1 is not zero.
+static bool to_gnark_coefficients(Fr* h, int log_n) {
+  size_t n = (size_t)1 << log_n;
+  ntt_inverse(h, log_n);
+  Fr shift_inv = domain_root(log_n + 1).inv();
+  std::vector<Fr> out(n);
+  Fr acc = Fr::one();
+  for (size_t j = 0; j < n; ++j) {
+    size_t p = 0;
+    for (int bit = 0; bit < log_n; ++bit) p |= ((j >> bit) & 1) << (log_n - 1 - bit);
+    out[p] = h[j] * acc;
+    acc *= shift_inv;
+  }
+  std::copy(out.begin(), out.end(), h);
+  return out[n - 1].is_zero();
 }
 
 // ---------------------------------------------------------------------------------------
@@ -836,19 +877,54 @@ int orc_compute_h(const orc_key* k, const uint8_t* w, uint8_t* h) {
 // Stages 0-11 with fixed blinders (crates/groth16/src/prove.rs:175-229, :391-430).
 // msm_out: A G1 | B G2 | B G1 | L G1 | H G1 (affine, 64+128+64+64+64 bytes);
 // proof_out: A G1 | B G2 | C G1 (256 bytes).
+static int prove_tail(const orc_key* k, const Fr* w, std::vector<Fr>& h, const uint8_t* r_m,
+                      const uint8_t* s_m, uint8_t* msm_out, uint8_t* proof_out);
+
 int orc_prove(const orc_key* k, const uint8_t* wit, const uint8_t* r_m, const uint8_t* s_m,
               uint8_t* msm_out, uint8_t* proof_out) {
-  if (log2_exact(k->domain_size) < 0) return -1;
+  if (log2_exact(k->domain_size) < 0 || k->h_basis != 0) return -1;
   const Fr* w = reinterpret_cast<const Fr*>(wit);
-  size_t n = k->domain_size;
-  std::vector<Fr> h(n);
+  std::vector<Fr> h(k->domain_size);
   compute_h(k, w, h.data());
+  return prove_tail(k, w, h, r_m, s_m, msm_out, proof_out);
+}
+
+// prove_from_rows with fixed blinders (crates/groth16/src/cpu.rs:387-470,
+// crates/groth16/src/prove.rs:93-115 without the check): stage 0 from the rows, H in the key's
+// basis. -2: more rows than the domain; -3: gnark's coefficient n - 1 is not zero.
+int orc_prove_rows(const orc_key* k, const uint8_t* wit, const uint8_t* a_rows,
+                   const uint8_t* b_rows, size_t rows, const uint8_t* r_m, const uint8_t* s_m,
+                   uint8_t* msm_out, uint8_t* proof_out) {
+  int log_n = log2_exact(k->domain_size);
+  if (log_n < 0 || k->h_basis > 1) return -1;
+  if (rows > k->domain_size) return -2;
+  const Fr* w = reinterpret_cast<const Fr*>(wit);
+  std::vector<Fr> h(k->domain_size);
+  rows_h(k->domain_size, reinterpret_cast<const Fr*>(a_rows), reinterpret_cast<const Fr*>(b_rows),
+         rows, h.data());
+  if (k->h_basis == 1 && !to_gnark_coefficients(h.data(), log_n)) return -3;
+  return prove_tail(k, w, h, r_m, s_m, msm_out, proof_out);
+}
+
+// Stages 5-11 on a computed H (prove.rs:391-430).
+static int prove_tail(const orc_key* k, const Fr* w, std::vector<Fr>& h, const uint8_t* r_m,
+                      const uint8_t* s_m, uint8_t* msm_out, uint8_t* proof_out) {
+  size_t n = k->domain_size;
   uint32_t nl = k->n_vars - k->n_public - 1;
   G1J a = msm_pippenger<G1Curve>(k->a_query, w, k->n_vars, 0);
   G2J b2 = msm_pippenger<G2Curve>(k->b_g2_query, w, k->n_vars, 0);
   G1J b1 = msm_pippenger<G1Curve>(k->b_g1_query, w, k->n_vars, 0);
   G1J l = msm_pippenger<G1Curve>(k->l_query, w + k->n_public + 1, nl, 0);
-  G1J hh = msm_pippenger<G1Curve>(k->h_query, h.data(), n, 0);
+  G1J hh = G1J::infinity();
+  if (k->h_basis == 1) {
+    // The MSM over gnark's n - 1 bases, then times -1/2 = 1/t on the odd coset.
+    hh = msm_pippenger<G1Curve>(k->h_query, h.data(), n - 1, 0);
+    uint64_t half[4];
+    (Fr::zero() - Fr::from_u64(2).inv()).to_std(half);
+    hh = hh.mul(half);
+  } else {
+    hh = msm_pippenger<G1Curve>(k->h_query, h.data(), n, 0);
+  }
   if (msm_out) {
     store_g1(msm_out, a.to_affine());
     store_g2(msm_out + 64, b2.to_affine());

@@ -66,7 +66,7 @@ class Oracle:
         lib.orc_threads.restype = ctypes.c_int
         for name in ("orc_field_op", "orc_ntt", "orc_coset_pipeline", "orc_on_curve",
                      "orc_scalar_mul", "orc_point_add", "orc_fixed_base", "orc_msm",
-                     "orc_gather", "orc_compute_h", "orc_prove"):
+                     "orc_gather", "orc_compute_h", "orc_prove", "orc_prove_rows"):
             getattr(lib, name).restype = ctypes.c_int
         lib.orc_field_op.argtypes = [ctypes.c_int, ctypes.c_int, _u8p, _u8p, _u8p, ctypes.c_size_t]
         lib.orc_ntt.argtypes = [_u8p, ctypes.c_int, ctypes.c_int]
@@ -80,6 +80,8 @@ class Oracle:
         lib.orc_gather.argtypes = [ctypes.c_void_p, ctypes.c_int, _u8p, _u8p]
         lib.orc_compute_h.argtypes = [ctypes.c_void_p, _u8p, _u8p]
         lib.orc_prove.argtypes = [ctypes.c_void_p, _u8p, _u8p, _u8p, _u8p, _u8p]
+        lib.orc_prove_rows.argtypes = [ctypes.c_void_p, _u8p, _u8p, _u8p, ctypes.c_size_t, _u8p,
+                                       _u8p, _u8p, _u8p]
         lib.orc_domain_root.argtypes = [ctypes.c_int, _u8p]
         lib.orc_g1_generator.argtypes = [_u8p]
         lib.orc_g2_generator.argtypes = [_u8p]
@@ -184,6 +186,23 @@ class Oracle:
         rc = self.lib.orc_compute_h(ctypes.byref(s), _p(witness), _p(out))
         assert rc == 0
         return out
+
+    def prove_rows(self, key: ProvingKey, witness: np.ndarray, a_rows: np.ndarray,
+                   b_rows: np.ndarray, r: int, s: int):
+        """prove_from_rows with fixed blinders: (msm outputs, proof)."""
+        st = key.struct()
+        a_rows = np.ascontiguousarray(a_rows, dtype=np.uint8).reshape(-1, 32)
+        b_rows = np.ascontiguousarray(b_rows, dtype=np.uint8).reshape(-1, 32)
+        assert len(a_rows) == len(b_rows)
+        rb, sb = fr_array([r]), fr_array([s])
+        msm = np.zeros(384, dtype=np.uint8)
+        proof = np.zeros(256, dtype=np.uint8)
+        rc = self.lib.orc_prove_rows(ctypes.byref(st), _p(np.ascontiguousarray(witness)),
+                                     _p(a_rows), _p(b_rows), len(a_rows), _p(rb), _p(sb),
+                                     _p(msm), _p(proof))
+        if rc != 0:
+            raise ValueError(f"orc_prove_rows: {rc}")
+        return msm, proof
 
     def prove(self, key: ProvingKey, witness: np.ndarray, r: int, s: int):
         """Returns (msm outputs, proof) as 384- and 256-byte arrays."""

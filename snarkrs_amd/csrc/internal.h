@@ -50,6 +50,7 @@ void ntt_dit_forward(void* data, int nvec, int log_n, const void* tw_fwd, const 
 void launch_bitrev_permute(void* dst, const void* src, int log_n, hipStream_t s);
 void launch_scale(void* data, size_t n, const void* factor, hipStream_t s);
 void launch_scale_rp(void* data, size_t n, const void* factor_rp, hipStream_t s);
+void launch_mul_table_rp(void* data, size_t n, const void* table_rp, hipStream_t s);
 
 // ---- prove stages (kernels_prove.hip) ----
 void launch_gather(const uint32_t* row_ptr, const uint32_t* signal, const uint32_t* value_id,
@@ -57,6 +58,7 @@ void launch_gather(const uint32_t* row_ptr, const uint32_t* signal, const uint32
                    hipStream_t s);
 void launch_mul_into(void* out, const void* a, const void* b, size_t n, hipStream_t s);
 void launch_h(void* h, const void* a, const void* b, const void* c, size_t n, hipStream_t s);
+void launch_rows_abc(void* a, void* b, void* c, size_t rows, size_t n, hipStream_t s);
 
 // ---- MSM (kernels_msm.hip) ----
 struct MsmPlan {

@@ -231,6 +231,20 @@ __global__ void k_scale_rp(uint4* data, size_t n, const uint4* fp) {
   stfr(data + 2 * i, mul_rp(ldfr(data + 2 * i), ldfr(fp)));
 }
 
+// data[i] *= table[i] (table in the R' form).
+__global__ void k_mul_table_rp(uint4* data, size_t n, const uint4* table) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  stfr(data + 2 * i, mul_rp(ldfr(data + 2 * i), ldfr(table + 2 * i)));
+}
+
+void launch_mul_table_rp(void* data, size_t n, const void* table_rp, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_mul_table_rp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                     static_cast<uint4*>(data), n, static_cast<const uint4*>(table_rp));
+  G16_LAUNCH_CHECK("table product");
+}
+
 void launch_scale_rp(void* data, size_t n, const void* factor_rp, hipStream_t s) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_scale_rp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,

@@ -59,6 +59,30 @@ void launch_mul_into(void* out, const void* a, const void* b, size_t n, hipStrea
   G16_LAUNCH_CHECK("pointwise product");
 }
 
+// Stage 0 from the caller's rows (h_and_msms_from_rows, crates/groth16/src/cpu.rs:387-424):
+// a[0..rows) and b[0..rows) hold A.w and B.w already; C = A*B there, and all three are zero
+// on the padding rows (written every proof: the transforms run in place on the slot's buffer).
+__global__ void k_rows_abc(uint4* a, uint4* b, uint4* c, size_t rows, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (i < rows) {
+    stfr4(c + 2 * i, mul(ldfr4(a + 2 * i), ldfr4(b + 2 * i)));
+  } else {
+    Fr z = fp_zero<FrCfg>();
+    stfr4(a + 2 * i, z);
+    stfr4(b + 2 * i, z);
+    stfr4(c + 2 * i, z);
+  }
+}
+
+void launch_rows_abc(void* a, void* b, void* c, size_t rows, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_rows_abc, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                     static_cast<uint4*>(a), static_cast<uint4*>(b), static_cast<uint4*>(c),
+                     rows, n);
+  G16_LAUNCH_CHECK("rows: C = A*B");
+}
+
 // H = A*B - C on the odd coset; no division by Z (folded into the H bases at setup).
 __global__ void k_h(uint4* h, const uint4* a, const uint4* b, const uint4* c, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -72,6 +72,9 @@ struct g16hip_circuit {
   bool merged = false; // bases stored as window tables (one bucket set per MSM)
   G1Aff alpha_g1, beta_g1, delta_g1;
   G2Aff beta_g2, delta_g2;
+  bool has_matrices = true;  // false: proves from A.w and B.w rows only
+  uint32_t h_basis = 0;      // 0 snarkjs' odd coset, 1 gnark's bit-reversed coefficients
+  uint32_t n_h = 0;          // H bases: domain_size, or domain_size - 1 for gnark's
   VkBytes vk;  // the verifying key, for the check of a proof's result
   DevBuf row_ptr[2], signal[2], value_id[2], values;
   DevBuf a_q, b1_q, b2_q, l_q, h_q;
@@ -101,7 +104,13 @@ static void validate_key(const g16hip_key* k) {
   if (n > (1u << 27)) throw ArgError{"domain size exceeds 2^27"};
   if ((uint64_t)k->n_public + 1 > k->n_vars)
     throw ArgError{"n_public " + std::to_string(k->n_public) + " exceeds n_vars " + std::to_string(k->n_vars)};
-  for (int m = 0; m < 2; ++m) {
+  if (k->h_basis > 1) throw ArgError{"unknown H basis " + std::to_string(k->h_basis)};
+  if (k->h_basis == 1 && n < 2) throw ArgError{"a gnark-basis key needs a domain of at least 2"};
+  if (!k->a_query || !k->b_g1_query || !k->b_g2_query || !k->h_query ||
+      (!k->l_query && k->n_vars > k->n_public + 1))
+    throw ArgError{"missing query points"};
+  const bool matrices = k->row_ptr[0] || k->row_ptr[1];
+  for (int m = 0; m < 2 && matrices; ++m) {
     const uint32_t* rp = k->row_ptr[m];
     if (!rp) throw ArgError{"missing row_ptr"};
     if (rp[0] != 0) throw ArgError{"matrix " + std::to_string(m) + " row_ptr starts at " + std::to_string(rp[0])};
@@ -131,6 +140,9 @@ static void upload_key(g16hip_circuit* C, const g16hip_key* k, hipStream_t s) {
   C->domain = n;
   C->log_n = 0;
   while ((1u << C->log_n) < n) ++C->log_n;
+  C->has_matrices = k->row_ptr[0] != nullptr;
+  C->h_basis = k->h_basis;
+  C->n_h = k->h_basis == 1 ? n - 1 : n;
   C->alpha_g1 = g1_load(k->alpha_g1);
   C->beta_g1 = g1_load(k->beta_g1);
   C->delta_g1 = g1_load(k->delta_g1);
@@ -141,18 +153,19 @@ static void upload_key(g16hip_circuit* C, const g16hip_key* k, hipStream_t s) {
   std::memcpy(C->vk.gamma_g2, k->gamma_g2, 128);
   std::memcpy(C->vk.delta_g2, k->delta_g2, 128);
   C->vk.ic.assign(k->ic, k->ic + 64 * ((size_t)k->n_public + 1));
-  for (int m = 0; m < 2; ++m) {
+  for (int m = 0; m < 2 && C->has_matrices; ++m) {
     uint32_t nnz = k->row_ptr[m][n];
     upload(C->row_ptr[m], k->row_ptr[m], 4 * (size_t)(n + 1), s);
     upload(C->signal[m], k->signal[m], 4 * (size_t)nnz, s);
     upload(C->value_id[m], k->value_id[m], 4 * (size_t)nnz, s);
   }
-  upload(C->values, k->values, 32 * (size_t)k->n_values, s);
+  if (C->has_matrices) upload(C->values, k->values, 32 * (size_t)k->n_values, s);
+  const size_t nh = C->n_h;
   size_t nv = k->n_vars;
   // The witness digits feed A, B1, L (G1) and B2 (G2); H feeds one G1 MSM.
   // G16HIP_C_WITNESS / G16HIP_C_H override the widths (2..16).
   C->cw = msm_choose_c((uint32_t)nv, 3, 1);
-  C->ch = msm_default_c(n);
+  C->ch = msm_default_c((uint32_t)nh);
   if (const char* e = std::getenv("G16HIP_C_WITNESS")) C->cw = std::max(2, std::min(16, std::atoi(e)));
   if (const char* e = std::getenv("G16HIP_C_H")) C->ch = std::max(2, std::min(16, std::atoi(e)));
   const int Ww = (255 + C->cw - 1) / C->cw, Wh = (255 + C->ch - 1) / C->ch;
@@ -161,7 +174,7 @@ static void upload_key(g16hip_circuit* C, const g16hip_key* k, hipStream_t s) {
   // the window reduction run once instead of once per window. They take W times the bases'
   // memory and are used when the device keeps 8 GiB free after them (G16HIP_WINDOW_TABLES=0
   // turns them off).
-  const size_t table_bytes = (3 * 64 + 128) * nv * Ww + 64 * (size_t)n * Wh;
+  const size_t table_bytes = (3 * 64 + 128) * nv * Ww + 64 * nh * Wh;
   size_t free_b = 0, total_b = 0;
   G16_CHECK(hipMemGetInfo(&free_b, &total_b));
   const char* wt = std::getenv("G16HIP_WINDOW_TABLES");
@@ -182,25 +195,27 @@ static void upload_key(g16hip_circuit* C, const g16hip_key* k, hipStream_t s) {
   if (nv > pub)
     G16_CHECK(hipMemcpyAsync(static_cast<char*>(C->l_q.p) + 64 * pub, k->l_query, 64 * (nv - pub),
                              hipMemcpyHostToDevice, s));
-  put(C->h_q, k->h_query, n, 64, th);
+  put(C->h_q, k->h_query, nh, 64, th);
   // The bases serve only the MSMs: keep them in the MSM kernels' R' = 2^261 form.
   msm_to_msm_form(C->a_q.p, C->a_q.p, 2 * nv, s);
   msm_to_msm_form(C->b1_q.p, C->b1_q.p, 2 * nv, s);
   msm_to_msm_form(C->l_q.p, C->l_q.p, 2 * nv, s);
-  msm_to_msm_form(C->h_q.p, C->h_q.p, 2 * (size_t)n, s);
+  msm_to_msm_form(C->h_q.p, C->h_q.p, 2 * nh, s);
   msm_to_msm_form(C->b2_q.p, C->b2_q.p, 4 * nv, s);
   if (C->merged) {
     msm_window_tables(1, C->a_q.p, nv, C->cw, Ww, s);
     msm_window_tables(1, C->b1_q.p, nv, C->cw, Ww, s);
     msm_window_tables(1, C->l_q.p, nv, C->cw, Ww, s);
     msm_window_tables(2, C->b2_q.p, nv, C->cw, Ww, s);
-    msm_window_tables(1, C->h_q.p, n, C->ch, Wh, s);
+    msm_window_tables(1, C->h_q.p, nh, C->ch, Wh, s);
   }
   build_ntt_tables(C->tables, C->log_n, s);
+  if (C->h_basis == 1) build_gnark_h_table(C->tables, C->log_n, s);
   G16_CHECK(hipStreamSynchronize(s));
   DevBuf* bufs[] = {&C->row_ptr[0], &C->row_ptr[1], &C->signal[0], &C->signal[1],
                     &C->value_id[0], &C->value_id[1], &C->values, &C->a_q, &C->b1_q, &C->b2_q,
-                    &C->l_q, &C->h_q, &C->tables.fwd, &C->tables.inv, &C->tables.coset};
+                    &C->l_q, &C->h_q, &C->tables.fwd, &C->tables.inv, &C->tables.coset,
+                    &C->tables.gnark};
   C->key_bytes = 0;
   for (auto b : bufs) C->key_bytes += b->bytes;
 }
@@ -238,9 +253,9 @@ static void size_slot(const g16hip_circuit* C, Slot* S) {
   S->abc.alloc(32 * 3 * n);
   S->hbuf.alloc(32 * n);
   S->sort_w.alloc(msm_sort_scratch_bytes((uint32_t)nv, C->cw));
-  S->sort_h.alloc(msm_sort_scratch_bytes((uint32_t)n, C->ch));
+  S->sort_h.alloc(msm_sort_scratch_bytes(C->n_h, C->ch));
   msm_plan_bind(S->plan_w, S->sort_w.p, (uint32_t)nv, C->cw, C->merged);
-  msm_plan_bind(S->plan_h, S->sort_h.p, (uint32_t)n, C->ch, C->merged);
+  msm_plan_bind(S->plan_h, S->sort_h.p, C->n_h, C->ch, C->merged);
   S->accum_w1.alloc(msm_accum_scratch_bytes(S->plan_w, 1));
   S->accum_w2.alloc(msm_accum_scratch_bytes(S->plan_w, 2));
   S->accum_h.alloc(msm_accum_scratch_bytes(S->plan_h, 1));
@@ -271,12 +286,12 @@ static size_t slot_bytes_estimate(const g16hip_circuit* C) {
   pw.merged = C->merged;
   ph.c = C->ch;
   ph.windows = (255 + C->ch - 1) / C->ch;
-  ph.n = (uint32_t)n;
+  ph.n = C->n_h;
   ph.nb = 1u << (C->ch - 1);
-  ph.max_entries = n * ph.windows;
+  ph.max_entries = (size_t)C->n_h * ph.windows;
   ph.merged = C->merged;
   return 32 * nv + 32 * 4 * n + msm_sort_scratch_bytes((uint32_t)nv, C->cw) +
-         msm_sort_scratch_bytes((uint32_t)n, C->ch) + msm_accum_scratch_bytes(pw, 1) +
+         msm_sort_scratch_bytes(C->n_h, C->ch) + msm_accum_scratch_bytes(pw, 1) +
          msm_accum_scratch_bytes(pw, 2) + msm_accum_scratch_bytes(ph, 1) +
          256 * (4 * msm_out_points(pw) + msm_out_points(ph));
 }
@@ -344,26 +359,75 @@ static void load_witness(const g16hip_circuit* C, Slot* S, const void* w, size_t
                            S->stream));
 }
 
-// Stages 0-4 on the slot's main stream into `h_out` (device).
-static void stages_0_4(const g16hip_circuit* C, Slot* S, void* h_out, bool timed) {
+// Stage 0 handed over by the caller: constraint i's A.w and B.w (host memory), i < n.
+struct Rows {
+  const uint8_t* a;
+  const uint8_t* b;
+  size_t n;
+};
+
+// The gather path needs the key's matrices and snarkjs' H basis; a key without them proves
+// from rows (crates/groth16/src/cpu.rs:176-189).
+static void require_gather(const g16hip_circuit* C) {
+  if (C->h_basis == 1)
+    throw ArgError{"a gnark key carries no constraint matrices; prove it from A.w and B.w with "
+                   "h_and_msms_from_rows"};
+  if (!C->has_matrices)
+    throw ArgError{"the key carries no constraint matrices; prove it from A.w and B.w with "
+                   "h_and_msms_from_rows"};
+}
+
+// Stages 0-4 on the slot's main stream into `h_out` (device): stage 0 by gather or from
+// `rows`, then H on the odd coset, taken to gnark's coefficient basis for a gnark key.
+static void stages_0_4(const g16hip_circuit* C, Slot* S, void* h_out, bool timed,
+                       const Rows* rows = nullptr) {
   hipStream_t s = S->stream;
   size_t n = C->domain;
   char* abc = S->abc.as<char>();
   void* A = abc;
   void* B = abc + 32 * n;
   void* Cv = abc + 64 * n;
+  if (!rows) require_gather(C);
   if (timed) G16_CHECK(hipEventRecord(S->ev[0], s));
-  launch_gather(C->row_ptr[0].as<uint32_t>(), C->signal[0].as<uint32_t>(),
-                C->value_id[0].as<uint32_t>(), C->values.p, S->witness.p, A, C->domain, s);
-  launch_gather(C->row_ptr[1].as<uint32_t>(), C->signal[1].as<uint32_t>(),
-                C->value_id[1].as<uint32_t>(), C->values.p, S->witness.p, B, C->domain, s);
-  launch_mul_into(Cv, A, B, n, s);
+  if (rows) {
+    if (rows->n) {
+      G16_CHECK(hipMemcpyAsync(A, rows->a, 32 * rows->n, hipMemcpyHostToDevice, s));
+      G16_CHECK(hipMemcpyAsync(B, rows->b, 32 * rows->n, hipMemcpyHostToDevice, s));
+    }
+    launch_rows_abc(A, B, Cv, rows->n, n, s);
+  } else {
+    launch_gather(C->row_ptr[0].as<uint32_t>(), C->signal[0].as<uint32_t>(),
+                  C->value_id[0].as<uint32_t>(), C->values.p, S->witness.p, A, C->domain, s);
+    launch_gather(C->row_ptr[1].as<uint32_t>(), C->signal[1].as<uint32_t>(),
+                  C->value_id[1].as<uint32_t>(), C->values.p, S->witness.p, B, C->domain, s);
+    launch_mul_into(Cv, A, B, n, s);
+  }
   if (timed) G16_CHECK(hipEventRecord(S->ev[1], s));
   ntt_dif_inverse(abc, 3, C->log_n, C->tables.inv.p, s);
   ntt_dit_forward(abc, 3, C->log_n, C->tables.fwd.p, C->tables.coset.p, s);
   if (timed) G16_CHECK(hipEventRecord(S->ev[2], s));
   launch_h(h_out, A, B, Cv, n, s);
+  if (C->h_basis == 1) {
+    // Coefficients in bit-reversed order, the order gnark's computeH leaves and pk.G1.Z
+    // expects (crates/groth16/src/cpu.rs:250-286).
+    ntt_dif_inverse(h_out, 1, C->log_n, C->tables.inv.p, s);
+    launch_mul_table_rp(h_out, n, C->tables.gnark.p, s);
+  }
   if (timed) G16_CHECK(hipEventRecord(S->ev[3], s));
+}
+
+// H's coefficient n - 1 (position n - 1 in the bit-reversed order) is zero whenever the
+// arithmetic is right, since A*B - C vanishes on the domain; a gnark-basis proof checks it.
+static void check_gnark_h(const g16hip_circuit* C, const Slot* S) {
+  if (C->h_basis != 1) return;
+  uint32_t last[8];
+  G16_CHECK(hipMemcpy(last, S->hbuf.as<char>() + 32 * ((size_t)C->domain - 1), 32,
+                      hipMemcpyDeviceToHost));
+  for (uint32_t v : last)
+    if (v)
+      throw StatusError{G16HIP_BACKEND,
+                        "H has a nonzero coefficient " + std::to_string(C->domain - 1) +
+                            ", so the quotient by the vanishing polynomial is not exact"};
 }
 
 static const size_t kMsmCount = 5;  // A, B1, L, B2, H
@@ -386,7 +450,8 @@ static WinLayout win_layout(const Slot* S) {
 // rows are written by its kernel into pinned host memory. With `timed`, events bracket every
 // phase (done_ev[k] marks MSM k's rows on the host) and the entry counts are copied out at
 // the end for the kernel statistics; a graph capture passes timed = false.
-static void enqueue_stages(const g16hip_circuit* C, Slot* S, bool with_h, void* h, bool timed) {
+static void enqueue_stages(const g16hip_circuit* C, Slot* S, bool with_h, void* h, bool timed,
+                           const Rows* rows = nullptr) {
   hipStream_t s0 = S->stream, s1 = S->side[0], s2 = S->side[1];
   MsmPlan& pw = S->plan_w;
   MsmPlan& ph = S->plan_h;
@@ -419,7 +484,7 @@ static void enqueue_stages(const g16hip_circuit* C, Slot* S, bool with_h, void* 
   msm_accumulate(pw, 1, C->l_q.p, S->accum_w1.p, wout + L.off[2], s1, mev2);
   rows_done(2, s1);
   G16_CHECK(hipEventRecord(S->sync_ev[2], s1));
-  if (with_h) stages_0_4(C, S, h, timed);
+  if (with_h) stages_0_4(C, S, h, timed, rows);
   msm_sort(ph, h, s0);
   msm_accumulate(ph, 1, C->h_q.p, S->accum_h.p, wout + L.off[4], s0, mev4);
   rows_done(4, s0);
@@ -498,8 +563,8 @@ static void sync_slot(Slot* S) {
 
 // Eager run with events: timings, kernel statistics, the optional per-proof timeline.
 static void run_timed(g16hip_circuit* C, Slot* S, bool with_h, void* h, g16hip_msm_out* out,
-                      g16hip_timings* t, BlindProducts* blind) {
-  enqueue_stages(C, S, with_h, h, true);
+                      g16hip_timings* t, BlindProducts* blind, const Rows* rows = nullptr) {
+  enqueue_stages(C, S, with_h, h, true, rows);
   std::exception_ptr err;
   try {
     finish_msms(C, S, true, out, blind);
@@ -508,6 +573,7 @@ static void run_timed(g16hip_circuit* C, Slot* S, bool with_h, void* h, g16hip_m
   }
   sync_slot(S);
   if (err) std::rethrow_exception(err);
+  if (rows) check_gnark_h(C, S);
   WinLayout L = win_layout(S);
   const uint32_t* totals = reinterpret_cast<const uint32_t*>(S->host_wout + L.totals);
   for (size_t k = 0; k < kMsmCount; ++k)
@@ -606,7 +672,7 @@ static void assemble(const g16hip_circuit* C, const g16hip_msm_out& m, const Bli
 // graph replay.
 static void prove_on_slot(g16hip_circuit* C, Slot* S, const void* w, size_t n, bool on_device,
                           const uint8_t* r8, const uint8_t* s8, g16hip_proof* proof,
-                          g16hip_timings* t, bool graph) {
+                          g16hip_timings* t, bool graph, const Rows* rows = nullptr) {
   check_witness_len(C, n);
   // w[0] is the constant-one wire (crates/groth16/src/prove.rs:272-283).
   Fr w0;
@@ -624,8 +690,8 @@ static void prove_on_slot(g16hip_circuit* C, Slot* S, const void* w, size_t n, b
   HostTask th([&] { terms = blinder_terms(C, bp.r_std, bp.s_std, rs_std); });
   g16hip_msm_out m;
   load_witness(C, S, w, n, on_device);
-  if (graph) run_graph(C, S, &m, &bp);
-  else run_timed(C, S, true, S->hbuf.p, &m, t, &bp);
+  if (graph && !rows) run_graph(C, S, &m, &bp);
+  else run_timed(C, S, true, S->hbuf.p, &m, t, &bp, rows);
   th.join();
   if (th.err) std::rethrow_exception(th.err);
   Timer ta;
@@ -875,6 +941,29 @@ g16hip_status g16hip_h_and_msms_device(g16hip_circuit* C, const void* w, size_t 
   return h_and_msms_impl(C, w, n, true, out, t);
 }
 
+// The rows' shape (crates/groth16/src/cpu.rs:394-406).
+static Rows check_rows(const g16hip_circuit* C, const uint8_t* a, const uint8_t* b, size_t rows) {
+  if (rows > C->domain)
+    throw ArgError{"A.w and B.w have " + std::to_string(rows) + " rows; both must be one per "
+                   "constraint, at most the domain size " + std::to_string(C->domain)};
+  if (rows && (!a || !b)) throw ArgError{"null rows"};
+  return Rows{a, b, rows};
+}
+
+g16hip_status g16hip_h_and_msms_from_rows(g16hip_circuit* C, const uint8_t* w, size_t n,
+                                          const uint8_t* a_rows, const uint8_t* b_rows,
+                                          size_t n_rows, g16hip_msm_out* out, g16hip_timings* t) {
+  return guarded([&] {
+    if (!C || !w || !out) throw ArgError{"null argument"};
+    Rows rows = check_rows(C, a_rows, b_rows, n_rows);
+    set_device(C->dev);
+    SlotLease lease(C);
+    load_witness(C, lease.S, w, n, false);
+    run_timed(C, lease.S, true, lease.S->hbuf.p, out, t, nullptr, &rows);
+    return G16HIP_OK;
+  });
+}
+
 static g16hip_status prove_impl(g16hip_circuit* C, const void* w, size_t n, bool on_device,
                                 const uint8_t* r, const uint8_t* s, g16hip_proof* proof,
                                 g16hip_timings* t) {
@@ -898,6 +987,28 @@ g16hip_status g16hip_prove_with_blinders(g16hip_circuit* C, const uint8_t* w, si
 g16hip_status g16hip_prove(g16hip_circuit* C, const uint8_t* w, size_t n, const uint8_t* r,
                            const uint8_t* s, g16hip_proof* proof, g16hip_timings* t) {
   g16hip_status st = prove_impl(C, w, n, false, r, s, proof, t);
+  if (st != G16HIP_OK) return st;
+  Timer tv;
+  std::string why;
+  st = groth16_check(C->vk, w + 32, C->n_public, *proof, VerifyMode::Checked, &why);
+  if (t) t->verify_us += tv.us();
+  if (st != G16HIP_OK) return fail(G16HIP_SELF_VERIFY, "the proof failed its check: " + why);
+  return G16HIP_OK;
+}
+
+// prove_from_rows (crates/groth16/src/prove.rs:93-115): checked like prove.
+g16hip_status g16hip_prove_from_rows(g16hip_circuit* C, const uint8_t* w, size_t n,
+                                     const uint8_t* a_rows, const uint8_t* b_rows, size_t n_rows,
+                                     const uint8_t* r, const uint8_t* s, g16hip_proof* proof,
+                                     g16hip_timings* t) {
+  g16hip_status st = guarded([&] {
+    if (!C || !w || !r || !s || !proof) throw ArgError{"null argument"};
+    Rows rows = check_rows(C, a_rows, b_rows, n_rows);
+    set_device(C->dev);
+    SlotLease lease(C);
+    prove_on_slot(C, lease.S, w, n, false, r, s, proof, t, false, &rows);
+    return G16HIP_OK;
+  });
   if (st != G16HIP_OK) return st;
   Timer tv;
   std::string why;

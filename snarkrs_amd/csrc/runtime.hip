@@ -21,13 +21,15 @@ void set_last_error(const std::string& msg) { t_last_error = msg; }
 void build_ntt_tables(NttTables& t, int L, hipStream_t s) {
   size_t n = (size_t)1 << L;
   size_t half = n / 2 ? n / 2 : 1;
-  Fr host[6];
+  Fr host[8];
   host[0] = domain_root(L);
   host[1] = inv(host[0]);
   host[2] = domain_root(L + 1);  // coset shift: primitive 2n-th root (snarkjs' odd coset)
   host[3] = inv(fr_from_u64(n));
   host[4] = fp_one<FrCfg>();
   host[5] = fp_zero<FrCfg>();
+  host[6] = inv(host[2]);
+  host[7] = neg(inv(fr_from_u64(2 * n)));
   upload(t.scalars, host, sizeof host, s);
   t.fwd.alloc(32 * (n > 1 ? n - 1 : 1));
   t.inv.alloc(32 * (n > 1 ? n - 1 : 1));
@@ -42,6 +44,18 @@ void build_ntt_tables(NttTables& t, int L, hipStream_t s) {
   launch_coset_table(t.coset.p, sc + 2, sc + 3, L, s);
   launch_fr_to_rp(t.coset.p, n, s);
   G16_CHECK(hipStreamSynchronize(s));  // before `powers` is freed
+}
+
+// H's odd-coset evaluations to gnark's basis (crates/cuda/src/kernels/rows.cu:37-68): the
+// unscaled DIF inverse leaves n g_bitrev(p) at position p, where g are the coefficients of
+// V(shift x), V = A*B - C = t H, and t = x^n - 1 = -2 on the odd coset; so H's coefficient
+// bitrev(p) is that times shift^-bitrev(p) / n * (-1/2), which this table holds (R' form).
+void build_gnark_h_table(NttTables& t, int L, hipStream_t s) {
+  size_t n = (size_t)1 << L;
+  t.gnark.alloc(32 * n);
+  Fr* sc = t.scalars.as<Fr>();
+  launch_coset_table(t.gnark.p, sc + 6, sc + 7, L, s);
+  launch_fr_to_rp(t.gnark.p, n, s);
 }
 
 NttTables& ntt_tables_for(g16hip_device* d, int L) {
@@ -94,11 +108,200 @@ static Xyzz<F> run_msm(g16hip_device* d, int group, const void* bases, const voi
 }
 
 
+// Device bytes one standalone MSM of n points at width c takes: the uploaded bases and
+// scalars, the bases in the MSM form, the sort scratch and the accumulation scratch.
+static size_t msm_run_bytes(int group, uint32_t n, int c) {
+  size_t pw = group == 1 ? 64 : 128;
+  MsmPlan plan;
+  static char anchor;  // the plan is bound only to size its pieces
+  msm_plan_bind(plan, &anchor, n, c);
+  return 2 * pw * (size_t)n + 32 * (size_t)n + msm_sort_scratch_bytes(n, c) +
+         msm_accum_scratch_bytes(plan, group) + 256 * msm_out_points(plan);
+}
+
+// One MSM over host slices: upload, run, result in XYZZ (the caller holds d->mu).
+template <class F>
+static Xyzz<F> msm_from_host(g16hip_device* d, int group, const uint8_t* bases,
+                             const uint8_t* scalars, size_t n) {
+  size_t pw = group == 1 ? 64 : 128;
+  upload(d->scratch_a, bases, n * pw, d->stream);
+  upload(d->scratch_b, scalars, n * 32, d->stream);
+  return run_msm<F>(d, group, d->scratch_a.p, d->scratch_b.p, (uint32_t)n, 0);
+}
+
+static Fq fq_small(uint32_t k) {
+  Fq x = fp_zero<FqCfg>();
+  x.v[0] = k;
+  return from_std(x);
+}
+
+template <class F>
+static bool xyzz_on_curve(const Xyzz<F>& p);
+
+template <>
+bool xyzz_on_curve<Fq>(const G1X& p) {
+  G1Aff a = xyzz_to_aff(p);
+  if (aff_is_inf(a)) return true;
+  Fq b = fq_small(3);
+  return eq(sqr(a.y), add(mul(sqr(a.x), a.x), b));
+}
+
+template <>
+bool xyzz_on_curve<Fq2>(const G2X& p) {
+  G2Aff a = xyzz_to_aff(p);
+  if (aff_is_inf(a)) return true;
+  // b' = 3 / (9 + u)
+  Fq2 xi{fq_small(9), fq_small(1)};
+  Fq2 three{fq_small(3), fp_zero<FqCfg>()};
+  Fq2 b = mul(three, inv(xi));
+  return eq(sqr(a.y), add(mul(sqr(a.x), a.x), b));
+}
+
+}  // namespace g16
+
+// The ceremony's MSM seam (crates/cuda/src/ceremony.rs:65-312): MSMs over host slices for
+// `groth16 setup` and `zkey verify`'s recompute, cut into runs of at most max_run points.
+struct g16hip_ceremony {
+  g16hip_device* dev = nullptr;
+  size_t max_run = 1u << 21;
+  size_t vram_limit = 0;  // 0: whatever hipMalloc grants
+  std::mutex one;         // one call on the device at a time
+  uint64_t last_runs = 0;
+  g16hip_ceremony_stats tally = {};
+};
+
+namespace g16 {
+
+constexpr size_t CEREMONY_MIN_RUN = 1u << 10;
+constexpr int CEREMONY_ATTEMPTS = 8;
+
+// One run, re-run while its sum is off the curve (CEREMONY_ATTEMPTS in all); a memory
+// refusal comes straight out, before anything is allocated when a limit is set.
+template <class F>
+static Xyzz<F> ceremony_run(g16hip_ceremony* cer, int group, const uint8_t* bases,
+                            const uint8_t* scalars, size_t n) {
+  g16hip_device* d = cer->dev;
+  if (cer->vram_limit) {
+    size_t need = msm_run_bytes(group, (uint32_t)n, msm_default_c((uint32_t)n));
+    if (need > cer->vram_limit)
+      throw StatusError{G16HIP_DEVICE_MEMORY,
+                        "a run of " + std::to_string(n) + " points needs " + std::to_string(need) +
+                            " bytes, over the limit of " + std::to_string(cer->vram_limit)};
+  }
+  for (int attempt = 1;; ++attempt) {
+    Xyzz<F> p;
+    {
+      std::lock_guard<std::mutex> lk(d->mu);
+      set_device(d);
+      p = msm_from_host<F>(d, group, bases, scalars, n);
+    }
+    if (xyzz_on_curve(p)) return p;
+    cer->tally.retries += 1;
+    if (attempt >= CEREMONY_ATTEMPTS)
+      throw StatusError{G16HIP_DEVICE, "ceremony MSM: the sum is not a point on the curve after " +
+                                           std::to_string(attempt) + " attempts"};
+    std::fprintf(stderr, "g16hip: ceremony MSM attempt %d of %d gave a sum off the curve; re-running\n",
+                 attempt, CEREMONY_ATTEMPTS);
+  }
+}
+
+template <class F>
+static Xyzz<F> ceremony_msm(g16hip_ceremony* cer, int group, const uint8_t* bases,
+                            const uint8_t* scalars, size_t n) {
+  size_t pw = group == 1 ? 64 : 128;
+  Timer t;
+  Xyzz<F> sum = xyzz_inf<F>();
+  size_t at = 0, len = std::max(cer->max_run, CEREMONY_MIN_RUN);
+  uint64_t runs = 0;
+  while (at < n) {
+    size_t m = std::min(len, n - at);
+    try {
+      Xyzz<F> p = ceremony_run<F>(cer, group, bases + pw * at, scalars + 32 * at, m);
+      sum = xyzz_add(sum, p);
+      at += m;
+      ++runs;
+    } catch (const StatusError& e) {
+      if (e.st != G16HIP_DEVICE_MEMORY || m <= CEREMONY_MIN_RUN) throw;
+      len = std::max(m / 2, CEREMONY_MIN_RUN);
+    } catch (const HipError& e) {
+      bool oom = e.msg.find("out of memory") != std::string::npos;
+      if (!oom || m <= CEREMONY_MIN_RUN) throw;
+      len = std::max(m / 2, CEREMONY_MIN_RUN);
+    }
+  }
+  cer->last_runs = runs;
+  cer->tally.calls += 1;
+  cer->tally.runs += runs;
+  cer->tally.points += n;
+  cer->tally.ms += t.ms();
+  return sum;
+}
+
 }  // namespace g16
 
 using namespace g16;
 
 extern "C" {
+
+g16hip_status g16hip_ceremony_open(g16hip_device* d, size_t max_run, size_t vram_limit,
+                                   g16hip_ceremony** out) {
+  return guarded([&] {
+    if (!d || !out) throw ArgError{"null argument"};
+    auto* c = new g16hip_ceremony();
+    c->dev = d;
+    if (max_run) c->max_run = max_run;
+    c->vram_limit = vram_limit;
+    *out = c;
+    return G16HIP_OK;
+  });
+}
+
+void g16hip_ceremony_close(g16hip_ceremony* c) { delete c; }
+
+g16hip_status g16hip_ceremony_msm(g16hip_ceremony* c, int group, const uint8_t* bases,
+                                  const uint8_t* scalars, size_t n, uint8_t* out) {
+  return guarded([&] {
+    if (!c || !out || (n && (!bases || !scalars))) throw ArgError{"null argument"};
+    if (group != 1 && group != 2) throw ArgError{"group must be 1 or 2"};
+    size_t pw = group == 1 ? 64 : 128;
+    std::lock_guard<std::mutex> lk(c->one);
+    if (n == 0) {
+      c->last_runs = 0;
+      std::memset(out, 0, pw);
+      return G16HIP_OK;
+    }
+    if (group == 1) g1_store(out, xyzz_to_aff(ceremony_msm<Fq>(c, 1, bases, scalars, n)));
+    else g2_store(out, xyzz_to_aff(ceremony_msm<Fq2>(c, 2, bases, scalars, n)));
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_msm_scratch_bytes(int group, size_t n, size_t* bytes) {
+  return guarded([&] {
+    if (!bytes || (group != 1 && group != 2)) throw ArgError{"bad argument"};
+    if (n >= (1u << 31)) throw ArgError{"MSM larger than 2^31 points"};
+    *bytes = n ? msm_run_bytes(group, (uint32_t)n, msm_default_c((uint32_t)n)) : 0;
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_ceremony_last_runs(g16hip_ceremony* c, uint64_t* runs) {
+  return guarded([&] {
+    if (!c || !runs) throw ArgError{"null argument"};
+    std::lock_guard<std::mutex> lk(c->one);
+    *runs = c->last_runs;
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_ceremony_stats_get(g16hip_ceremony* c, g16hip_ceremony_stats* out) {
+  return guarded([&] {
+    if (!c || !out) throw ArgError{"null argument"};
+    std::lock_guard<std::mutex> lk(c->one);
+    *out = c->tally;
+    return G16HIP_OK;
+  });
+}
 
 const char* g16hip_last_error(void) { return t_last_error.c_str(); }
 
@@ -216,12 +419,8 @@ static g16hip_status msm_host(g16hip_device* d, int group, const uint8_t* bases,
     if (n >= (1u << 31)) throw ArgError{"MSM larger than 2^31 points"};
     std::lock_guard<std::mutex> lk(d->mu);
     set_device(d);
-    upload(d->scratch_a, bases, n * pw, d->stream);
-    upload(d->scratch_b, scalars, n * 32, d->stream);
-    if (group == 1)
-      g1_store(out, xyzz_to_aff(run_msm<Fq>(d, 1, d->scratch_a.p, d->scratch_b.p, (uint32_t)n, 0)));
-    else
-      g2_store(out, xyzz_to_aff(run_msm<Fq2>(d, 2, d->scratch_a.p, d->scratch_b.p, (uint32_t)n, 0)));
+    if (group == 1) g1_store(out, xyzz_to_aff(msm_from_host<Fq>(d, 1, bases, scalars, n)));
+    else g2_store(out, xyzz_to_aff(msm_from_host<Fq2>(d, 2, bases, scalars, n)));
     return G16HIP_OK;
   });
 }

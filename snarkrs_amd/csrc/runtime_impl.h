@@ -206,10 +206,12 @@ struct HostTask {
 // ---- devices ---------------------------------------------------------------------------
 struct NttTables {
   // fwd, inv: per-pass twiddle tables (2^L - 1 entries); coset: shift^bitrev(p) / n; both in
-  // the R' = 2^261 form. scalars (R form): [w, w^-1, shift, 1/n, 1, 0].
-  DevBuf fwd, inv, coset, scalars;
+  // the R' = 2^261 form. scalars (R form): [w, w^-1, shift, 1/n, 1, 0, shift^-1, -1/(2n)].
+  // gnark (built for gnark-basis keys only): -shift^-bitrev(p) / (2n), R' form.
+  DevBuf fwd, inv, coset, scalars, gnark;
 };
 void build_ntt_tables(NttTables& t, int L, hipStream_t s);
+void build_gnark_h_table(NttTables& t, int L, hipStream_t s);
 
 }  // namespace g16
 

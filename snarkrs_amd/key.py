@@ -15,6 +15,9 @@ _u8p = ctypes.POINTER(ctypes.c_uint8)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 
 
+H_BASIS_SNARKJS, H_BASIS_GNARK = 0, 1
+
+
 class KeyStruct(ctypes.Structure):
     """``g16hip_key`` (include/g16hip.h)."""
 
@@ -39,6 +42,7 @@ class KeyStruct(ctypes.Structure):
         ("b_g2_query", _u8p),
         ("l_query", _u8p),
         ("h_query", _u8p),
+        ("h_basis", ctypes.c_uint32),
     ]
 
 
@@ -73,6 +77,10 @@ class ProvingKey:
     b_g2_query: np.ndarray
     l_query: np.ndarray
     h_query: np.ndarray
+    # What h_query is a basis for (HBasis, crates/formats/src/lib.rs:66-89): H_BASIS_SNARKJS
+    # (domain_size bases, the odd coset) or H_BASIS_GNARK (domain_size - 1 bases
+    # [tau^bitrev(j) t(tau) / delta]_1, H's coefficients in bit-reversed order).
+    h_basis: int = 0
     extra: dict = field(default_factory=dict)
 
     def struct(self) -> KeyStruct:
@@ -81,7 +89,7 @@ class ProvingKey:
         s.n_vars = self.n_vars
         s.n_public = self.n_public
         s.domain_size = self.domain_size
-        s.n_values = len(self.values)
+        s.n_values = 0 if self.values is None else len(self.values)
         s.alpha_g1 = _p8(self.alpha_g1)
         s.beta_g1 = _p8(self.beta_g1)
         s.beta_g2 = _p8(self.beta_g2)
@@ -89,16 +97,19 @@ class ProvingKey:
         s.delta_g1 = _p8(self.delta_g1)
         s.delta_g2 = _p8(self.delta_g2)
         s.ic = _p8(self.ic)
-        for m in range(2):
-            s.row_ptr[m] = _p32(self.row_ptr[m])
-            s.signal[m] = _p32(self.signal[m])
-            s.value_id[m] = _p32(self.value_id[m])
-        s.values = _p8(self.values)
+        if self.row_ptr is not None:  # a key without matrices proves from A.w, B.w rows only
+            for m in range(2):
+                s.row_ptr[m] = _p32(self.row_ptr[m])
+                s.signal[m] = _p32(self.signal[m])
+                s.value_id[m] = _p32(self.value_id[m])
+        if self.values is not None:
+            s.values = _p8(self.values)
         s.a_query = _p8(self.a_query)
         s.b_g1_query = _p8(self.b_g1_query)
         s.b_g2_query = _p8(self.b_g2_query)
         s.l_query = _p8(self.l_query)
         s.h_query = _p8(self.h_query)
+        s.h_basis = self.h_basis
         return s
 
     @property

@@ -159,6 +159,15 @@ _lib = None
 _lib_lock = threading.Lock()
 
 
+class CeremonyStats(ctypes.Structure):
+    """``g16hip_ceremony_stats``: where a ceremony handle's calls went."""
+    _fields_ = [("calls", ctypes.c_uint64), ("runs", ctypes.c_uint64), ("points", ctypes.c_uint64),
+                ("retries", ctypes.c_uint64), ("ms", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class VkStruct(ctypes.Structure):
     """``g16hip_vk`` = VerifyingKey (crates/formats/src/lib.rs:353-360)."""
     _fields_ = [("alpha_g1", _u8p), ("beta_g2", _u8p), ("gamma_g2", _u8p), ("delta_g2", _u8p),
@@ -234,6 +243,20 @@ EXPORTS = {
                                           ctypes.c_size_t]),
     "g16hip_valu_probe": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "g16hip_fixed_base": (ctypes.c_int, [_vp, ctypes.c_int, _u8p, ctypes.c_size_t, _u8p]),
+    "g16hip_ceremony_open": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t,
+                                            ctypes.POINTER(_vp)]),
+    "g16hip_ceremony_close": (None, [_vp]),
+    "g16hip_ceremony_msm": (ctypes.c_int, [_vp, ctypes.c_int, _u8p, _u8p, ctypes.c_size_t, _u8p]),
+    "g16hip_msm_scratch_bytes": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t,
+                                                ctypes.POINTER(ctypes.c_size_t)]),
+    "g16hip_ceremony_last_runs": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
+    "g16hip_ceremony_stats_get": (ctypes.c_int, [_vp, ctypes.POINTER(CeremonyStats)]),
+    "g16hip_h_and_msms_from_rows": (ctypes.c_int, [_vp, _u8p, ctypes.c_size_t, _u8p, _u8p,
+                                                   ctypes.c_size_t, ctypes.POINTER(MsmOut),
+                                                   ctypes.POINTER(Timings)]),
+    "g16hip_prove_from_rows": (ctypes.c_int, [_vp, _u8p, ctypes.c_size_t, _u8p, _u8p,
+                                              ctypes.c_size_t, _u8p, _u8p,
+                                              ctypes.POINTER(ProofOut), ctypes.POINTER(Timings)]),
     "g16hip_prove": (ctypes.c_int, [_vp, _u8p, ctypes.c_size_t, _u8p, _u8p,
                                     ctypes.POINTER(ProofOut), ctypes.POINTER(Timings)]),
     "g16hip_circuit_vk": (ctypes.c_int, [_vp, _u8p]),
@@ -484,6 +507,37 @@ class Circuit:
                                      ctypes.byref(out), ctypes.byref(t or Timings())))
         return out.as_array()
 
+    @staticmethod
+    def _rows(a_rows, b_rows):
+        a = _c8(a_rows).reshape(-1, 32)
+        b = _c8(b_rows).reshape(-1, 32)
+        if len(a) != len(b):
+            raise BackendError(BACKEND, f"A.w has {len(a)} rows and B.w has {len(b)}")
+        return a, b
+
+    def h_and_msms_from_rows(self, witness: np.ndarray, a_rows, b_rows,
+                             t: Timings | None = None) -> np.ndarray:
+        """``h_and_msms_from_rows`` (crates/groth16/src/lib.rs:425-444): stage 0 from the
+        caller's A.w and B.w (one per constraint, Fr Montgomery)."""
+        w = _c8(witness).reshape(-1, 32)
+        a, b = self._rows(a_rows, b_rows)
+        out = MsmOut()
+        _check(self.lib.g16hip_h_and_msms_from_rows(
+            self.h, _p8(w), len(w), _p8(a) if len(a) else None, _p8(b) if len(b) else None,
+            len(a), ctypes.byref(out), ctypes.byref(t or Timings())))
+        return out.as_array()
+
+    def prove_from_rows(self, witness: np.ndarray, a_rows, b_rows, r: np.ndarray,
+                        s: np.ndarray, t: Timings | None = None) -> np.ndarray:
+        """``prove_from_rows`` (crates/groth16/src/prove.rs:93-115), checked like ``prove``."""
+        w = _c8(witness).reshape(-1, 32)
+        a, b = self._rows(a_rows, b_rows)
+        out = ProofOut()
+        _check(self.lib.g16hip_prove_from_rows(
+            self.h, _p8(w), len(w), _p8(a) if len(a) else None, _p8(b) if len(b) else None,
+            len(a), _p8(_c8(r)), _p8(_c8(s)), ctypes.byref(out), ctypes.byref(t or Timings())))
+        return out.as_array()
+
     def vk_bytes(self) -> np.ndarray:
         """``key().vk`` as alpha_g1 | beta_g2 | gamma_g2 | delta_g2 | ic (wire encoding)."""
         nv, npub, dom = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
@@ -501,6 +555,68 @@ class Circuit:
                                                           ctypes.byref(out),
                                                           ctypes.byref(t or Timings())))
         return out.as_array()
+
+
+class CeremonyMsm:
+    """The ceremony's ``MsmBackend`` (``CudaMsmBackend``, crates/cuda/src/ceremony.rs:65-312):
+    MSMs over host slices, cut into runs of at most ``max_run`` points summed on the host,
+    a run refused for memory halved, a sum off the curve re-run. ``vram_limit`` (bytes, 0 =
+    none) refuses a run whose scratch would exceed it before allocating."""
+
+    MIN_RUN = 1 << 10
+
+    def __init__(self, dev: Device, max_run: int = 1 << 21, vram_limit: int = 0):
+        self.lib = dev.lib
+        self.dev = dev
+        self.h = _vp()
+        _check(self.lib.g16hip_ceremony_open(dev.h, max_run, vram_limit, ctypes.byref(self.h)))
+
+    @staticmethod
+    def name() -> str:
+        return "hip"
+
+    def _msm(self, group: int, bases, scalars) -> np.ndarray:
+        w = 64 if group == 1 else 128
+        bases = _c8(bases).reshape(-1, w)
+        scalars = _c8(scalars).reshape(-1, 32)
+        if len(bases) != len(scalars):
+            raise BackendError(BACKEND, f"{len(bases)} bases but {len(scalars)} scalars")
+        out = np.zeros(w, np.uint8)
+        n = len(bases)
+        _check(self.lib.g16hip_ceremony_msm(self.h, group, _p8(bases) if n else None,
+                                            _p8(scalars) if n else None, n, _p8(out)))
+        return out
+
+    def msm_g1(self, bases, scalars) -> np.ndarray:
+        return self._msm(1, bases, scalars)
+
+    def msm_g2(self, bases, scalars) -> np.ndarray:
+        return self._msm(2, bases, scalars)
+
+    @staticmethod
+    def scratch_bytes(group: int, n: int) -> int:
+        """Device bytes one run of n points takes (what ``vram_limit`` is compared with)."""
+        b = ctypes.c_size_t()
+        _check(load().g16hip_msm_scratch_bytes(group, n, ctypes.byref(b)))
+        return b.value
+
+    def last_runs(self) -> int:
+        r = ctypes.c_uint64()
+        _check(self.lib.g16hip_ceremony_last_runs(self.h, ctypes.byref(r)))
+        return r.value
+
+    def stats(self) -> dict:
+        st = CeremonyStats()
+        _check(self.lib.g16hip_ceremony_stats_get(self.h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.g16hip_ceremony_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 def set_graph_replay(on: bool) -> None:

@@ -104,3 +104,38 @@ def test_gather_bit_shortcuts_match_multiplication(orc):
             want = [sum(vals[vid[k]] * v for k in range(rp[c], rp[c + 1])) % R_MOD
                     for c in range(key.domain_size)]
             assert fr_ints(orc.gather(key, m, wit)) == want
+
+
+# ---- the rows path and gnark's H basis (crates/groth16/src/cpu.rs:250-286, :387-470) ----
+
+@pytest.mark.parametrize("nc", [1, 7, 300])
+def test_rows_path_gives_the_gather_paths_proof_in_both_h_bases(orc, nc):
+    from tests import rowsref
+    from snarkrs_amd.verify import VerifyingKey, verify
+    syn = Synth(orc, nc, n_public=2, n_private_in=5, bool_permille=300, input_bool_permille=300,
+                seed=40 + nc, setup_seed=41)
+    key, wit = syn.key()
+    msm, proof = orc.prove(key, wit, R_BLIND, S_BLIND)
+    a, b = rowsref.rows(orc, key, wit)
+    assert len(a) < key.domain_size or nc == 1
+    # snarkjs' basis from rows (trimmed or full) = the gather path, bit for bit.
+    for ra, rb in ((a, b), rowsref.rows(orc, key, wit, trim=False)):
+        m2, p2 = orc.prove_rows(key, wit, ra, rb, R_BLIND, S_BLIND)
+        assert (m2 == msm).all() and (p2 == proof).all()
+    # gnark's basis: H's coefficients against [tau^bitrev(j) t(tau)/delta]_1 put the same
+    # [H(tau) t(tau)/delta]_1 into the proof, so every output is the same point.
+    gk = rowsref.gnark_key(key, syn.arrays()["toxic"], orc.fixed_base)
+    assert len(gk.h_query) == key.domain_size - 1 and gk.row_ptr is None
+    m3, p3 = orc.prove_rows(gk, wit, a, b, R_BLIND, S_BLIND)
+    assert (m3 == msm).all() and (p3 == proof).all()
+    verify(VerifyingKey.from_proving_key(gk), np.asarray(wit).reshape(-1, 32)[1:3], p3)
+
+
+def test_rows_path_refusals(orc):
+    from tests import rowsref
+    syn = Synth(orc, 20, n_public=1, n_private_in=3, seed=9, setup_seed=10)
+    key, wit = syn.key()
+    a, b = rowsref.rows(orc, key, wit, trim=False)
+    extra = np.zeros((1, 32), np.uint8)
+    with pytest.raises(ValueError, match="-2"):
+        orc.prove_rows(key, wit, np.concatenate([a, extra]), np.concatenate([b, extra]), 1, 2)
