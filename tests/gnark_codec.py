@@ -167,3 +167,100 @@ def read_public(data: bytes) -> np.ndarray:
     vals = [int.from_bytes(data[i:i + 32], "big") for i in range(0, len(data), 32)]
     assert all(v < R_MOD for v in vals)
     return fr_array(vals)
+
+
+class _Reader:
+    def __init__(self, data: bytes):
+        self.d, self.p = data, 0
+
+    def take(self, n: int) -> bytes:
+        assert self.p + n <= len(self.d), "truncated"
+        b = self.d[self.p:self.p + n]
+        self.p += n
+        return b
+
+    def u64_le(self) -> int:
+        return int.from_bytes(self.take(8), "little")
+
+    def u64_be(self) -> int:
+        return int.from_bytes(self.take(8), "big")
+
+    def u32_be(self) -> int:
+        return int.from_bytes(self.take(4), "big")
+
+    def g1(self):
+        p, n = g1_from_gnark(self.d[self.p:])
+        self.p += n
+        return g1_array([p])[0]
+
+    def g2(self):
+        p, n = g2_from_gnark(self.d[self.p:])
+        self.p += n
+        return g2_array([p])[0]
+
+    def dumped(self, stride: int) -> np.ndarray:
+        """unsafe.WriteSlice: u64 LE length, then the elements' memory (Montgomery limbs,
+        little-endian, G2 coordinates A0 before A1: the boundary's own encoding)."""
+        n = self.u64_le()
+        return np.frombuffer(self.take(n * stride), np.uint8).reshape(n, stride).copy()
+
+
+MARKER = (0xDEADBEEF).to_bytes(8, "little")
+
+
+def read_pk_dump(data: bytes) -> dict:
+    """pk.dump (ProvingKey.WriteDump, crates/gnark/src/pk.rs:1-20) -> header values and the
+    queries re-expanded to one base per wire (infinity where gnark leaves a wire out)."""
+    r = _Reader(data)
+    assert r.take(8) == MARKER
+    card = r.u64_be()
+    dom = [int.from_bytes(r.take(32), "big") for _ in range(5)]
+    assert r.take(1)[0] in (0, 1)
+    out = {"domain_size": card, "card_inv": dom[0], "generator": dom[1],
+           "generator_inv": dom[2]}
+    for name in ("alpha_g1", "beta_g1", "delta_g1"):
+        out[name] = r.g1()
+    for name in ("beta_g2", "delta_g2"):
+        out[name] = r.g2()
+    n_wires, inf_a, inf_b = r.u64_be(), r.u64_be(), r.u64_be()
+    flags_a = np.frombuffer(r.take(n_wires), np.uint8)
+    flags_b = np.frombuffer(r.take(n_wires), np.uint8)
+    assert int(flags_a.sum()) == inf_a and int(flags_b.sum()) == inf_b
+    assert r.u32_be() == 0  # no commitments
+    g1_a, g1_b, g1_z, g1_k = r.dumped(64), r.dumped(64), r.dumped(64), r.dumped(64)
+    g2_b = r.dumped(128)
+    assert r.p == len(data)
+
+    def expand(pts, flags, width):
+        full = np.zeros((n_wires, width), np.uint8)
+        full[flags == 0] = pts
+        return full
+
+    out.update(n_wires=n_wires, a_query=expand(g1_a, flags_a, 64),
+               b_g1_query=expand(g1_b, flags_b, 64), b_g2_query=expand(g2_b, flags_b, 128),
+               z=g1_z, k=g1_k)
+    assert len(g1_z) == card - 1
+    return out
+
+
+def read_witness_dump(data: bytes):
+    """witness.bin (crates/gnark/src/witness.rs:1-12): W (every wire, W[0] = 1), A.w, B.w."""
+    r = _Reader(data)
+    assert r.take(8) == MARKER
+    w, aw, bw = r.dumped(32), r.dumped(32), r.dumped(32)
+    assert r.p == len(data)
+    return w, aw, bw
+
+
+def toy_proving_key(pk: dict, vk: dict):
+    """The toy's gnark key as a g16hip proving key (gnark's H basis, no matrices)."""
+    from snarkrs_amd.key import H_BASIS_GNARK, ProvingKey
+    n_pub = len(vk["ic"]) - 1
+    return ProvingKey(
+        n_vars=pk["n_wires"], n_public=n_pub, domain_size=pk["domain_size"],
+        alpha_g1=pk["alpha_g1"], beta_g1=pk["beta_g1"], beta_g2=pk["beta_g2"],
+        gamma_g2=vk["gamma_g2"], delta_g1=pk["delta_g1"], delta_g2=pk["delta_g2"],
+        ic=np.ascontiguousarray(vk["ic"]), row_ptr=None, signal=None, value_id=None,
+        values=None, a_query=pk["a_query"], b_g1_query=pk["b_g1_query"],
+        b_g2_query=pk["b_g2_query"], l_query=np.ascontiguousarray(pk["k"]),
+        h_query=np.ascontiguousarray(pk["z"]), h_basis=H_BASIS_GNARK)

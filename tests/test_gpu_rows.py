@@ -75,3 +75,20 @@ def test_rows_on_a_gpu_built_2_16_key(dev, orc):
     a, b = rowsref.rows(orc, key, wit)
     gk = rowsref.gnark_key(key, toxic, dev.fixed_base)
     assert (dev.prepare(gk).prove_from_rows(wit, a, b, r, s) == proof).all()
+
+
+def test_the_gnark_toy_key_proves_on_the_gpu(dev, orc):
+    # A key from gnark's setup, a witness and rows from gnark's solver (test-support/gnark/toy):
+    # the GPU proof equals the oracle's and verifies under gnark's own verifying key.
+    from snarkrs_amd.verify import VerifyingKey, verify
+    from tests.test_verify import _toy, _toy_key
+    vk, _, pub, _ = _toy()
+    _, _, key, w, aw, bw = _toy_key()
+    msm_o, proof_o = orc.prove_rows(key, w, aw, bw, R_BLIND, S_BLIND)
+    circ = dev.prepare(key)
+    r, s = fr_array([R_BLIND]), fr_array([S_BLIND])
+    assert (circ.h_and_msms_from_rows(w, aw, bw) == msm_o).all()
+    proof = circ.prove_from_rows(w, aw, bw, r, s)
+    assert (proof == proof_o).all()
+    verify(vk, pub, proof)
+    assert (VerifyingKey.from_bytes(circ.vk_bytes()).to_bytes() == vk.to_bytes()).all()

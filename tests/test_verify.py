@@ -345,3 +345,39 @@ def test_verification_key_json(orc):
         VerifyingKey.from_json_str(_vk_json(unblinded))
     assert (VerifyingKey.from_json_str(_vk_json(unblinded), check_structure=False).to_bytes()
             == unblinded.to_bytes()).all()
+
+
+# ---- the gnark toy key proved through the rows path (crates/gnark/tests/prove.rs) --------
+
+def _toy_key():
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    rd = lambda f: open(os.path.join(root, TOY, f), "rb").read()
+    pk = gc.read_pk_dump(rd("pk.dump"))
+    vk = gc.read_vk(rd("vk.bin"))
+    w, aw, bw = gc.read_witness_dump(rd("witness.bin"))
+    return pk, vk, gc.toy_proving_key(pk, vk), w, aw, bw
+
+
+def test_gnark_toy_dump_matches_its_verifying_key_and_domain():
+    pk, vk, key, w, aw, bw = _toy_key()
+    n = pk["domain_size"]
+    log_n = n.bit_length() - 1
+    # gnark's domain generator is the one every prover here uses for 2^12.
+    assert pk["generator"] == pow(pow(5, (R_MOD - 1) >> 28, R_MOD), 1 << (28 - log_n), R_MOD)
+    assert pk["card_inv"] == pow(n, -1, R_MOD)
+    for k in ("alpha_g1", "beta_g1", "delta_g1", "beta_g2", "delta_g2"):
+        assert (pk[k] == vk[k]).all(), k
+    assert key.n_vars == len(w) == 3072 and key.n_public == 4 and len(aw) == len(bw) == 2071
+    assert fr_ints(w[:1]) == [1]
+    _, _, pub, _ = _toy()
+    assert (w[1:5] == pub).all()
+
+
+def test_the_oracle_proves_the_gnark_toy_key_from_rows(orc):
+    vk, _, pub, _ = _toy()
+    _, _, key, w, aw, bw = _toy_key()
+    _, proof = orc.prove_rows(key, w, aw, bw, 0x1234567, 0x7654321)
+    verify(vk, pub, proof)
+    with pytest.raises(native.PairingFailed):
+        verify(vk, fr_array([6] + fr_ints(pub)[1:]), proof)
