@@ -335,6 +335,8 @@ def main() -> int:
     ap.add_argument("--loads", default="0.8,1.25",
                     help="arrival rates as multiples of the plain path's capacity (serve)")
     ap.add_argument("--reserve-gib", type=int, default=16)
+    ap.add_argument("--constant-work", action="store_true",
+                    help="constant-work MSMs (every digit an entry), opt in")
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="GPU_MAX_HW_QUEUES for this process (batch, serve; 0 = leave as set)")
     args = ap.parse_args()
@@ -357,6 +359,8 @@ def main() -> int:
     from snarkrs_amd.wire import fr_array
 
     dev = Device(0)
+    if args.constant_work:
+        dev.set_constant_work(True)
     wl = LARGE_1_1M if args.workload == "large" else PLAIN_2_16
     t = time.time()
     syn = Synth(wl)
@@ -480,6 +484,7 @@ def main() -> int:
             "constraints": wl.n_constraints,
             "domain": key.domain_size,
             "n_vars": key.n_vars,
+            "msm_work": "constant" if args.constant_work else "variable",
         },
         "latency_ms": {"p50": percentile(lat, 50) * 1e3, "p99": percentile(lat, 99) * 1e3,
                        "min": min(lat) * 1e3, "max": max(lat) * 1e3},

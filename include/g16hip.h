@@ -321,6 +321,16 @@ g16hip_status g16hip_msm_g1_device(g16hip_device* dev, const void* d_bases, cons
 g16hip_status g16hip_msm_g2_device(g16hip_device* dev, const void* d_bases, const void* d_scalars,
                                    size_t n, int c, uint8_t* out);
 
+/* Work::Constant (crates/cuda/src/msm.rs:143-164, CudaBackend::with_constant_work,
+ * crates/cuda/src/backend.rs:100-118), opt in: for standalone MSMs on dev and circuits
+ * prepared on dev after the call, every (scalar, window) emits an entry, zero digits into
+ * their window's buckets (i mod 2^(c-1)) as entries whose addition is computed and discarded,
+ * ones as ordinary digits, so the entry count, the sort and the accumulation grid are
+ * n * windows whatever the scalars. Results are bit-identical. Not constant time: bucket
+ * sizes, the merge of buckets that span segments and the point formulas' special cases still
+ * follow the scalars. Off by default. */
+g16hip_status g16hip_set_constant_work(g16hip_device* dev, int on);
+
 /* Phase timings of the last standalone MSM on `dev` (HIP events on the stream its kernels
  * ran on), the per-stage split the reference's StageTimings gives for a proof
  * (crates/groth16/src/lib.rs:322-368) at kernel granularity. */

@@ -72,6 +72,11 @@ struct MsmPlan {
   // Merged windows: the bases come as window tables (entry w n + i = 2^(c w) P_i), so the
   // digits of every window fall into one set of nb buckets and the window sum is the MSM.
   bool merged = false;
+  // Constant work (opt in): every (scalar, window) emits an entry, so the entry count is
+  // n * windows whatever the scalars; a zero digit goes to bucket (i mod nb) flagged as
+  // discarded, and the accumulation computes its addition and keeps the old sum by a select.
+  // Entry values are then index << 2 | discard << 1 | sign (index < 2^30).
+  bool constant = false;
   // device scratch (owned by the caller)
   uint32_t* count = nullptr;    // [windows*nb + 1] entries per bucket
   uint32_t* offset = nullptr;   // [windows*nb + 1] first entry of each bucket; [K] = total
@@ -86,7 +91,8 @@ struct MsmPlan {
 // Bucket sets the accumulation and reduction work on: one per window, or one when merged.
 inline int msm_bucket_sets(const MsmPlan& p) { return p.merged ? 1 : p.windows; }
 size_t msm_sort_scratch_bytes(uint32_t n, int c);
-void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c, bool merged = false);
+void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c, bool merged = false,
+                   bool constant = false);
 // Window tables for merged MSMs: `table` holds n affine bases (MSM form) at [0, n); fills
 // entries w n + i = 2^(c w) P_i for 1 <= w < windows. group 1 = G1, 2 = G2.
 void msm_window_tables(int group, void* table, size_t n, int c, int windows, hipStream_t s);

@@ -53,8 +53,10 @@ __device__ __forceinline__ int signed_digit(const uint32_t s[8], int i, int c) {
   return d + carry;
 }
 
-// Digit encoding: (|d| - 1) | sign << 16, or DIGIT_ZERO for d = 0.
+// Digit encoding: (|d| - 1) | sign << 16, or DIGIT_ZERO for d = 0; under constant work a zero
+// digit is (i mod nb) | DIGIT_DISCARD instead, an entry whose addition is computed and dropped.
 static constexpr uint32_t DIGIT_ZERO = 1u << 17;
+static constexpr uint32_t DIGIT_DISCARD = 1u << 18;
 // Scalars per counting-sort workgroup: a bucket's count within a chunk fits 16 bits, so the
 // LDS histogram packs two buckets per word (64 KB at c = 16).
 static constexpr uint32_t SORT_CHUNK = 32768;
@@ -62,13 +64,14 @@ static constexpr int SORT_THREADS = 1024;
 static constexpr uint32_t SCATTER_SPAN = 16384;  // buckets per scatter workgroup (64 KB LDS)
 
 __global__ void __launch_bounds__(256) k_digits(const uint4* scalars, uint32_t n, int c, int W,
-                                                uint32_t* digits) {
+                                                uint32_t* digits, int constant) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   Fr s = to_std(ld_fr(scalars + 2 * (size_t)i));
+  const uint32_t zero = constant ? (i & ((1u << (c - 1)) - 1u)) | DIGIT_DISCARD : DIGIT_ZERO;
   for (int w = 0; w < W; ++w) {
     int d = signed_digit(s.v, w, c);
-    uint32_t e = d == 0 ? DIGIT_ZERO
+    uint32_t e = d == 0 ? zero
                         : ((uint32_t)(d < 0 ? -d : d) - 1u) | (d < 0 ? 1u << 16 : 0u);
     digits[(size_t)w * n + i] = e;
   }
@@ -151,7 +154,8 @@ __global__ void __launch_bounds__(256) k_window_scan(uint32_t* count, uint32_t n
 // window table (w n + i).
 __global__ void __launch_bounds__(SORT_THREADS) k_chunk_scatter(
     const uint32_t* digits, uint32_t n, uint32_t nb, uint32_t chunks, uint32_t span,
-    const uint32_t* chunk_count, const uint32_t* offset, const uint32_t* wpre, uint2* ent) {
+    const uint32_t* chunk_count, const uint32_t* offset, const uint32_t* wpre, uint2* ent,
+    int constant) {
   extern __shared__ uint32_t cur[];  // span words
   uint32_t p = blockIdx.x, w = blockIdx.y, b0 = blockIdx.z * span;
   const uint32_t* cc = chunk_count + ((size_t)w * chunks + p) * nb + b0;
@@ -171,7 +175,9 @@ __global__ void __launch_bounds__(SORT_THREADS) k_chunk_scatter(
     uint32_t pos = atomicAdd(&cur[b], 1u);
     const uint32_t key = (wpre ? 0u : w * nb) + b0 + b;
     const uint32_t idx = wpre ? w * n + (uint32_t)i : (uint32_t)i;
-    ent[pos] = make_uint2(key, (idx << 1) | ((e >> 16) & 1u));
+    const uint32_t sign = (e >> 16) & 1u;
+    ent[pos] = make_uint2(key, constant ? (idx << 2) | ((e & DIGIT_DISCARD) ? 2u : 0u) | sign
+                                        : (idx << 1) | sign);
   }
 }
 
@@ -266,9 +272,10 @@ size_t msm_sort_scratch_bytes(uint32_t n, int c) {
          align_up(4 * K * sort_chunks(n)) + align_up(4 * (nb + 1)) + align_up(16);
 }
 
-void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c, bool merged) {
+void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c, bool merged, bool constant) {
   if (c < 2 || c > 16) throw ArgError{"MSM window width must be in [2, 16]"};
   p.merged = merged;
+  p.constant = constant;
   p.c = c;
   p.windows = (RECODE_BITS + c - 1) / c;
   p.n = n;
@@ -280,6 +287,8 @@ void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c, bool merged) {
   if ((K + 1023) / 1024 > 1024) throw ArgError{"MSM window too wide for the scan"};
   if ((size_t)n >= (1u << 31)) throw ArgError{"MSM larger than 2^31 points"};
   if (p.max_entries >= (1ull << 31)) throw ArgError{"MSM entry count exceeds 2^31"};
+  if (constant && (merged ? p.max_entries : (size_t)n) >= (1ull << 30))
+    throw ArgError{"constant-work MSM: base indices must stay below 2^30"};
   char* q = static_cast<char*>(scratch);
   auto take = [&](size_t bytes) {
     char* r = q;
@@ -300,7 +309,8 @@ void msm_sort(MsmPlan& p, const void* scalars, hipStream_t s) {
   uint32_t K = (uint32_t)p.windows * p.nb;
   if (p.n) {
     hipLaunchKernelGGL(k_digits, dim3((p.n + 255) / 256), dim3(256), 0, s,
-                       static_cast<const uint4*>(scalars), p.n, p.c, p.windows, p.digits);
+                       static_cast<const uint4*>(scalars), p.n, p.c, p.windows, p.digits,
+                       (int)p.constant);
     G16_LAUNCH_CHECK("msm digits");
     hipLaunchKernelGGL(k_chunk_count, dim3(p.chunks, p.windows), dim3(SORT_THREADS),
                        4 * (p.nb / 2), s, p.digits, p.n, p.nb, p.chunks, p.chunk_count);
@@ -332,7 +342,8 @@ void msm_sort(MsmPlan& p, const void* scalars, hipStream_t s) {
     uint32_t span = p.nb < SCATTER_SPAN ? p.nb : SCATTER_SPAN;
     hipLaunchKernelGGL(k_chunk_scatter, dim3(p.chunks, p.windows, p.nb / span),
                        dim3(SORT_THREADS), 4 * span, s, p.digits, p.n, p.nb, p.chunks, span,
-                       p.chunk_count, p.offset, p.merged ? p.count : nullptr, p.ent);
+                       p.chunk_count, p.offset, p.merged ? p.count : nullptr, p.ent,
+                       (int)p.constant);
     G16_LAUNCH_CHECK("msm chunk scatter");
   }
 }

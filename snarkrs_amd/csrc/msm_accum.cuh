@@ -118,7 +118,9 @@ __global__ void __launch_bounds__(128) k_window_tables(uint4* table, size_t n, i
 // ---------------------------------------------------------------------------------------
 // bucket accumulation
 // ---------------------------------------------------------------------------------------
-template <class F>
+// CONST: constant-work entries (index << 2 | discard << 1 | sign): the addition of a
+// discarded entry is computed and its result dropped by a select.
+template <class F, bool CONST>
 __global__ void __launch_bounds__(128) k_accumulate(const uint2* __restrict__ ent,
                                                     const uint32_t* total, size_t nseg,
                                                     const uint4* __restrict__ bases,
@@ -137,8 +139,9 @@ __global__ void __launch_bounds__(128) k_accumulate(const uint2* __restrict__ en
   // G1: the next entry's point is loaded before this one is added, so the load latency of a
   // random base hides behind a mixed addition (the G2 loop has no registers to spare).
   constexpr bool PREFETCH = sizeof(F) == 32;
+  constexpr int SH = CONST ? 2 : 1;
   uint32_t v = ent[b].y;
-  Aff<F> a = ld_aff<F>(bases, v >> 1);  // merged: v >> 1 = w n + i, a window-table entry
+  Aff<F> a = ld_aff<F>(bases, v >> SH);  // merged: v >> SH = w n + i, a window-table entry
   for (size_t k = b; k < e; ++k) {
     uint32_t key = ent[k].x;
     if (key != cur) {
@@ -152,15 +155,20 @@ __global__ void __launch_bounds__(128) k_accumulate(const uint2* __restrict__ en
     }
     if (!PREFETCH && k != b) {
       v = ent[k].y;
-      a = ld_aff<F>(bases, v >> 1);
+      a = ld_aff<F>(bases, v >> SH);
     }
     uint32_t v_next = 0;
     Aff<F> a_next;
     if (PREFETCH && k + 1 < e) {
       v_next = ent[k + 1].y;
-      a_next = ld_aff<F>(bases, v_next >> 1);
+      a_next = ld_aff<F>(bases, v_next >> SH);
     }
-    acc = xyzz_add_aff(acc, (v & 1) ? aff_neg(a) : a);
+    if (CONST) {
+      Xyzz<F> sum = xyzz_add_aff(acc, (v & 1) ? aff_neg(a) : a);
+      acc = (v & 2) ? acc : sum;
+    } else {
+      acc = xyzz_add_aff(acc, (v & 1) ? aff_neg(a) : a);
+    }
     if (PREFETCH) {
       v = v_next;
       a = a_next;
@@ -506,8 +514,12 @@ inline void accumulate_impl(const MsmPlan& p, const void* bases, void* scratch, 
 
   if (ev) G16_CHECK(hipEventRecord(ev[0], s));
   if (nseg) {
-    hipLaunchKernelGGL(k_accumulate<F>, dim3((unsigned)((nseg + 127) / 128)), dim3(128), 0, s,
-                       p.ent, p.total, nseg, bp, a.buckets, a.pfirst, a.plast);
+    if (p.constant)
+      hipLaunchKernelGGL((k_accumulate<F, true>), dim3((unsigned)((nseg + 127) / 128)), dim3(128),
+                         0, s, p.ent, p.total, nseg, bp, a.buckets, a.pfirst, a.plast);
+    else
+      hipLaunchKernelGGL((k_accumulate<F, false>), dim3((unsigned)((nseg + 127) / 128)), dim3(128),
+                         0, s, p.ent, p.total, nseg, bp, a.buckets, a.pfirst, a.plast);
     G16_LAUNCH_CHECK("msm accumulate");
   }
   if (ev) G16_CHECK(hipEventRecord(ev[1], s));

@@ -75,6 +75,7 @@ struct g16hip_circuit {
   bool has_matrices = true;  // false: proves from A.w and B.w rows only
   uint32_t h_basis = 0;      // 0 snarkjs' odd coset, 1 gnark's bit-reversed coefficients
   uint32_t n_h = 0;          // H bases: domain_size, or domain_size - 1 for gnark's
+  bool constant = false;     // constant-work MSMs (the device's setting at prepare)
   VkBytes vk;  // the verifying key, for the check of a proof's result
   DevBuf row_ptr[2], signal[2], value_id[2], values;
   DevBuf a_q, b1_q, b2_q, l_q, h_q;
@@ -254,8 +255,8 @@ static void size_slot(const g16hip_circuit* C, Slot* S) {
   S->hbuf.alloc(32 * n);
   S->sort_w.alloc(msm_sort_scratch_bytes((uint32_t)nv, C->cw));
   S->sort_h.alloc(msm_sort_scratch_bytes(C->n_h, C->ch));
-  msm_plan_bind(S->plan_w, S->sort_w.p, (uint32_t)nv, C->cw, C->merged);
-  msm_plan_bind(S->plan_h, S->sort_h.p, C->n_h, C->ch, C->merged);
+  msm_plan_bind(S->plan_w, S->sort_w.p, (uint32_t)nv, C->cw, C->merged, C->constant);
+  msm_plan_bind(S->plan_h, S->sort_h.p, C->n_h, C->ch, C->merged, C->constant);
   S->accum_w1.alloc(msm_accum_scratch_bytes(S->plan_w, 1));
   S->accum_w2.alloc(msm_accum_scratch_bytes(S->plan_w, 2));
   S->accum_h.alloc(msm_accum_scratch_bytes(S->plan_h, 1));
@@ -813,6 +814,10 @@ g16hip_status g16hip_prepare(g16hip_device* d, const g16hip_key* key, g16hip_cir
     set_device(d);
     auto C = std::make_unique<g16hip_circuit>();
     C->dev = d;
+    {
+      std::lock_guard<std::mutex> lk(d->mu);
+      C->constant = d->constant_work;
+    }
     // The key goes up on slot 0's main stream (a failure destroys the slot, which waits for
     // its streams first).
     auto S = new_slot();

@@ -76,7 +76,7 @@ static Xyzz<F> run_msm(g16hip_device* d, int group, const void* bases, const voi
   if (c < 2 || c > 16) throw ArgError{"window width " + std::to_string(c) + " is outside [2, 16]"};
   MsmPlan plan;
   d->scratch_d.alloc(msm_sort_scratch_bytes(n, c));
-  msm_plan_bind(plan, d->scratch_d.p, n, c);
+  msm_plan_bind(plan, d->scratch_d.p, n, c, false, d->constant_work);
   const size_t npts = msm_out_points(plan);
   d->scratch_e.alloc(msm_accum_scratch_bytes(plan, group) + sizeof(Xyzz<F>) * npts);
   const size_t coords = (size_t)n * (group == 1 ? 2 : 4);
@@ -272,6 +272,15 @@ g16hip_status g16hip_ceremony_msm(g16hip_ceremony* c, int group, const uint8_t* 
     }
     if (group == 1) g1_store(out, xyzz_to_aff(ceremony_msm<Fq>(c, 1, bases, scalars, n)));
     else g2_store(out, xyzz_to_aff(ceremony_msm<Fq2>(c, 2, bases, scalars, n)));
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_set_constant_work(g16hip_device* d, int on) {
+  return guarded([&] {
+    if (!d) throw ArgError{"null device"};
+    std::lock_guard<std::mutex> lk(d->mu);
+    d->constant_work = on != 0;
     return G16HIP_OK;
   });
 }

@@ -226,6 +226,7 @@ struct g16hip_device {
   g16hip_msm_stats stats = {};
   std::mutex stats_mu;
   g16hip_kernel_stats kstats = {};
+  bool constant_work = false;  // MsmPlan::constant for standalone MSMs and new circuits
 };
 
 namespace g16 {

@@ -247,6 +247,7 @@ EXPORTS = {
                                             ctypes.POINTER(_vp)]),
     "g16hip_ceremony_close": (None, [_vp]),
     "g16hip_ceremony_msm": (ctypes.c_int, [_vp, ctypes.c_int, _u8p, _u8p, ctypes.c_size_t, _u8p]),
+    "g16hip_set_constant_work": (ctypes.c_int, [_vp, ctypes.c_int]),
     "g16hip_msm_scratch_bytes": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t,
                                                 ctypes.POINTER(ctypes.c_size_t)]),
     "g16hip_ceremony_last_runs": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
@@ -375,6 +376,10 @@ class Device:
         fn = self.lib.g16hip_msm_g1_device if group == 1 else self.lib.g16hip_msm_g2_device
         _check(fn(self.h, _vp(d_bases), _vp(d_scalars), n, c, _p8(out)))
         return out
+
+    def set_constant_work(self, on: bool) -> None:
+        """Work::Constant for standalone MSMs and circuits prepared after the call."""
+        _check(self.lib.g16hip_set_constant_work(self.h, int(on)))
 
     def msm_stats(self) -> dict:
         st = MsmStats()
