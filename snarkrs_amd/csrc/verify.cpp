@@ -55,6 +55,11 @@ struct Big {
     for (int i = 0; i < n; ++i) b.v[i] = limbs[i];
     return b;
   }
+  static Big small(uint64_t x) {
+    Big b;
+    b.v[0] = x;
+    return b;
+  }
   int bits() const {
     for (int i = L - 1; i >= 0; --i)
       if (v[i]) return 64 * i + 64 - __builtin_clzll(v[i]);
@@ -341,7 +346,7 @@ Fp<T> f_pow(const Fp<T>& a, const Big& e) {
 
 template <class T>
 Fp<T> f_inv(const Fp<T>& a) {  // a^(p - 2); 0 maps to 0
-  Big e = big_sub(Big::of(T::P, 4), Big::of((const uint64_t[]){2}, 1));
+  Big e = big_sub(Big::of(T::P, 4), Big::small(2));
   return f_pow(a, e);
 }
 
@@ -548,11 +553,11 @@ struct Tower {
     Fq2 xi{f_from_u64<QTag>(9), f_from_u64<QTag>(1)};
     b2 = e2_mul(Fq2{b1, f_zero<QTag>()}, e2_inv(xi));
     Big qj = q;
-    Big one = Big::of((const uint64_t[]){1}, 1);
+    Big one = Big::small(1);
     for (int j = 1; j <= 3; ++j) {
       Big e = big_sub(qj, one);
       Big rem;
-      Big e6 = big_div(e, Big::of((const uint64_t[]){6}, 1), &rem);
+      Big e6 = big_div(e, Big::small(6), &rem);
       if (!rem.is_zero()) {
         err = "6 does not divide q^j - 1";
         return;
@@ -835,8 +840,13 @@ Fq12 miller_loop_parallel(const std::vector<std::pair<G1A, G2A>>& pairs) {
   if (pairs.size() < 2) return miller_loop(pairs);
   std::vector<Fq12> part(pairs.size());
   std::vector<std::thread> th;
-  for (size_t k = 1; k < pairs.size(); ++k)
-    th.emplace_back([&, k] { part[k] = miller_loop({pairs[k]}); });
+  try {
+    for (size_t k = 1; k < pairs.size(); ++k)
+      th.emplace_back([&, k] { part[k] = miller_loop({pairs[k]}); });
+  } catch (...) {  // no thread left running if one cannot be started
+    for (auto& t : th) t.join();
+    throw;
+  }
   part[0] = miller_loop({pairs[0]});
   for (auto& t : th) t.join();
   Fq12 f = part[0];
@@ -1272,7 +1282,7 @@ Fp<T> parse_field(const JVal& v, const std::string& what) {
   Big acc;
   for (; i < s.size(); ++i) {
     if (s[i] < '0' || s[i] > '9') jfail(what + ": \"" + s + "\" is not a decimal integer");
-    acc = big_add_small(big_mul(acc, Big::of((const uint64_t[]){10}, 1)), (uint64_t)(s[i] - '0'));
+    acc = big_add_small(big_mul(acc, Big::small(10)), (uint64_t)(s[i] - '0'));
     if (acc.bits() > 256) jfail(what + ": " + s + " is not less than the field modulus");
   }
   Fp<T> x{{acc.v[0], acc.v[1], acc.v[2], acc.v[3]}};
