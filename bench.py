@@ -51,6 +51,12 @@ def log(msg: str) -> None:
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
+def reference_window(m: int) -> int:
+    """The reference's window model (crates/msm/src/lib.rs:160-172): the c in [3, 16]
+    minimising ceil(255 / c) * (m + 3 * 2^(c - 1)) on the general scalar count m."""
+    return min(range(3, 17), key=lambda c: (-(-255 // c)) * (m + 3 * (1 << (c - 1))))
+
+
 def percentile(xs, q):
     return float(np.percentile(np.asarray(xs), q))
 
@@ -356,7 +362,7 @@ def main() -> int:
     from snarkrs_amd.devmem import DeviceArray, synchronize
     from snarkrs_amd.native import Device, Timings
     from snarkrs_amd.synth import LARGE_1_1M, PLAIN_2_16, Synth, scalars
-    from snarkrs_amd.wire import fr_array
+    from snarkrs_amd.wire import R_MOD, fr_array
 
     dev = Device(0)
     if args.constant_work:
@@ -452,6 +458,46 @@ def main() -> int:
                          for k in ("sort_ms", "accumulate_ms", "merge_ms", "reduce_ms", "total_ms")},
         }
         log(f"MSM G1 2^{args.msm_log_n}: {med * 1e3:.2f} ms -> {n / med / 1e6:.1f} M points/s")
+        # The sort's achieved HBM rate, on the algorithmic bytes of the reference model:
+        # 32 n + 16 m W + 12 W 2^(c*-1) with c* the reference window on the m general scalars.
+        one = (pow(2, 256, R_MOD)).to_bytes(32, "little")
+        rows = sc.reshape(-1, 32)
+        zero_or_one = int((~rows.any(axis=1)).sum() +
+                          (rows == np.frombuffer(one, np.uint8)).all(axis=1).sum())
+        m = n - zero_or_one
+        c_ref = reference_window(m)
+        w_ref = -(-255 // c_ref)
+        sort_bytes = 32 * n + 16 * m * w_ref + 12 * w_ref * (1 << (c_ref - 1))
+        sort_ms = msm["phase_ms"]["sort_ms"]
+        msm["sort_hbm"] = {"algorithmic_bytes": sort_bytes, "general_scalars": m,
+                           "reference_c": c_ref, "ms": sort_ms,
+                           "GB_per_s": sort_bytes / (sort_ms / 1e3) / 1e9,
+                           "frac_of_8TB_per_s": sort_bytes / (sort_ms / 1e3) / 8e12}
+        log(f"sort: {sort_bytes / 1e6:.0f} MB algorithmic in {sort_ms:.3f} ms -> "
+            f"{msm['sort_hbm']['GB_per_s']:.0f} GB/s")
+
+    ntt = None
+    if not args.no_msm:
+        # NTT over 2^21 resident Fr: one forward transform (64 B per element algorithmic: one
+        # read and one write) and the prover's coset pipeline (two transforms), timed around the
+        # synchronised library call.
+        from snarkrs_amd.devmem import synchronize as _sync
+        log_n = 21
+        d_x = DeviceArray(scalars(77, 1 << log_n, 0))
+        ntt = {}
+        for kind, name, passes in ((0, "forward", 1), (2, "coset_pipeline", 2)):
+            dev.ntt_device(d_x.addr, log_n, kind)
+            ts = []
+            for _ in range(10):
+                _sync()
+                t0 = time.perf_counter()
+                dev.ntt_device(d_x.addr, log_n, kind)
+                ts.append(time.perf_counter() - t0)
+            t = float(np.median(ts))
+            b = 64 * (1 << log_n) * passes
+            ntt[name] = {"log_n": log_n, "ms": t * 1e3, "algorithmic_bytes": b,
+                         "GB_per_s": b / t / 1e9, "frac_of_8TB_per_s": b / t / 8e12}
+            log(f"NTT 2^{log_n} {name}: {t * 1e3:.3f} ms -> {b / t / 1e9:.0f} GB/s algorithmic")
 
     cpu = None
     if not args.no_cpu:
@@ -492,6 +538,7 @@ def main() -> int:
         "proof_verifies": True,
         "verify_ms": verify_ms,
         "msm_g1": msm,
+        "ntt": ntt,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
