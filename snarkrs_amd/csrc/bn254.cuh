@@ -125,17 +125,29 @@ HD bool is_one(const Fp<C>& a) {
   return x == 0;
 }
 
+// 32-bit add and subtract with carry: one v_add_co / v_addc_co (v_sub_co / v_subb_co) per
+// limb on the device. The same chains written with 64-bit intermediates compile to 64-bit
+// shifts and register moves, about three times the instructions.
+HD uint32_t addc32(uint32_t x, uint32_t y, uint32_t& carry) {
+  unsigned c;
+  uint32_t r = __builtin_addc(x, y, carry, &c);
+  carry = c;
+  return r;
+}
+HD uint32_t subb32(uint32_t x, uint32_t y, uint32_t& borrow) {
+  unsigned b;
+  uint32_t r = __builtin_subc(x, y, borrow, &b);
+  borrow = b;
+  return r;
+}
+
 // a - p if a >= p, computed as a subtraction with borrow and a select.
 template <class C>
 HD void reduce_once(uint32_t a[8]) {
   uint32_t d[8];
   uint32_t borrow = 0;
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    uint64_t t = (uint64_t)a[i] - C::p(i) - borrow;
-    d[i] = (uint32_t)t;
-    borrow = (uint32_t)(t >> 63);
-  }
+  for (int i = 0; i < 8; ++i) d[i] = subb32(a[i], C::p(i), borrow);
 #pragma unroll
   for (int i = 0; i < 8; ++i) a[i] = borrow ? a[i] : d[i];
 }
@@ -145,11 +157,7 @@ HD Fp<C> add(const Fp<C>& a, const Fp<C>& b) {
   Fp<C> r;
   uint32_t carry = 0;
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    uint64_t t = (uint64_t)a.v[i] + b.v[i] + carry;
-    r.v[i] = (uint32_t)t;
-    carry = (uint32_t)(t >> 32);
-  }
+  for (int i = 0; i < 8; ++i) r.v[i] = addc32(a.v[i], b.v[i], carry);
   reduce_once<C>(r.v);  // p < 2^254: the sum never carries out of 256 bits
   return r;
 }
@@ -159,20 +167,12 @@ HD Fp<C> sub(const Fp<C>& a, const Fp<C>& b) {
   Fp<C> r;
   uint32_t borrow = 0;
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    uint64_t t = (uint64_t)a.v[i] - b.v[i] - borrow;
-    r.v[i] = (uint32_t)t;
-    borrow = (uint32_t)(t >> 63);
-  }
+  for (int i = 0; i < 8; ++i) r.v[i] = subb32(a.v[i], b.v[i], borrow);
   // Add p back when the difference went negative: mask the modulus with the borrow.
   uint32_t mask = 0u - borrow;
   uint32_t carry = 0;
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    uint64_t t = (uint64_t)r.v[i] + (C::p(i) & mask) + carry;
-    r.v[i] = (uint32_t)t;
-    carry = (uint32_t)(t >> 32);
-  }
+  for (int i = 0; i < 8; ++i) r.v[i] = addc32(r.v[i], C::p(i) & mask, carry);
   return r;
 }
 
@@ -276,7 +276,8 @@ HD void mont29(const uint32_t a[8], const uint32_t b[8], uint32_t out[8]) {
   for (int i = 0; i < 9; ++i) {
 #pragma unroll
     for (int j = 0; j < 9; ++j) t[j] += (uint64_t)A[j] * B[i];
-    const uint32_t m = (uint32_t)((uint64_t)((uint32_t)t[0] & MASK) * M29::N29) & MASK;
+    // m = t0 * N29 mod 2^29; the low 32 bits of the product suffice, unmasked t0 included.
+    const uint32_t m = ((uint32_t)t[0] * M29::N29) & MASK;
 #pragma unroll
     for (int j = 0; j < 9; ++j) t[j] += (uint64_t)m * M29::p29(j);
     t[1] += t[0] >> 29;  // t[0] is now a multiple of 2^29
@@ -326,17 +327,9 @@ HD Fp<FqmCfg> add<FqmCfg>(const Fp<FqmCfg>& a, const Fp<FqmCfg>& b) {
   Fp<FqmCfg> r, d;
   uint32_t carry = 0, borrow = 0;
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {  // a + b < 4q < 2^256
-    uint64_t t = (uint64_t)a.v[i] + b.v[i] + carry;
-    r.v[i] = (uint32_t)t;
-    carry = (uint32_t)(t >> 32);
-  }
+  for (int i = 0; i < 8; ++i) r.v[i] = addc32(a.v[i], b.v[i], carry);  // a + b < 4q < 2^256
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    uint64_t t = (uint64_t)r.v[i] - fqm_2q(i) - borrow;
-    d.v[i] = (uint32_t)t;
-    borrow = (uint32_t)(t >> 63);
-  }
+  for (int i = 0; i < 8; ++i) d.v[i] = subb32(r.v[i], fqm_2q(i), borrow);
 #pragma unroll
   for (int i = 0; i < 8; ++i) r.v[i] = borrow ? r.v[i] : d.v[i];
   return r;
@@ -347,18 +340,10 @@ HD Fp<FqmCfg> sub<FqmCfg>(const Fp<FqmCfg>& a, const Fp<FqmCfg>& b) {
   Fp<FqmCfg> r;
   uint32_t borrow = 0;
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    uint64_t t = (uint64_t)a.v[i] - b.v[i] - borrow;
-    r.v[i] = (uint32_t)t;
-    borrow = (uint32_t)(t >> 63);
-  }
+  for (int i = 0; i < 8; ++i) r.v[i] = subb32(a.v[i], b.v[i], borrow);
   uint32_t mask = 0u - borrow, carry = 0;  // negative: add 2q
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    uint64_t t = (uint64_t)r.v[i] + (fqm_2q(i) & mask) + carry;
-    r.v[i] = (uint32_t)t;
-    carry = (uint32_t)(t >> 32);
-  }
+  for (int i = 0; i < 8; ++i) r.v[i] = addc32(r.v[i], fqm_2q(i) & mask, carry);
   return r;
 }
 
