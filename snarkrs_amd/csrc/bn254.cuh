@@ -455,16 +455,27 @@ template <bool INL, class B> HD Fq2T<INL, B> neg(const Fq2T<INL, B>& a) {
 template <bool INL, class B> HD Fq2T<INL, B> dbl(const Fq2T<INL, B>& a) { return add(a, a); }
 
 // Karatsuba: 3 base-field products.
+// A sum that only feeds a product: the MSM form's product takes any 256-bit operand, so there
+// a + b (< 4q < 2^256) needs no reduction; the other fields reduce as usual.
+template <class B> HD B add_into_mul(const B& a, const B& b) { return add(a, b); }
+template <> HD Fp<FqmCfg> add_into_mul<Fp<FqmCfg>>(const Fp<FqmCfg>& a, const Fp<FqmCfg>& b) {
+  Fp<FqmCfg> r;
+  uint32_t carry = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) r.v[i] = addc32(a.v[i], b.v[i], carry);
+  return r;
+}
+
 template <bool INL, class B> HD Fq2T<INL, B> mul(const Fq2T<INL, B>& a, const Fq2T<INL, B>& b) {
   B v0 = fq2_part_mul<INL>(a.c0, b.c0);
   B v1 = fq2_part_mul<INL>(a.c1, b.c1);
-  B s = fq2_part_mul<INL>(add(a.c0, a.c1), add(b.c0, b.c1));
+  B s = fq2_part_mul<INL>(add_into_mul(a.c0, a.c1), add_into_mul(b.c0, b.c1));
   return {sub(v0, v1), sub(sub(s, v0), v1)};
 }
 
 // (a0 + a1 u)^2 = (a0 + a1)(a0 - a1) + 2 a0 a1 u: 2 base-field products.
 template <bool INL, class B> HD Fq2T<INL, B> sqr(const Fq2T<INL, B>& a) {
-  B m = fq2_part_mul<INL>(add(a.c0, a.c1), sub(a.c0, a.c1));
+  B m = fq2_part_mul<INL>(add_into_mul(a.c0, a.c1), sub(a.c0, a.c1));
   B t = fq2_part_mul<INL>(a.c0, a.c1);
   return {m, add(t, t)};
 }
