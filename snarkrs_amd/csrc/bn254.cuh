@@ -258,17 +258,33 @@ struct Fr29 {
   static constexpr uint32_t N29 = 0x0fffffffu;
 };
 
-// With FULL = false the result is left in [0, 2 modulus) (any 256-bit inputs): the MSM form
-// keeps its values there (see the FqmCfg operations below).
-template <class M29, bool FULL = true>
-HD void mont29(const uint32_t a[8], const uint32_t b[8], uint32_t out[8]) {
-  constexpr uint32_t MASK = (1u << 29) - 1u;
-  uint32_t A[9], B[9];
+// The 29-bit limbs of an 8 x 32-bit value, and back (a value below 2^256 in limbs < 2^29).
+HD void limbs29_of(const uint32_t v[8], uint32_t A[9]) {
 #pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    A[k] = limb29(a, k);
-    B[k] = limb29(b, k);
+  for (int k = 0; k < 9; ++k) A[k] = limb29(v, k);
+}
+HD void pack29(const uint32_t R[9], uint32_t out[8]) {
+  uint64_t acc = 0;
+  int bits = 0, w = 0;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    acc |= (uint64_t)R[j] << bits;
+    bits += 29;
+    if (bits >= 32) {
+      if (w < 8) out[w] = (uint32_t)acc;
+      ++w;
+      acc >>= 32;
+      bits -= 32;
+    }
   }
+  if (w < 8) out[w] = (uint32_t)acc;
+}
+
+// The product on limbs: A B 2^-261 mod the modulus, in [0, 2 modulus) (any inputs below
+// 2^256), as nine 29-bit limbs.
+template <class M29>
+HD void mont29_limbs(const uint32_t A[9], const uint32_t B[9], uint32_t R[9]) {
+  constexpr uint32_t MASK = (1u << 29) - 1u;
   uint64_t t[10];
 #pragma unroll
   for (int j = 0; j < 10; ++j) t[j] = 0;
@@ -285,23 +301,25 @@ HD void mont29(const uint32_t a[8], const uint32_t b[8], uint32_t out[8]) {
     for (int j = 0; j < 9; ++j) t[j] = t[j + 1];
     t[9] = 0;
   }
-  // Carry the columns into 29-bit limbs and repack as 8 x 32 bits (< 2 modulus < 2^256).
-  uint64_t c = 0, acc = 0;
-  int bits = 0, w = 0;
+  // Carry the columns into 29-bit limbs (the value is below 2 modulus < 2^255).
+  uint64_t c = 0;
 #pragma unroll
   for (int j = 0; j < 9; ++j) {
     const uint64_t x = t[j] + c;
     c = x >> 29;
-    acc |= (uint64_t)((uint32_t)x & MASK) << bits;
-    bits += 29;
-    if (bits >= 32) {
-      if (w < 8) out[w] = (uint32_t)acc;
-      ++w;
-      acc >>= 32;
-      bits -= 32;
-    }
+    R[j] = (uint32_t)x & MASK;
   }
-  if (w < 8) out[w] = (uint32_t)acc;
+}
+
+// With FULL = false the result is left in [0, 2 modulus) (any 256-bit inputs): the MSM form
+// keeps its values there (see the FqmCfg operations below).
+template <class M29, bool FULL = true>
+HD void mont29(const uint32_t a[8], const uint32_t b[8], uint32_t out[8]) {
+  uint32_t A[9], B[9], R[9];
+  limbs29_of(a, A);
+  limbs29_of(b, B);
+  mont29_limbs<M29>(A, B, R);
+  pack29(R, out);
   if (FULL) reduce_once<typename M29::Cfg>(out);
 }
 

@@ -118,6 +118,100 @@ __global__ void __launch_bounds__(128) k_window_tables(uint4* table, size_t n, i
 // ---------------------------------------------------------------------------------------
 // bucket accumulation
 // ---------------------------------------------------------------------------------------
+// The bucket loop's running sum. Generic: an XYZZ point. G1 (Fm): X and Y as words, ZZ and
+// ZZZ as nine 29-bit limbs. ZZ and ZZZ only ever feed 29-bit products, so keeping them (and
+// PP, PPP inside the addition) in limbs skips the word/limb round trip each product would
+// make; the value and every output are the same.
+template <class F>
+struct Acc {
+  using T = Xyzz<F>;
+  static __device__ __forceinline__ T inf() { return xyzz_inf<F>(); }
+  static __device__ __forceinline__ T madd(const T& p, const Aff<F>& a) { return xyzz_add_aff(p, a); }
+  static __device__ __forceinline__ Xyzz<F> out(const T& p) { return p; }
+};
+
+struct AccFm {
+  Fm X, Y;
+  uint32_t ZZ[9], ZZZ[9];
+};
+
+template <>
+struct Acc<Fm> {
+  using T = AccFm;
+  static __device__ __forceinline__ T from(const Xyzz<Fm>& p) {
+    T r;
+    r.X = p.X;
+    r.Y = p.Y;
+    limbs29_of(p.ZZ.v, r.ZZ);
+    limbs29_of(p.ZZZ.v, r.ZZZ);
+    return r;
+  }
+  static __device__ __forceinline__ T inf() { return from(xyzz_inf<Fm>()); }
+  static __device__ __forceinline__ Xyzz<Fm> out(const T& p) {
+    Xyzz<Fm> r;
+    r.X = p.X;
+    r.Y = p.Y;
+    pack29(p.ZZ, r.ZZ.v);
+    pack29(p.ZZZ, r.ZZZ.v);
+    return r;
+  }
+  // The MSM form's zero is 0 or q.
+  static __device__ __forceinline__ bool zero29(const uint32_t a[9]) {
+    uint32_t z = 0, q = 0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+      z |= a[j];
+      q |= a[j] ^ Fq29::p29(j);
+    }
+    return z == 0 || q == 0;
+  }
+  static __device__ __forceinline__ Fm prod(const uint32_t A[9], const uint32_t B[9]) {
+    uint32_t R[9];
+    mont29_limbs<Fq29>(A, B, R);
+    Fm r;
+    pack29(R, r.v);
+    return r;
+  }
+  // madd-2008-s as xyzz_add_aff, on the mixed form.
+  static __device__ __forceinline__ T madd(const T& p, const Aff<Fm>& a) {
+    if (aff_is_inf(a)) return p;
+    if (zero29(p.ZZ)) return from({a.x, a.y, fp_one<FqmCfg>(), fp_one<FqmCfg>()});
+    uint32_t L0[9], L1[9];
+    limbs29_of(a.x.v, L0);
+    const Fm P = sub(prod(L0, p.ZZ), p.X);
+    SB();
+    limbs29_of(a.y.v, L1);
+    const Fm R = sub(prod(L1, p.ZZZ), p.Y);
+    SB();
+    if (is_zero(P)) return from(is_zero(R) ? xyzz_dbl_aff(a) : xyzz_inf<Fm>());
+    uint32_t P29[9], PP[9], PPP[9];
+    limbs29_of(P.v, P29);
+    mont29_limbs<Fq29>(P29, P29, PP);
+    SB();
+    mont29_limbs<Fq29>(P29, PP, PPP);
+    SB();
+    T r;
+    mont29_limbs<Fq29>(p.ZZ, PP, r.ZZ);
+    SB();
+    limbs29_of(p.X.v, L0);
+    const Fm Q = prod(L0, PP);
+    SB();
+    mont29_limbs<Fq29>(p.ZZZ, PPP, r.ZZZ);
+    SB();
+    limbs29_of(p.Y.v, L1);
+    const Fm YP = prod(L1, PPP);
+    SB();
+    Fm PPPw;
+    pack29(PPP, PPPw.v);
+    limbs29_of(R.v, L0);
+    r.X = sub(sub(prod(L0, L0), PPPw), dbl(Q));
+    SB();
+    limbs29_of(sub(Q, r.X).v, L1);
+    r.Y = sub(prod(L0, L1), YP);
+    return r;
+  }
+};
+
 // CONST: constant-work entries (index << 2 | discard << 1 | sign): the addition of a
 // discarded entry is computed and its result dropped by a select.
 template <class F, bool CONST>
@@ -135,7 +229,8 @@ __global__ void __launch_bounds__(128) k_accumulate(const uint2* __restrict__ en
   uint32_t cur = ent[b].x;
   bool starts = (b == 0) || ent[b - 1].x != cur;
   size_t rs = b;
-  Xyzz<F> acc = xyzz_inf<F>();
+  using A = Acc<F>;
+  typename A::T acc = A::inf();
   // G1: the next entry's point is loaded before this one is added, so the load latency of a
   // random base hides behind a mixed addition (the G2 loop has no registers to spare).
   constexpr bool PREFETCH = sizeof(F) == 32;
@@ -146,9 +241,9 @@ __global__ void __launch_bounds__(128) k_accumulate(const uint2* __restrict__ en
     uint32_t key = ent[k].x;
     if (key != cur) {
       // Run [rs, k) of bucket `cur` ends inside the segment.
-      if (starts) st_xyzz(buckets, cur, acc);
-      else st_xyzz(pfirst, t, acc);  // rs == b: the run began in an earlier segment
-      acc = xyzz_inf<F>();
+      if (starts) st_xyzz(buckets, cur, A::out(acc));
+      else st_xyzz(pfirst, t, A::out(acc));  // rs == b: the run began in an earlier segment
+      acc = A::inf();
       cur = key;
       starts = true;
       rs = k;
@@ -164,10 +259,10 @@ __global__ void __launch_bounds__(128) k_accumulate(const uint2* __restrict__ en
       a_next = ld_aff<F>(bases, v_next >> SH);
     }
     if (CONST) {
-      Xyzz<F> sum = xyzz_add_aff(acc, (v & 1) ? aff_neg(a) : a);
+      typename A::T sum = A::madd(acc, (v & 1) ? aff_neg(a) : a);
       acc = (v & 2) ? acc : sum;
     } else {
-      acc = xyzz_add_aff(acc, (v & 1) ? aff_neg(a) : a);
+      acc = A::madd(acc, (v & 1) ? aff_neg(a) : a);
     }
     if (PREFETCH) {
       v = v_next;
@@ -175,11 +270,12 @@ __global__ void __launch_bounds__(128) k_accumulate(const uint2* __restrict__ en
     }
   }
   bool ends = (e == T) || ent[e].x != cur;
+  const Xyzz<F> sum = A::out(acc);
   if (starts && ends) {
-    st_xyzz(buckets, cur, acc);
+    st_xyzz(buckets, cur, sum);
   } else {
-    if (rs == b) st_xyzz(pfirst, t, acc);
-    st_xyzz(plast, t, acc);
+    if (rs == b) st_xyzz(pfirst, t, sum);
+    st_xyzz(plast, t, sum);
   }
 }
 
