@@ -39,7 +39,7 @@ MUL32_PER_MADD = {1: 10 * 136, 2: 28 * 136}
 # HBM bytes per k_accumulate<Fm, false> launch in the 1.1M-constraint prove (FETCH_SIZE + WRITE_SIZE,
 # scaled by the counters' reading of a 256 MiB device copy: fetch x2, write x1), from the
 # rocprofv3 --pmc runs in profiles/k_accumulate_traffic_1_1M.txt.
-ACCUMULATE_TRAFFIC_BYTES_1_1M = 3.65e9
+ACCUMULATE_TRAFFIC_BYTES_1_1M = 3.67e9
 # Reference, CUDA back end on a Tesla T4: single prove, synthetic key, domain 2^21, warm:
 # 1263 ms (bench/results/cuda-size-sweep-2026-11-10.md:26).
 REFERENCE_T4_PROVE_MS_2_21 = 1263.0
@@ -476,6 +476,30 @@ def main() -> int:
         log(f"sort: {sort_bytes / 1e6:.0f} MB algorithmic in {sort_ms:.3f} ms -> "
             f"{msm['sort_hbm']['GB_per_s']:.0f} GB/s")
 
+    msm_g2 = None
+    if not args.no_msm:
+        # G2 MSM at 2^18 (bases k_i G2 from the GPU fixed-base setup, same scalar recipe).
+        n2 = 1 << 18
+        bases2 = dev.fixed_base(2, scalars(4343, n2, 0))
+        sc2 = scalars(2121, n2, 300)
+        d_b2, d_s2 = DeviceArray(bases2), DeviceArray(sc2)
+        dev.msm_device(2, d_b2.addr, d_s2.addr, n2)
+        times2, stats2 = [], []
+        for _ in range(args.msm_reps):
+            synchronize()
+            ts = time.perf_counter()
+            dev.msm_device(2, d_b2.addr, d_s2.addr, n2)
+            times2.append(time.perf_counter() - ts)
+            stats2.append(dev.msm_stats())
+        med2 = float(np.median(times2))
+        msm_g2 = {
+            "n": n2, "points_per_s": n2 / med2, "ms_median": med2 * 1e3, "c": stats2[-1]["c"],
+            "entries": stats2[-1]["entries"],
+            "phase_ms": {k: float(np.median([st[k] for st in stats2]))
+                         for k in ("sort_ms", "accumulate_ms", "merge_ms", "reduce_ms", "total_ms")},
+        }
+        log(f"MSM G2 2^18: {med2 * 1e3:.2f} ms -> {n2 / med2 / 1e6:.1f} M points/s")
+
     ntt = None
     if not args.no_msm:
         # NTT over 2^21 resident Fr: one forward transform (64 B per element algorithmic: one
@@ -538,6 +562,7 @@ def main() -> int:
         "proof_verifies": True,
         "verify_ms": verify_ms,
         "msm_g1": msm,
+        "msm_g2": msm_g2,
         "ntt": ntt,
         "roofline": roofline,
         "cpu_baseline": cpu,
