@@ -521,7 +521,6 @@ static void finish_msms(const g16hip_circuit* C, Slot* S, bool per_msm_events,
     wait(k);
     std::vector<G1X> v(L.len[k] / sizeof(G1X));
     std::memcpy(v.data(), hw + L.off[k], L.len[k]);
-    rows_from_msm_form(v);
     return msm_combine(v.data(), p);
   };
   bool want = blind && blind->want;
@@ -540,7 +539,6 @@ static void finish_msms(const g16hip_circuit* C, Slot* S, bool per_msm_events,
     wait(3);
     std::vector<G2X> v(L.len[3] / sizeof(G2X));
     std::memcpy(v.data(), hw + L.off[3], L.len[3]);
-    rows_from_msm_form(v);
     g2_store(out->b_g2, xyzz_to_aff(msm_combine(v.data(), pw)));
   });
   std::exception_ptr err;

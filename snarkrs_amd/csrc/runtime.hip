@@ -103,7 +103,6 @@ static Xyzz<F> run_msm(g16hip_device* d, int group, const void* bases, const voi
   st.c = (uint32_t)c;
   st.windows = (uint32_t)plan.windows;
   note_accumulate(d, group, entries, st.accumulate_ms);
-  rows_from_msm_form(S);
   return msm_combine(S.data(), plan);
 }
 

@@ -13,6 +13,7 @@
 
 #include "../../include/g16hip.h"
 #include "bn254.cuh"
+#include "host64.h"
 #include "internal.h"
 
 namespace g16 {
@@ -129,43 +130,117 @@ inline Fr fr_from_u64(uint64_t x) {
   return from_std(r);
 }
 
-// Rows come off the MSM kernels in the R' = 2^261 Montgomery form: x_R = x_R' / 32, one
-// R-form product by 2^251 (= 2^256 / 32).
-inline Fq from_msm_form(const Fq& x) {
-  Fq c = fp_zero<FqCfg>();
-  c.v[7] = 0x08000000u;
-  return mul(x, c);
-}
-inline Fq2 from_msm_form(const Fq2& x) { return {from_msm_form(x.c0), from_msm_form(x.c1)}; }
+// The host's finish of a windowed MSM: a few hundred point operations over the reduction's
+// rows, on the 64-bit-limb host field code (the 32-bit-limb field code the kernels share
+// runs at a third of the speed on the host). Both keep field elements fully reduced in the
+// R = 2^256 Montgomery form, so a value means the same bytes in either.
+namespace hx {
+using h64::Fq;
+using h64::Fq2;
+inline Fq mul(const Fq& a, const Fq& b) { return h64::f_mul(a, b); }
+inline Fq sqr(const Fq& a) { return h64::f_mul(a, a); }
+inline Fq add(const Fq& a, const Fq& b) { return h64::f_add(a, b); }
+inline Fq sub(const Fq& a, const Fq& b) { return h64::f_sub(a, b); }
+inline bool is_zero(const Fq& a) { return h64::f_is_zero(a); }
+inline Fq2 mul(const Fq2& a, const Fq2& b) { return h64::e2_mul(a, b); }
+inline Fq2 sqr(const Fq2& a) { return h64::e2_sqr(a); }
+inline Fq2 add(const Fq2& a, const Fq2& b) { return h64::e2_add(a, b); }
+inline Fq2 sub(const Fq2& a, const Fq2& b) { return h64::e2_sub(a, b); }
+inline bool is_zero(const Fq2& a) { return h64::e2_is_zero(a); }
 template <class F>
-void rows_from_msm_form(std::vector<Xyzz<F>>& rows) {
-  for (auto& r : rows) {
-    r.X = from_msm_form(r.X);
-    r.Y = from_msm_form(r.Y);
-    r.ZZ = from_msm_form(r.ZZ);
-    r.ZZZ = from_msm_form(r.ZZZ);
-  }
+F dbl(const F& a) {
+  return add(a, a);
 }
 
-// The MSM from the reduction's rows (msm_accumulate): sum over windows w and rows j of
-// 2^e P[w][j], e = c w for row 0 and c w + lg + j - 1 for row j >= 1 — one Horner pass over
-// the exponents, one doubling each (the reference finishes sum_w 2^(cw) S_w on the host the
-// same way, crates/cuda/src/msm.rs:3976-4010).
+template <class F>
+struct Pt {  // XYZZ, ZZ = 0 at infinity
+  F X, Y, ZZ, ZZZ;
+};
+
+// dbl-2008-s-1 (a = 0), as xyzz_dbl.
+template <class F>
+Pt<F> pt_dbl(const Pt<F>& p) {
+  if (is_zero(p.ZZ)) return p;
+  F U = dbl(p.Y), V = sqr(U), W = mul(U, V), S = mul(p.X, V);
+  F X2 = sqr(p.X), M = add(dbl(X2), X2);
+  Pt<F> r;
+  r.ZZ = mul(V, p.ZZ);
+  r.ZZZ = mul(W, p.ZZZ);
+  r.X = sub(sqr(M), dbl(S));
+  r.Y = sub(mul(M, sub(S, r.X)), mul(W, p.Y));
+  return r;
+}
+
+// add-2008-s with the doubling and cancellation cases, as xyzz_add.
+template <class F>
+Pt<F> pt_add(const Pt<F>& p, const Pt<F>& q) {
+  if (is_zero(p.ZZ)) return q;
+  if (is_zero(q.ZZ)) return p;
+  F U1 = mul(p.X, q.ZZ), S1 = mul(p.Y, q.ZZZ);
+  F P = sub(mul(q.X, p.ZZ), U1), R = sub(mul(q.Y, p.ZZZ), S1);
+  F ZZ12 = mul(p.ZZ, q.ZZ), ZZZ12 = mul(p.ZZZ, q.ZZZ);
+  if (is_zero(P)) {
+    if (is_zero(R)) return pt_dbl(Pt<F>{U1, S1, ZZ12, ZZZ12});
+    return Pt<F>{};
+  }
+  F PP = sqr(P), PPP = mul(P, PP), Q = mul(U1, PP);
+  Pt<F> r;
+  r.ZZ = mul(ZZ12, PP);
+  r.ZZZ = mul(ZZZ12, PPP);
+  r.X = sub(sub(sqr(R), PPP), dbl(Q));
+  r.Y = sub(mul(R, sub(Q, r.X)), mul(S1, PPP));
+  return r;
+}
+
+// Rows come off the MSM kernels in the R' = 2^261 Montgomery form, below 2q: x_R = x_R' / 32,
+// one R-form product by 2^251 (= 2^256 / 32) after a conditional subtraction of q.
+inline Fq from_msm_form(Fq x) {
+  if (!h64::lt_p<h64::QTag>(x.v)) h64::sub_p<h64::QTag>(x.v);
+  return h64::f_mul(x, Fq{{0, 0, 0, 0x0800000000000000ull}});
+}
+inline Fq2 from_msm_form(const Fq2& x) {
+  return {from_msm_form(x.c0), from_msm_form(x.c1)};
+}
+
+template <class F> struct Of;
+template <> struct Of<g16::Fq> { typedef Fq T; };
+template <> struct Of<g16::Fq2> { typedef Fq2 T; };
+}  // namespace hx
+
+// The MSM from the reduction's rows (msm_accumulate, in the kernels' R' form): sum over
+// windows w and rows j of 2^e P[w][j], e = c w for row 0 and c w + lg + j - 1 for row
+// j >= 1 — one Horner pass over the exponents, one doubling each (the reference finishes
+// sum_w 2^(cw) S_w on the host the same way, crates/cuda/src/msm.rs:3976-4010).
 template <class F>
 Xyzz<F> msm_combine(const Xyzz<F>* P, const MsmPlan& p) {
+  typedef typename hx::Of<F>::T H;
+  static_assert(sizeof(H) == sizeof(F), "host field layouts");
   int rows, lg;
   msm_reduce_shape(p, &rows, &lg);
   const int c = p.c, W = msm_bucket_sets(p);  // one set when the windows are merged
   int emax = c * (W - 1) + (rows > 1 ? lg + rows - 2 : 0);
-  std::vector<Xyzz<F>> E((size_t)emax + 1, xyzz_inf<F>());
+  std::vector<hx::Pt<H>> E((size_t)emax + 1, hx::Pt<H>{});
   for (int w = 0; w < W; ++w)
     for (int j = 0; j < rows; ++j) {
       int e = c * w + (j == 0 ? 0 : lg + j - 1);
-      E[e] = xyzz_add(E[e], P[(size_t)w * rows + j]);
+      hx::Pt<H> q;
+      const Xyzz<F>& s = P[(size_t)w * rows + j];
+      std::memcpy(&q.X, &s.X, sizeof(H));
+      std::memcpy(&q.Y, &s.Y, sizeof(H));
+      std::memcpy(&q.ZZ, &s.ZZ, sizeof(H));
+      std::memcpy(&q.ZZZ, &s.ZZZ, sizeof(H));
+      q = {hx::from_msm_form(q.X), hx::from_msm_form(q.Y), hx::from_msm_form(q.ZZ),
+           hx::from_msm_form(q.ZZZ)};
+      E[e] = hx::pt_add(E[e], q);
     }
-  Xyzz<F> acc = E[emax];
-  for (int e = emax - 1; e >= 0; --e) acc = xyzz_add(xyzz_dbl(acc), E[e]);
-  return acc;
+  hx::Pt<H> acc = E[emax];
+  for (int e = emax - 1; e >= 0; --e) acc = hx::pt_add(hx::pt_dbl(acc), E[e]);
+  Xyzz<F> out;
+  std::memcpy(&out.X, &acc.X, sizeof(H));
+  std::memcpy(&out.Y, &acc.Y, sizeof(H));
+  std::memcpy(&out.ZZ, &acc.ZZ, sizeof(H));
+  std::memcpy(&out.ZZZ, &acc.ZZZ, sizeof(H));
+  return out;
 }
 
 inline float ev_ms(hipEvent_t a, hipEvent_t b) {
