@@ -439,7 +439,7 @@ typedef Fq2T<true> Fq2i;
 typedef Fq2T<true, Fm> Fm2i;
 
 template <class B>
-__host__ __device__ __attribute__((noinline)) B fq_mul_ool(const B& a, const B& b) {
+__host__ __device__ __attribute__((noinline)) B fq_mul_ool(B a, B b) {
   return mul(a, b);
 }
 

@@ -314,9 +314,9 @@ __global__ void __launch_bounds__(256) k_merge(const uint32_t* offset, uint32_t 
     }
     return;
   }
+  // One addition site: a G2 addition is over 100 KB of code.
   Xyzz<F> acc = ld_xyzz<F>(plast, sa);
-  for (uint32_t t = sa + 1; t < sb; ++t) acc = xyzz_add(acc, ld_xyzz<F>(pfirst, t));
-  acc = xyzz_add(acc, ld_xyzz<F>(pfirst, sb));
+  for (uint32_t t = sa + 1; t <= sb; ++t) acc = xyzz_add(acc, ld_xyzz<F>(pfirst, t));
   st_xyzz(buckets, k, acc);
 }
 
@@ -406,8 +406,8 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_tasks(const uint32_t* cnt, cons
   }
 }
 
-// Level 2: a heavy bucket's task sums (strided over the block, then a tree) plus its two end
-// pieces.
+// Level 2: a heavy bucket's task sums and its two end pieces (item nt: plast of the first
+// segment, nt + 1: pfirst of the last), strided over the block, then a tree.
 template <class F, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_heavy_finish(const uint32_t* offset,
                                                         const uint32_t* cnt,
@@ -418,17 +418,16 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_finish(const uint32_t* offset,
   uint32_t nh = cnt[0];
   for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
     uint32_t k = heavy[3 * h], tb = heavy[3 * h + 1], nt = heavy[3 * h + 2];
+    uint32_t sa = offset[k] / SEG, sb = (offset[k + 1] - 1) / SEG;
     Xyzz<F> acc = xyzz_inf<F>();
-    for (uint32_t t = threadIdx.x; t < nt; t += BLOCK) acc = xyzz_add(acc, ld_xyzz<F>(task_sum, tb + t));
+    for (uint32_t t = threadIdx.x; t < nt + 2; t += BLOCK) {
+      const uint4* src = t < nt ? task_sum : (t == nt ? plast : pfirst);
+      const size_t i = t < nt ? (size_t)tb + t : (t == nt ? sa : sb);
+      acc = xyzz_add(acc, ld_xyzz<F>(src, i));
+    }
     sh[threadIdx.x] = acc;
     block_tree<F, BLOCK>(sh);
-    if (threadIdx.x == 0) {
-      uint32_t s = offset[k], e = offset[k + 1];
-      uint32_t sa = s / SEG, sb = (e - 1) / SEG;
-      Xyzz<F> r = xyzz_add(ld_xyzz<F>(plast, sa), sh[0]);
-      r = xyzz_add(r, ld_xyzz<F>(pfirst, sb));
-      st_xyzz(buckets, k, r);
-    }
+    if (threadIdx.x == 0) st_xyzz(buckets, k, sh[0]);
     __syncthreads();
   }
 }
@@ -468,14 +467,12 @@ __global__ void __launch_bounds__(BLOCK) k_reduce_rows(const uint4* T, const uin
   uint32_t w = blockIdx.x / rows, r = blockIdx.x % rows;
   size_t base = (size_t)w * G;
   Xyzz<F> acc = xyzz_inf<F>();
-  if (r == 0) {
-    for (uint32_t g = threadIdx.x; g < G; g += BLOCK) acc = xyzz_add(acc, ld_xyzz<F>(U, base + g));
-  } else {
-    uint32_t k = r - 1, low = (1u << k) - 1u;
-    for (uint32_t m = threadIdx.x; m < G / 2; m += BLOCK) {
-      uint32_t g = ((m & ~low) << 1) | (1u << k) | (m & low);  // m with a 1 inserted at bit k
-      acc = xyzz_add(acc, ld_xyzz<F>(T, base + g));
-    }
+  const uint4* src = r == 0 ? U : T;
+  const uint32_t k = r == 0 ? 0 : r - 1, low = (1u << k) - 1u, count = r == 0 ? G : G / 2;
+  for (uint32_t m = threadIdx.x; m < count; m += BLOCK) {
+    // row 0: every g; row k + 1: m with a 1 inserted at bit k
+    const uint32_t g = r == 0 ? m : ((m & ~low) << 1) | (1u << k) | (m & low);
+    acc = xyzz_add(acc, ld_xyzz<F>(src, base + g));
   }
   sh[threadIdx.x] = acc;
   block_tree<F, BLOCK>(sh);
