@@ -440,9 +440,13 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_finish(const uint32_t* offset,
 //   S_w = sum_g U_g + B sum_g g T_g = sum_g U_g + sum_k 2^(k+lg) Q_k,
 //   Q_k = sum of T_g over the g whose bit k is set.
 // Level 1 (one thread per group, 2B additions) makes T and U; level 2 is a plain sum per
-// (window, row) — row 0 sums U, row k+1 is Q_k — with strided per-thread sums and an LDS
-// tree. Every chain is short; the doublings of the bit weights happen once, in the host's
-// Horner pass over all windows.
+// (window, row) — row 0 sums U, row k+1 is Q_k. A row's items are cut into S slices (S a
+// power of two, at most 64, dividing every row's count), one workgroup each: strided
+// per-thread sums and an LDS tree; with S > 1 the slice sums go to `out` = a scratch array
+// and k_reduce_slices adds each row's S of them, so a long row (one bucket set of 2^15
+// buckets: 8192 groups) costs two short trees rather than one long strided chain. Every
+// chain is short; the doublings of the bit weights happen once, in the host's Horner pass
+// over all windows.
 template <class F>
 __global__ void __launch_bounds__(128) k_reduce_groups(const uint4* buckets, uint32_t nb,
                                                        int W, uint32_t B, uint4* T,
@@ -460,16 +464,20 @@ __global__ void __launch_bounds__(128) k_reduce_groups(const uint4* buckets, uin
   st_xyzz(U, t, acc);
 }
 
+// Workgroup x = ((w rows + r) S + slice); writes out[x].
 template <class F, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_reduce_rows(const uint4* T, const uint4* U,
-                                                       uint32_t G, uint32_t rows, uint4* out) {
+                                                       uint32_t G, uint32_t rows, uint32_t S,
+                                                       uint4* out) {
   __shared__ Xyzz<F> sh[BLOCK];
-  uint32_t w = blockIdx.x / rows, r = blockIdx.x % rows;
+  const uint32_t sl = blockIdx.x % S, wr = blockIdx.x / S;
+  const uint32_t w = wr / rows, r = wr % rows;
   size_t base = (size_t)w * G;
   Xyzz<F> acc = xyzz_inf<F>();
   const uint4* src = r == 0 ? U : T;
   const uint32_t k = r == 0 ? 0 : r - 1, low = (1u << k) - 1u, count = r == 0 ? G : G / 2;
-  for (uint32_t m = threadIdx.x; m < count; m += BLOCK) {
+  const uint32_t per = count / S, m0 = sl * per;
+  for (uint32_t m = m0 + threadIdx.x; m < m0 + per; m += BLOCK) {
     // row 0: every g; row k + 1: m with a 1 inserted at bit k
     const uint32_t g = r == 0 ? m : ((m & ~low) << 1) | (1u << k) | (m & low);
     acc = xyzz_add(acc, ld_xyzz<F>(src, base + g));
@@ -482,18 +490,43 @@ __global__ void __launch_bounds__(BLOCK) k_reduce_rows(const uint4* T, const uin
   }
 }
 
-inline uint32_t reduce_group_size(const MsmPlan& p) { return p.nb < 8 ? p.nb : 8; }
+// Row x = w rows + r: out[x] = sum of parts[x S .. x S + S).
+template <class F, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_reduce_slices(const uint4* parts, uint32_t S,
+                                                         uint4* out) {
+  __shared__ Xyzz<F> sh[BLOCK];
+  Xyzz<F> acc = xyzz_inf<F>();
+  for (uint32_t i = threadIdx.x; i < S; i += BLOCK)
+    acc = xyzz_add(acc, ld_xyzz<F>(parts, (size_t)blockIdx.x * S + i));
+  sh[threadIdx.x] = acc;
+  block_tree<F, BLOCK>(sh);
+  if (threadIdx.x == 0) {
+    st_xyzz(out, blockIdx.x, sh[0]);
+    __threadfence_system();  // `out` may be pinned host memory the host reads next
+  }
+}
 
+inline uint32_t reduce_group_size(const MsmPlan& p) { return p.nb < 4 ? p.nb : 4; }
 inline int ilog2_u32(uint32_t x) {
   int l = 0;
   while ((2u << l) <= x) ++l;
   return l;
 }
 
+// Slices per row of G groups over `sets` bucket sets: rows hold G or G / 2 items, at least
+// 512 and 256 per slice, and the workgroups stay within what the GPU runs at once (512 of 256
+// lanes: a G2 addition allows two waves per SIMD), so a tree is never paid twice.
+inline uint32_t reduce_slices(uint32_t G, uint32_t sets) {
+  const uint32_t rows = 1 + (uint32_t)ilog2_u32(G);
+  uint32_t S = 1;
+  while (S < 64 && 2 * S * 512 <= G && (uint64_t)2 * S * rows * sets <= 512) S *= 2;
+  return S;
+}
+
 // Scratch of one accumulation (one base vector over one sorted entry list): buckets, the
 // segment end partials, heavy-bucket bookkeeping, task sums, reduction groups, counters.
 struct AccumLayout {
-  size_t buckets, pfirst, plast, heavy, tasks, task_sum, gT, gU, cnt, total;
+  size_t buckets, pfirst, plast, heavy, tasks, task_sum, gT, gU, parts, cnt, total;
 };
 
 inline AccumLayout accum_layout(const MsmPlan& p, size_t pt) {
@@ -518,6 +551,8 @@ inline AccumLayout accum_layout(const MsmPlan& p, size_t pt) {
   L.task_sum = take(pt * max_tasks);
   L.gT = take(pt * G * msm_bucket_sets(p));
   L.gU = take(pt * G * msm_bucket_sets(p));
+  const uint32_t S = reduce_slices((uint32_t)G, (uint32_t)msm_bucket_sets(p));
+  L.parts = take(S > 1 ? pt * S * (1 + ilog2_u32((uint32_t)G)) * msm_bucket_sets(p) : 0);
   L.cnt = take(16);
   L.total = off;
   return L;
@@ -529,7 +564,7 @@ inline size_t msm_accum_scratch_bytes_impl(const MsmPlan& p, int group) {
 
 // Pointers into one accumulation's scratch.
 struct AccumPtrs {
-  uint4 *buckets, *pfirst, *plast, *task_sum, *gT, *gU;
+  uint4 *buckets, *pfirst, *plast, *task_sum, *gT, *gU, *parts;
   uint32_t *heavy, *tasks, *cnt;
 };
 
@@ -578,9 +613,16 @@ inline void tail_reduce_impl(const MsmPlan& p, const AccumPtrs& a, uint32_t B, v
   hipLaunchKernelGGL(k_reduce_groups<F>, dim3((nthreads + 127) / 128), dim3(128), 0, s,
                      a.buckets, p.nb, sets, B, a.gT, a.gU);
   G16_LAUNCH_CHECK("msm reduce groups");
-  hipLaunchKernelGGL((k_reduce_rows<F, 256>), dim3((unsigned)sets * rows), dim3(256), 0, s,
-                     a.gT, a.gU, G, rows, static_cast<uint4*>(window_out));
+  const uint32_t S = reduce_slices(G, (uint32_t)sets);
+  uint4* wout = static_cast<uint4*>(window_out);
+  hipLaunchKernelGGL((k_reduce_rows<F, 256>), dim3((unsigned)sets * rows * S), dim3(256), 0, s,
+                     a.gT, a.gU, G, rows, S, S > 1 ? a.parts : wout);
   G16_LAUNCH_CHECK("msm reduce rows");
+  if (S > 1) {
+    hipLaunchKernelGGL((k_reduce_slices<F, 64>), dim3((unsigned)sets * rows), dim3(64), 0, s,
+                       a.parts, S, wout);
+    G16_LAUNCH_CHECK("msm reduce slices");
+  }
 }
 
 // F: the coordinate field in the MSM's Montgomery form R' = 2^261 (Fm, or Fm2i for G2); the
@@ -602,6 +644,7 @@ inline void accumulate_impl(const MsmPlan& p, const void* bases, void* scratch, 
   a.task_sum = reinterpret_cast<uint4*>(q + L.task_sum);
   a.gT = reinterpret_cast<uint4*>(q + L.gT);
   a.gU = reinterpret_cast<uint4*>(q + L.gU);
+  a.parts = reinterpret_cast<uint4*>(q + L.parts);
   a.cnt = reinterpret_cast<uint32_t*>(q + L.cnt);
   auto bp = static_cast<const uint4*>(bases);
 
