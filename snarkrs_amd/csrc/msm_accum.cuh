@@ -514,12 +514,13 @@ inline int ilog2_u32(uint32_t x) {
 }
 
 // Slices per row of G groups over `sets` bucket sets: rows hold G or G / 2 items, at least
-// 512 and 256 per slice, and the workgroups stay within what the GPU runs at once (512 of 256
-// lanes: a G2 addition allows two waves per SIMD), so a tree is never paid twice.
+// 512 and 256 per slice, and no more than 256 workgroups of four waves (one wave per SIMD):
+// past that, waves share SIMDs and every tree level slows by as much as slicing saves. In
+// practice only a single bucket set (merged windows) is sliced.
 inline uint32_t reduce_slices(uint32_t G, uint32_t sets) {
   const uint32_t rows = 1 + (uint32_t)ilog2_u32(G);
   uint32_t S = 1;
-  while (S < 64 && 2 * S * 512 <= G && (uint64_t)2 * S * rows * sets <= 512) S *= 2;
+  while (S < 64 && 2 * S * 512 <= G && (uint64_t)2 * S * rows * sets <= 256) S *= 2;
   return S;
 }
 
