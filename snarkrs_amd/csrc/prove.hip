@@ -526,24 +526,24 @@ static void finish_msms(const g16hip_circuit* C, Slot* S, bool per_msm_events,
   bool want = blind && blind->want;
   HostTask ta([&] {
     G1X A = g1_sum(0, pw);
-    g1_store(out->a_g1, xyzz_to_aff(A));
-    if (want) blind->sA = xyzz_mul(A, blind->s_std.v);
+    g1_store(out->a_g1, host_to_aff(A));
+    if (want) blind->sA = host_mul(A, blind->s_std.v);
   });
   HostTask tb([&] {
     G1X B1 = g1_sum(1, pw);
-    g1_store(out->b_g1, xyzz_to_aff(B1));
-    if (want) blind->rB1 = xyzz_mul(B1, blind->r_std.v);
+    g1_store(out->b_g1, host_to_aff(B1));
+    if (want) blind->rB1 = host_mul(B1, blind->r_std.v);
   });
-  HostTask tl([&] { g1_store(out->l_g1, xyzz_to_aff(g1_sum(2, pw))); });
+  HostTask tl([&] { g1_store(out->l_g1, host_to_aff(g1_sum(2, pw))); });
   HostTask t2([&] {
     wait(3);
     std::vector<G2X> v(L.len[3] / sizeof(G2X));
     std::memcpy(v.data(), hw + L.off[3], L.len[3]);
-    g2_store(out->b_g2, xyzz_to_aff(msm_combine(v.data(), pw)));
+    g2_store(out->b_g2, host_to_aff(msm_combine(v.data(), pw)));
   });
   std::exception_ptr err;
   try {
-    g1_store(out->h_g1, xyzz_to_aff(g1_sum(4, ph)));
+    g1_store(out->h_g1, host_to_aff(g1_sum(4, ph)));
   } catch (...) {
     err = std::current_exception();
   }
@@ -646,25 +646,27 @@ struct BlinderTerms {
 static BlinderTerms blinder_terms(const g16hip_circuit* C, const Fr& r_std, const Fr& s_std,
                                   const Fr& rs_std) {
   BlinderTerms b;
-  b.d1r = xyzz_mul_plain(C->delta_g1, r_std.v);
-  b.k1 = xyzz_add(xyzz_add(xyzz_mul_plain(C->alpha_g1, s_std.v),
-                           xyzz_mul_plain(C->beta_g1, r_std.v)),
-                  xyzz_mul_plain(C->delta_g1, rs_std.v));
-  b.d2s = xyzz_mul_plain(C->delta_g2, s_std.v);
+  b.d1r = host_mul(xyzz_from_aff(C->delta_g1), r_std.v);
+  b.k1 = host_add(host_add(host_mul(xyzz_from_aff(C->alpha_g1), s_std.v),
+                           host_mul(xyzz_from_aff(C->beta_g1), r_std.v)),
+                  host_mul(xyzz_from_aff(C->delta_g1), rs_std.v));
+  b.d2s = host_mul(xyzz_from_aff(C->delta_g2), s_std.v);
   return b;
 }
 
 static void assemble(const g16hip_circuit* C, const g16hip_msm_out& m, const BlinderTerms& b,
                      const BlindProducts& bp, g16hip_proof* proof) {
-  G1X pi_a = xyzz_add(xyzz_add_aff(xyzz_from_aff(g1_load(m.a_g1)), C->alpha_g1), b.d1r);
-  G2X pi_b = xyzz_add(xyzz_add_aff(xyzz_from_aff(g2_load(m.b_g2)), C->beta_g2), b.d2s);
-  G1X pi_c = xyzz_add_aff(xyzz_from_aff(g1_load(m.l_g1)), g1_load(m.h_g1));
-  pi_c = xyzz_add(pi_c, bp.sA);
-  pi_c = xyzz_add(pi_c, bp.rB1);
-  pi_c = xyzz_add(pi_c, b.k1);
-  g1_store(proof->a, xyzz_to_aff(pi_a));
-  g2_store(proof->b, xyzz_to_aff(pi_b));
-  g1_store(proof->c, xyzz_to_aff(pi_c));
+  G1X pi_a = host_add(host_add(xyzz_from_aff(g1_load(m.a_g1)), xyzz_from_aff(C->alpha_g1)),
+                      b.d1r);
+  G2X pi_b = host_add(host_add(xyzz_from_aff(g2_load(m.b_g2)), xyzz_from_aff(C->beta_g2)),
+                      b.d2s);
+  G1X pi_c = host_add(xyzz_from_aff(g1_load(m.l_g1)), xyzz_from_aff(g1_load(m.h_g1)));
+  pi_c = host_add(pi_c, bp.sA);
+  pi_c = host_add(pi_c, bp.rB1);
+  pi_c = host_add(pi_c, b.k1);
+  g1_store(proof->a, host_to_aff(pi_a));
+  g2_store(proof->b, host_to_aff(pi_b));
+  g1_store(proof->c, host_to_aff(pi_c));
 }
 
 // Stages 0-11 for one witness on a leased slot: eager with timings (`graph` false) or as a

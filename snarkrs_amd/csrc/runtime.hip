@@ -139,7 +139,7 @@ static bool xyzz_on_curve(const Xyzz<F>& p);
 
 template <>
 bool xyzz_on_curve<Fq>(const G1X& p) {
-  G1Aff a = xyzz_to_aff(p);
+  G1Aff a = host_to_aff(p);
   if (aff_is_inf(a)) return true;
   Fq b = fq_small(3);
   return eq(sqr(a.y), add(mul(sqr(a.x), a.x), b));
@@ -147,7 +147,7 @@ bool xyzz_on_curve<Fq>(const G1X& p) {
 
 template <>
 bool xyzz_on_curve<Fq2>(const G2X& p) {
-  G2Aff a = xyzz_to_aff(p);
+  G2Aff a = host_to_aff(p);
   if (aff_is_inf(a)) return true;
   // b' = 3 / (9 + u)
   Fq2 xi{fq_small(9), fq_small(1)};
@@ -216,7 +216,7 @@ static Xyzz<F> ceremony_msm(g16hip_ceremony* cer, int group, const uint8_t* base
     size_t m = std::min(len, n - at);
     try {
       Xyzz<F> p = ceremony_run<F>(cer, group, bases + pw * at, scalars + 32 * at, m);
-      sum = xyzz_add(sum, p);
+      sum = host_add(sum, p);
       at += m;
       ++runs;
     } catch (const StatusError& e) {
@@ -269,8 +269,8 @@ g16hip_status g16hip_ceremony_msm(g16hip_ceremony* c, int group, const uint8_t* 
       std::memset(out, 0, pw);
       return G16HIP_OK;
     }
-    if (group == 1) g1_store(out, xyzz_to_aff(ceremony_msm<Fq>(c, 1, bases, scalars, n)));
-    else g2_store(out, xyzz_to_aff(ceremony_msm<Fq2>(c, 2, bases, scalars, n)));
+    if (group == 1) g1_store(out, host_to_aff(ceremony_msm<Fq>(c, 1, bases, scalars, n)));
+    else g2_store(out, host_to_aff(ceremony_msm<Fq2>(c, 2, bases, scalars, n)));
     return G16HIP_OK;
   });
 }
@@ -427,8 +427,8 @@ static g16hip_status msm_host(g16hip_device* d, int group, const uint8_t* bases,
     if (n >= (1u << 31)) throw ArgError{"MSM larger than 2^31 points"};
     std::lock_guard<std::mutex> lk(d->mu);
     set_device(d);
-    if (group == 1) g1_store(out, xyzz_to_aff(msm_from_host<Fq>(d, 1, bases, scalars, n)));
-    else g2_store(out, xyzz_to_aff(msm_from_host<Fq2>(d, 2, bases, scalars, n)));
+    if (group == 1) g1_store(out, host_to_aff(msm_from_host<Fq>(d, 1, bases, scalars, n)));
+    else g2_store(out, host_to_aff(msm_from_host<Fq2>(d, 2, bases, scalars, n)));
     return G16HIP_OK;
   });
 }
@@ -456,9 +456,9 @@ static g16hip_status msm_dev(g16hip_device* d, int group, const void* bases, con
     std::lock_guard<std::mutex> lk(d->mu);
     set_device(d);
     if (group == 1)
-      g1_store(out, xyzz_to_aff(run_msm<Fq>(d, 1, bases, scalars, (uint32_t)n, c)));
+      g1_store(out, host_to_aff(run_msm<Fq>(d, 1, bases, scalars, (uint32_t)n, c)));
     else
-      g2_store(out, xyzz_to_aff(run_msm<Fq2>(d, 2, bases, scalars, (uint32_t)n, c)));
+      g2_store(out, host_to_aff(run_msm<Fq2>(d, 2, bases, scalars, (uint32_t)n, c)));
     return G16HIP_OK;
   });
 }

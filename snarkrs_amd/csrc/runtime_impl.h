@@ -205,7 +205,83 @@ inline Fq2 from_msm_form(const Fq2& x) {
 template <class F> struct Of;
 template <> struct Of<g16::Fq> { typedef Fq T; };
 template <> struct Of<g16::Fq2> { typedef Fq2 T; };
+
+// a^(q - 2) (Fermat); 0 maps to 0.
+inline Fq inv(const Fq& a) {
+  const uint64_t* P = h64::QTag::P;
+  const uint64_t E[4] = {P[0] - 2, P[1], P[2], P[3]};
+  Fq acc = h64::f_one<h64::QTag>();
+  for (int i = 255; i >= 0; --i) {
+    acc = sqr(acc);
+    if ((E[i >> 6] >> (i & 63)) & 1) acc = mul(acc, a);
+  }
+  return acc;
+}
+// 1 / (a0 + a1 u) = (a0 - a1 u) / (a0^2 + a1^2).
+inline Fq2 inv(const Fq2& a) {
+  const Fq t = inv(add(sqr(a.c0), sqr(a.c1)));
+  return {mul(a.c0, t), h64::f_neg(mul(a.c1, t))};
+}
+
+template <class G>
+Pt<typename Of<G>::T> in(const Xyzz<G>& p) {
+  typedef typename Of<G>::T H;
+  static_assert(sizeof(H) == sizeof(G), "host field layouts");
+  Pt<H> q;
+  std::memcpy(&q.X, &p.X, sizeof(H));
+  std::memcpy(&q.Y, &p.Y, sizeof(H));
+  std::memcpy(&q.ZZ, &p.ZZ, sizeof(H));
+  std::memcpy(&q.ZZZ, &p.ZZZ, sizeof(H));
+  return q;
+}
+template <class G>
+Xyzz<G> out(const Pt<typename Of<G>::T>& q) {
+  typedef typename Of<G>::T H;
+  Xyzz<G> p;
+  std::memcpy(&p.X, &q.X, sizeof(H));
+  std::memcpy(&p.Y, &q.Y, sizeof(H));
+  std::memcpy(&p.ZZ, &q.ZZ, sizeof(H));
+  std::memcpy(&p.ZZZ, &q.ZZZ, sizeof(H));
+  return p;
+}
 }  // namespace hx
+
+// Point operations for the host's part of an MSM or a proof (finishing sums, blinding
+// products, stage 11, affine output), on the 64-bit field code and the kernels' types.
+template <class G>
+Xyzz<G> host_add(const Xyzz<G>& a, const Xyzz<G>& b) {
+  return hx::out<G>(hx::pt_add(hx::in(a), hx::in(b)));
+}
+
+// k * p for a plain (standard-form) 256-bit scalar, double-and-add from the top bit.
+template <class G>
+Xyzz<G> host_mul(const Xyzz<G>& p, const uint32_t k[8]) {
+  const auto q = hx::in(p);
+  decltype(hx::in(p)) acc{};
+  for (int i = 255; i >= 0; --i) {
+    acc = hx::pt_dbl(acc);
+    if ((k[i >> 5] >> (i & 31)) & 1) acc = hx::pt_add(acc, q);
+  }
+  return hx::out<G>(acc);
+}
+
+// x = X / ZZ, y = Y / ZZZ with one inversion: ZZ / ZZZ = 1 / Z, squared 1 / ZZ.
+template <class G>
+Aff<G> host_to_aff(const Xyzz<G>& p) {
+  typedef typename hx::Of<G>::T H;
+  const auto q = hx::in(p);
+  Aff<G> a;
+  if (hx::is_zero(q.ZZ)) {
+    std::memset(&a, 0, sizeof(a));
+    return a;
+  }
+  const H izzz = hx::inv(q.ZZZ);
+  const H iz2 = hx::sqr(hx::mul(izzz, q.ZZ));
+  const H x = hx::mul(q.X, iz2), y = hx::mul(q.Y, izzz);
+  std::memcpy(&a.x, &x, sizeof(H));
+  std::memcpy(&a.y, &y, sizeof(H));
+  return a;
+}
 
 // The MSM from the reduction's rows (msm_accumulate, in the kernels' R' form): sum over
 // windows w and rows j of 2^e P[w][j], e = c w for row 0 and c w + lg + j - 1 for row
