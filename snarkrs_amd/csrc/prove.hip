@@ -675,11 +675,18 @@ static void prove_on_slot(g16hip_circuit* C, Slot* S, const void* w, size_t n, b
                           const uint8_t* r8, const uint8_t* s8, g16hip_proof* proof,
                           g16hip_timings* t, bool graph, const Rows* rows = nullptr) {
   check_witness_len(C, n);
-  // w[0] is the constant-one wire (crates/groth16/src/prove.rs:272-283).
+  // w[0] is the constant-one wire (crates/groth16/src/prove.rs:272-283). A device witness's
+  // w[0] comes back with the proof (pinned bytes after the MSM totals) instead of through a
+  // synchronous copy that would hold the GPU idle before the proof starts; the call still
+  // fails with CONSTANT_WIRE and writes no proof.
   Fr w0;
-  if (on_device) G16_CHECK(hipMemcpy(w0.v, w, 32, hipMemcpyDeviceToHost));
-  else std::memcpy(w0.v, w, 32);
-  if (!is_one(w0)) throw StatusError{G16HIP_CONSTANT_WIRE, "w[0] is the constant-one wire and must be 1"};
+  uint8_t* w0_pinned = S->host_wout + win_layout(S).totals + 16;
+  if (on_device) {
+    G16_CHECK(hipMemcpyAsync(w0_pinned, w, 32, hipMemcpyDeviceToHost, S->stream));
+  } else {
+    std::memcpy(w0.v, w, 32);
+    if (!is_one(w0)) throw StatusError{G16HIP_CONSTANT_WIRE, "w[0] is the constant-one wire and must be 1"};
+  }
   Fr r = fr_load(r8), s = fr_load(s8);
   if (is_zero(r) || is_zero(s)) throw StatusError{G16HIP_ZERO_BLINDER, "a blinder is zero"};
   BlindProducts bp;
@@ -695,6 +702,10 @@ static void prove_on_slot(g16hip_circuit* C, Slot* S, const void* w, size_t n, b
   else run_timed(C, S, true, S->hbuf.p, &m, t, &bp, rows);
   th.join();
   if (th.err) std::rethrow_exception(th.err);
+  if (on_device) {  // the slot's streams are synchronised: the copy has landed
+    std::memcpy(w0.v, w0_pinned, 32);
+    if (!is_one(w0)) throw StatusError{G16HIP_CONSTANT_WIRE, "w[0] is the constant-one wire and must be 1"};
+  }
   Timer ta;
   assemble(C, m, terms, bp, proof);
   if (t) t->assemble_us += ta.us();

@@ -89,6 +89,15 @@ def test_error_paths(dev, orc, small):
         circ.prove_with_blinders(bad, r, s)
     with pytest.raises(native.ZeroBlinder):
         circ.prove_with_blinders(wit, fr_array([0]), s)
+    # A device witness's w[0] is checked after its proof ran: the call still fails, and the
+    # slot proves the next witness correctly.
+    from snarkrs_amd.devmem import DeviceArray
+    d_bad, d_good = DeviceArray(np.asarray(bad)), DeviceArray(np.asarray(wit))
+    with pytest.raises(native.ConstantWire):
+        circ.prove_with_blinders_device(d_bad.addr, len(bad), r, s)
+    assert (circ.prove_with_blinders_device(d_good.addr, len(wit), r, s)
+            == circ.prove_with_blinders(wit, r, s)).all()
+    del d_bad, d_good
     other = dev.prepare(key)
     hh = other.compute_h(wit)
     with pytest.raises(native.BackendError):
