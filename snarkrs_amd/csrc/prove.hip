@@ -14,6 +14,7 @@
 #include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
+#include <functional>
 #include <deque>
 #include <unordered_map>
 
@@ -472,6 +473,8 @@ static void enqueue_stages(const g16hip_circuit* C, Slot* S, bool with_h, void* 
   G16_CHECK(hipStreamWaitEvent(s1, S->sync_ev[0], 0));
   msm_sort(pw, S->witness.p, s1);
   G16_CHECK(hipEventRecord(S->sync_ev[1], s1));
+  // Entry totals for the kernel statistics, copied while the MSMs run rather than after them.
+  if (timed) G16_CHECK(hipMemcpyAsync(&totals[0], pw.total, 4, hipMemcpyDeviceToHost, s1));
   // B2 is the longest chain: it is submitted first, so it also starts first where streams
   // end up sharing a hardware queue.
   G16_CHECK(hipStreamWaitEvent(s2, S->sync_ev[1], 0));
@@ -487,15 +490,12 @@ static void enqueue_stages(const g16hip_circuit* C, Slot* S, bool with_h, void* 
   G16_CHECK(hipEventRecord(S->sync_ev[2], s1));
   if (with_h) stages_0_4(C, S, h, timed, rows);
   msm_sort(ph, h, s0);
+  if (timed) G16_CHECK(hipMemcpyAsync(&totals[1], ph.total, 4, hipMemcpyDeviceToHost, s0));
   msm_accumulate(ph, 1, C->h_q.p, S->accum_h.p, wout + L.off[4], s0, mev4);
   rows_done(4, s0);
   G16_CHECK(hipStreamWaitEvent(s0, S->sync_ev[2], 0));
   G16_CHECK(hipStreamWaitEvent(s0, S->sync_ev[3], 0));
-  if (timed) {
-    G16_CHECK(hipEventRecord(S->ev[5], s0));
-    G16_CHECK(hipMemcpyAsync(&totals[0], pw.total, 4, hipMemcpyDeviceToHost, s0));
-    G16_CHECK(hipMemcpyAsync(&totals[1], ph.total, 4, hipMemcpyDeviceToHost, s0));
-  }
+  if (timed) G16_CHECK(hipEventRecord(S->ev[5], s0));
 }
 
 // The stage-11 products that need an MSM output: s*A and r*B1 (prove.rs:391-430 groups the
@@ -554,23 +554,23 @@ static void finish_msms(const g16hip_circuit* C, Slot* S, bool per_msm_events,
   if (err) std::rethrow_exception(err);
 }
 
-static void sync_slot(Slot* S) {
-  G16_CHECK(hipStreamSynchronize(S->stream));
-  G16_CHECK(hipStreamSynchronize(S->side[0]));
-  G16_CHECK(hipStreamSynchronize(S->side[1]));
-}
-
 // Eager run with events: timings, kernel statistics, the optional per-proof timeline.
+// `beside` runs on the calling thread once the GPU work is enqueued (host work that overlaps
+// it without a thread of its own).
 static void run_timed(g16hip_circuit* C, Slot* S, bool with_h, void* h, g16hip_msm_out* out,
-                      g16hip_timings* t, BlindProducts* blind, const Rows* rows = nullptr) {
+                      g16hip_timings* t, BlindProducts* blind, const Rows* rows = nullptr,
+                      const std::function<void()>& beside = nullptr) {
   enqueue_stages(C, S, with_h, h, true, rows);
   std::exception_ptr err;
   try {
+    if (beside) beside();
     finish_msms(C, S, true, out, blind);
   } catch (...) {
     err = std::current_exception();
   }
-  sync_slot(S);
+  // ev[5] follows every stream's work (the main stream waited for both side streams before
+  // recording it): one event wait instead of three stream synchronisations.
+  G16_CHECK(hipEventSynchronize(S->ev[5]));
   if (err) std::rethrow_exception(err);
   if (rows) check_gnark_h(C, S);
   WinLayout L = win_layout(S);
@@ -610,7 +610,8 @@ static void run_timed(g16hip_circuit* C, Slot* S, bool with_h, void* h, g16hip_m
 // the graph; there are no memset nodes: every clear is a fill kernel), then replayed.
 // Replays go on the slot's own main stream, which lives as long as the graph: launching a
 // slot's graph on a stream that is destroyed before the graph's next launch is not done.
-static void run_graph(g16hip_circuit* C, Slot* S, g16hip_msm_out* out, BlindProducts* blind) {
+static void run_graph(g16hip_circuit* C, Slot* S, g16hip_msm_out* out, BlindProducts* blind,
+                      const std::function<void()>& beside = nullptr) {
   if (!S->exec) {
     G16_CHECK(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
     try {
@@ -625,7 +626,14 @@ static void run_graph(g16hip_circuit* C, Slot* S, g16hip_msm_out* out, BlindProd
     G16_CHECK(hipGraphInstantiate(&S->exec, S->graph, nullptr, nullptr, 0));
   }
   G16_CHECK(hipGraphLaunch(S->exec, S->stream));
+  std::exception_ptr err;
+  try {
+    if (beside) beside();
+  } catch (...) {
+    err = std::current_exception();
+  }
   G16_CHECK(hipStreamSynchronize(S->stream));
+  if (err) std::rethrow_exception(err);
   finish_msms(C, S, false, out, blind);
 }
 
@@ -695,13 +703,11 @@ static void prove_on_slot(g16hip_circuit* C, Slot* S, const void* w, size_t n, b
   bp.s_std = to_std(s);
   Fr rs_std = to_std(mul(r, s));
   BlinderTerms terms;
-  HostTask th([&] { terms = blinder_terms(C, bp.r_std, bp.s_std, rs_std); });
+  const std::function<void()> beside = [&] { terms = blinder_terms(C, bp.r_std, bp.s_std, rs_std); };
   g16hip_msm_out m;
   load_witness(C, S, w, n, on_device);
-  if (graph && !rows) run_graph(C, S, &m, &bp);
-  else run_timed(C, S, true, S->hbuf.p, &m, t, &bp, rows);
-  th.join();
-  if (th.err) std::rethrow_exception(th.err);
+  if (graph && !rows) run_graph(C, S, &m, &bp, beside);
+  else run_timed(C, S, true, S->hbuf.p, &m, t, &bp, rows, beside);
   if (on_device) {  // the slot's streams are synchronised: the copy has landed
     std::memcpy(w0.v, w0_pinned, 32);
     if (!is_one(w0)) throw StatusError{G16HIP_CONSTANT_WIRE, "w[0] is the constant-one wire and must be 1"};
