@@ -116,7 +116,9 @@ void msm_reduce_shape(const MsmPlan& p, int* rows, int* lg);
 void msm_to_msm_form(const void* src, void* dst, size_t n_fq, hipStream_t s);
 size_t msm_out_points(const MsmPlan& p);
 int msm_default_c(uint32_t n);
-// Width for scalars sorted once for g1 G1 MSMs and g2 G2 MSMs.
+// Width for scalars sorted once for g1 G1 MSMs and g2 G2 MSMs (per-window buckets), and the
+// same over window tables (one bucket set).
 int msm_choose_c(uint32_t n, int g1, int g2);
+int msm_choose_c_merged(uint32_t n, int g1, int g2);
 
 }  // namespace g16

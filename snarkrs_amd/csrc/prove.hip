@@ -164,13 +164,14 @@ static void upload_key(g16hip_circuit* C, const g16hip_key* k, hipStream_t s) {
   if (C->has_matrices) upload(C->values, k->values, 32 * (size_t)k->n_values, s);
   const size_t nh = C->n_h;
   size_t nv = k->n_vars;
-  // The witness digits feed A, B1, L (G1) and B2 (G2); H feeds one G1 MSM.
+  // The witness digits feed A, B1, L (G1) and B2 (G2); H feeds one G1 MSM. Widths are chosen
+  // for window tables first and again per window if the tables do not fit.
   // G16HIP_C_WITNESS / G16HIP_C_H override the widths (2..16).
-  C->cw = msm_choose_c((uint32_t)nv, 3, 1);
-  C->ch = msm_default_c((uint32_t)nh);
+  C->cw = msm_choose_c_merged((uint32_t)nv, 3, 1);
+  C->ch = msm_choose_c_merged((uint32_t)nh, 1, 0);
   if (const char* e = std::getenv("G16HIP_C_WITNESS")) C->cw = std::max(2, std::min(16, std::atoi(e)));
   if (const char* e = std::getenv("G16HIP_C_H")) C->ch = std::max(2, std::min(16, std::atoi(e)));
-  const int Ww = (255 + C->cw - 1) / C->cw, Wh = (255 + C->ch - 1) / C->ch;
+  int Ww = (255 + C->cw - 1) / C->cw, Wh = (255 + C->ch - 1) / C->ch;
   // Window tables: with entry w n + i = 2^(c w) P_i precomputed (the key's bases are fixed),
   // an MSM accumulates the digits of every window into one set of buckets, so the merges and
   // the window reduction run once instead of once per window. They take W times the bases'
@@ -181,6 +182,12 @@ static void upload_key(g16hip_circuit* C, const g16hip_key* k, hipStream_t s) {
   G16_CHECK(hipMemGetInfo(&free_b, &total_b));
   const char* wt = std::getenv("G16HIP_WINDOW_TABLES");
   C->merged = !(wt && wt[0] == '0') && free_b > table_bytes + (8ull << 30);
+  if (!C->merged) {
+    if (!std::getenv("G16HIP_C_WITNESS")) C->cw = msm_choose_c((uint32_t)nv, 3, 1);
+    if (!std::getenv("G16HIP_C_H")) C->ch = msm_default_c((uint32_t)nh);
+    Ww = (255 + C->cw - 1) / C->cw;
+    Wh = (255 + C->ch - 1) / C->ch;
+  }
   const size_t tw = C->merged ? (size_t)Ww : 1, th = C->merged ? (size_t)Wh : 1;
   auto put = [&](DevBuf& b, const void* src, size_t npts, size_t pt, size_t windows) {
     b.alloc(pt * npts * windows);
