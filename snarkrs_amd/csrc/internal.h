@@ -66,7 +66,6 @@ struct MsmPlan {
   int windows = 0;        // ceil(255 / c)
   uint32_t n = 0;         // scalars
   uint32_t nb = 0;        // buckets per window, 2^(c-1)
-  uint32_t seg = 0;       // entries per accumulation segment
   size_t max_entries = 0; // n * windows
   uint32_t chunks = 0;    // scalar chunks of the counting sort
   // Merged windows: the bases come as window tables (entry w n + i = 2^(c w) P_i), so the

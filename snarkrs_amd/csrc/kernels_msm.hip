@@ -299,7 +299,6 @@ void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c, bool merged, bo
   p.windows = (RECODE_BITS + c - 1) / c;
   p.n = n;
   p.nb = 1u << (c - 1);
-  p.seg = SEG;
   p.max_entries = (size_t)n * p.windows;
   p.chunks = sort_chunks(n);
   size_t K = (size_t)p.windows * p.nb;

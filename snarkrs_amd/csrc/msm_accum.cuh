@@ -9,12 +9,17 @@ namespace g16 {
 
 inline size_t accum_align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
-static constexpr int SEG = 32;
+// Entries per accumulation segment: 32, and 40 for a G1 MSM over per-window buckets, whose
+// merges are one or two additions per bucket, so fewer and longer segments pay there (the
+// 2^20 G1 MSM: 2.13 against 2.25-2.30 ms; G2 and the prove's merged MSMs were slower).
+inline uint32_t accum_seg(const MsmPlan& p, size_t point_bytes) {
+  return !p.merged && point_bytes == 128 ? 40u : 32u;
+}
 
 template <int = 0>
 __global__ void k_clear_cnt(uint32_t* cnt) {
   if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
-}          // entries per accumulation segment
+}
 static constexpr int HEAVY_SPAN = 32;   // segments a merge thread folds itself
 
 __device__ __forceinline__ Fr ld_fr(const uint4* p) {
@@ -217,15 +222,16 @@ struct Acc<Fm> {
 template <class F, bool CONST>
 __global__ void __launch_bounds__(128) k_accumulate(const uint2* __restrict__ ent,
                                                     const uint32_t* total, size_t nseg,
+                                                    uint32_t seg,
                                                     const uint4* __restrict__ bases,
                                                     uint4* buckets, uint4* pfirst,
                                                     uint4* plast) {
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nseg) return;
   uint32_t T = total[0];
-  size_t b = t * SEG;
+  size_t b = t * seg;
   if (b >= T) return;
-  size_t e = b + SEG < T ? b + SEG : T;
+  size_t e = b + seg < T ? b + seg : T;
   uint32_t cur = ent[b].x;
   bool starts = (b == 0) || ent[b - 1].x != cur;
   size_t rs = b;
@@ -290,7 +296,8 @@ static constexpr int HEAVY_TASK = 256;
 template <class F>
 __global__ void __launch_bounds__(256) k_merge(const uint32_t* offset, uint32_t K, uint4* buckets,
                                                const uint4* pfirst, const uint4* plast,
-                                               uint32_t* cnt, uint32_t* heavy, uint32_t* tasks) {
+                                               uint32_t* cnt, uint32_t* heavy, uint32_t* tasks,
+                                               uint32_t seg) {
   uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= K) return;
   uint32_t s = offset[k], e = offset[k + 1];
@@ -298,7 +305,7 @@ __global__ void __launch_bounds__(256) k_merge(const uint32_t* offset, uint32_t 
     st_xyzz(buckets, k, xyzz_inf<F>());
     return;
   }
-  uint32_t sa = s / SEG, sb = (e - 1) / SEG;
+  uint32_t sa = s / seg, sb = (e - 1) / seg;
   if (sa == sb) return;  // stored whole by the accumulation
   if (sb - sa > HEAVY_SPAN) {
     uint32_t interior = sb - sa - 1;
@@ -336,7 +343,8 @@ template <class F>
 __global__ void __launch_bounds__(256) k_merge_wide(const uint32_t* offset, uint32_t K,
                                                     uint4* buckets, const uint4* pfirst,
                                                     const uint4* plast, uint32_t* cnt,
-                                                    uint32_t* heavy, uint32_t* tasks) {
+                                                    uint32_t* heavy, uint32_t* tasks,
+                                                    uint32_t seg) {
   __shared__ Xyzz<F> sh[256];
   const uint32_t lane = threadIdx.x % MERGE_LANES;
   const uint32_t k = blockIdx.x * (256 / MERGE_LANES) + threadIdx.x / MERGE_LANES;
@@ -347,7 +355,7 @@ __global__ void __launch_bounds__(256) k_merge_wide(const uint32_t* offset, uint
     s = offset[k];
     e = offset[k + 1];
     if (s != e) {
-      const uint32_t sa = s / SEG, sb = (e - 1) / SEG;
+      const uint32_t sa = s / seg, sb = (e - 1) / seg;
       if (sb - sa > WIDE_SPAN) {
         if (lane == 0) {
           uint32_t interior = sb - sa - 1;
@@ -416,12 +424,13 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_finish(const uint32_t* offset,
                                                         const uint32_t* cnt,
                                                         const uint32_t* heavy,
                                                         const uint4* task_sum, uint4* buckets,
-                                                        const uint4* pfirst, const uint4* plast) {
+                                                        const uint4* pfirst, const uint4* plast,
+                                                        uint32_t seg) {
   __shared__ Xyzz<F> sh[BLOCK];
   uint32_t nh = cnt[0];
   for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
     uint32_t k = heavy[3 * h], tb = heavy[3 * h + 1], nt = heavy[3 * h + 2];
-    uint32_t sa = offset[k] / SEG, sb = (offset[k + 1] - 1) / SEG;
+    uint32_t sa = offset[k] / seg, sb = (offset[k + 1] - 1) / seg;
     Xyzz<F> acc = xyzz_inf<F>();
     for (uint32_t t = threadIdx.x; t < nt + 2; t += BLOCK) {
       const uint4* src = t < nt ? task_sum : (t == nt ? plast : pfirst);
@@ -535,7 +544,8 @@ struct AccumLayout {
 
 inline AccumLayout accum_layout(const MsmPlan& p, size_t pt) {
   size_t K = (size_t)msm_bucket_sets(p) * p.nb;
-  size_t nseg = (p.max_entries + SEG - 1) / SEG;
+  const uint32_t seg = accum_seg(p, pt);
+  size_t nseg = (p.max_entries + seg - 1) / seg;
   size_t max_heavy = nseg / HEAVY_SPAN + 2;
   size_t max_tasks = nseg / HEAVY_TASK + max_heavy + 2;
   uint32_t B = reduce_group_size(p);
@@ -570,6 +580,7 @@ inline size_t msm_accum_scratch_bytes_impl(const MsmPlan& p, int group) {
 struct AccumPtrs {
   uint4 *buckets, *pfirst, *plast, *task_sum, *gT, *gU, *parts;
   uint32_t *heavy, *tasks, *cnt;
+  uint32_t seg;  // entries per segment
 };
 
 // Launchers of the merge and reduction kernels over field F. The G1 ones are defined here;
@@ -592,18 +603,18 @@ inline void tail_merge_impl(const MsmPlan& p, const AccumPtrs& a, hipStream_t s)
     constexpr unsigned per_block = 256 / MERGE_LANES;
     hipLaunchKernelGGL(k_merge_wide<F>, dim3((unsigned)((K + per_block - 1) / per_block)),
                        dim3(256), 0, s, p.offset, (uint32_t)K, a.buckets, a.pfirst, a.plast,
-                       a.cnt, a.heavy, a.tasks);
+                       a.cnt, a.heavy, a.tasks, a.seg);
     G16_LAUNCH_CHECK("msm merge (wide)");
   } else {
     hipLaunchKernelGGL(k_merge<F>, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, s, p.offset,
-                       (uint32_t)K, a.buckets, a.pfirst, a.plast, a.cnt, a.heavy, a.tasks);
+                       (uint32_t)K, a.buckets, a.pfirst, a.plast, a.cnt, a.heavy, a.tasks, a.seg);
     G16_LAUNCH_CHECK("msm merge");
   }
   hipLaunchKernelGGL((k_heavy_tasks<F, HEAVY_TASK>), dim3(1024), dim3(HEAVY_TASK), 0, s, a.cnt,
                      a.tasks, a.pfirst, a.task_sum);
   G16_LAUNCH_CHECK("msm heavy tasks");
   hipLaunchKernelGGL((k_heavy_finish<F, 64>), dim3(256), dim3(64), 0, s, p.offset, a.cnt, a.heavy,
-                     a.task_sum, a.buckets, a.pfirst, a.plast);
+                     a.task_sum, a.buckets, a.pfirst, a.plast, a.seg);
   G16_LAUNCH_CHECK("msm heavy finish");
 }
 
@@ -635,7 +646,8 @@ template <class F>
 inline void accumulate_impl(const MsmPlan& p, const void* bases, void* scratch, void* window_out,
                             hipStream_t s, hipEvent_t* ev) {
   constexpr size_t PT = sizeof(Xyzz<F>);
-  size_t nseg = (p.max_entries + SEG - 1) / SEG;
+  const uint32_t seg = accum_seg(p, PT);
+  size_t nseg = (p.max_entries + seg - 1) / seg;
   uint32_t B = reduce_group_size(p);
   AccumLayout L = accum_layout(p, PT);
   char* q = static_cast<char*>(scratch);
@@ -650,16 +662,17 @@ inline void accumulate_impl(const MsmPlan& p, const void* bases, void* scratch, 
   a.gU = reinterpret_cast<uint4*>(q + L.gU);
   a.parts = reinterpret_cast<uint4*>(q + L.parts);
   a.cnt = reinterpret_cast<uint32_t*>(q + L.cnt);
+  a.seg = seg;
   auto bp = static_cast<const uint4*>(bases);
 
   if (ev) G16_CHECK(hipEventRecord(ev[0], s));
   if (nseg) {
     if (p.constant)
       hipLaunchKernelGGL((k_accumulate<F, true>), dim3((unsigned)((nseg + 127) / 128)), dim3(128),
-                         0, s, p.ent, p.total, nseg, bp, a.buckets, a.pfirst, a.plast);
+                         0, s, p.ent, p.total, nseg, seg, bp, a.buckets, a.pfirst, a.plast);
     else
       hipLaunchKernelGGL((k_accumulate<F, false>), dim3((unsigned)((nseg + 127) / 128)), dim3(128),
-                         0, s, p.ent, p.total, nseg, bp, a.buckets, a.pfirst, a.plast);
+                         0, s, p.ent, p.total, nseg, seg, bp, a.buckets, a.pfirst, a.plast);
     G16_LAUNCH_CHECK("msm accumulate");
   }
   if (ev) G16_CHECK(hipEventRecord(ev[1], s));
