@@ -139,10 +139,27 @@ g16hip_status g16hip_open(int ordinal, g16hip_device** out);
 void g16hip_close(g16hip_device* dev);
 /* Free device memory in bytes, the admission budget (crates/cuda/src/backend.rs:626-643). */
 g16hip_status g16hip_mem_info(g16hip_device* dev, size_t* free_bytes, size_t* total_bytes);
+/* CudaBackend::with_vram_limit (crates/cuda/src/backend.rs:120-130): caps the device memory a
+ * circuit prepared after this call (its resident key and one proof slot) and each further slot
+ * may plan against, below what the driver reports free (0 = no cap). A key over the budget is
+ * refused by g16hip_prepare with G16HIP_DEVICE_MEMORY before anything is allocated
+ * (crates/cuda/src/backend.rs:367-381). G16HIP_VRAM_LIMIT (bytes) sets the same at
+ * g16hip_open; G16HIP_VRAM_RESERVE (bytes) leaves that much of the free memory to others. */
+g16hip_status g16hip_set_vram_limit(g16hip_device* dev, size_t bytes);
+/* The budget an allocation may plan against now (device_budget, crates/cuda/src/msm.rs:1057-1091):
+ * free memory less a 256 MiB headroom and the reserve, capped by the limit. */
+g16hip_status g16hip_device_budget(g16hip_device* dev, size_t* bytes);
+/* resident_bytes (crates/cuda/src/backend.rs:626-643): device bytes `key` and one proof slot take
+ * once prepared, with (window_tables != 0) or without the per-window base tables, each buffer
+ * rounded up to 2 MiB pages. g16hip_prepare uses the tables only when the budget keeps 8 GiB
+ * free after them. */
+g16hip_status g16hip_key_resident_bytes(const g16hip_key* key, int window_tables, size_t* bytes);
 
 /* Backend::prepare (crates/groth16/src/lib.rs:371-378): validates the key
  * (crates/groth16/src/cpu.rs:73-122), uploads CSR, twiddles, coset table and the five base
- * vectors, and keeps them resident. The key is only read. */
+ * vectors, and keeps them resident. The key is only read. A key whose resident data and first
+ * proof slot exceed the device budget (g16hip_device_budget) is G16HIP_DEVICE_MEMORY with
+ * nothing allocated. */
 g16hip_status g16hip_prepare(g16hip_device* dev, const g16hip_key* key, g16hip_circuit** out);
 void g16hip_circuit_release(g16hip_circuit* c);
 /* PreparedCircuit::{n_vars, n_public, domain_size} (crates/groth16/src/lib.rs:382-387). */

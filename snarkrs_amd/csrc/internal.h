@@ -11,6 +11,7 @@ namespace g16 {
 // Throws on a HIP error; the C ABI layer turns it into G16HIP_DEVICE + message.
 struct HipError {
   std::string msg;
+  hipError_t code = hipErrorUnknown;
 };
 struct ArgError {
   std::string msg;

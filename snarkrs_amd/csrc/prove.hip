@@ -38,6 +38,11 @@ struct Slot {
   // here (no copy on the stream, which would wait for CUs behind other streams' kernels).
   uint8_t* host_wout = nullptr;
   uint8_t* host_wout_dev = nullptr;
+  // Page-locked staging for a host witness: the caller's (pageable) witness is copied here in
+  // chunks and each chunk goes up by DMA as soon as it is in (the reference's pinned witness
+  // staging, crates/cuda/src/stages.rs:946-985).
+  uint8_t* host_witness = nullptr;
+  size_t host_witness_bytes = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   size_t bytes = 0;
@@ -57,6 +62,7 @@ struct Slot {
       for (auto& e : row)
         if (e) (void)hipEventDestroy(e);
     if (host_wout) (void)hipHostFree(host_wout);
+    if (host_witness) (void)hipHostFree(host_witness);
     for (hipStream_t st : side)
       if (st && st != stream) (void)hipStreamDestroy(st);
     if (stream) (void)hipStreamDestroy(stream);
@@ -135,7 +141,64 @@ static void validate_key(const g16hip_key* k) {
   if (!k->gamma_g2 || !k->ic) throw ArgError{"the key has no verifying-key points (gamma_g2, ic)"};
 }
 
-static void upload_key(g16hip_circuit* C, const g16hip_key* k, hipStream_t s) {
+// Window widths, window tables, and the device bytes the key and one proof slot take, fixed
+// before anything is allocated (the reference's resident_bytes check at prepare,
+// crates/cuda/src/backend.rs:367-381, :626-643).
+struct KeyPlan {
+  int cw = 0, ch = 0;
+  bool merged = false;
+  size_t key_bytes = 0, slot_bytes = 0;
+  size_t total() const { return key_bytes + slot_bytes; }
+};
+
+static size_t slot_bytes_for(uint32_t nv, uint32_t n, uint32_t nh, int cw, int ch, bool merged);
+
+static KeyPlan plan_key(const g16hip_key* k, bool merged) {
+  const size_t n = k->domain_size, nv = k->n_vars, nh = k->h_basis == 1 ? n - 1 : n;
+  KeyPlan P;
+  P.merged = merged;
+  // The witness digits feed A, B1, L (G1) and B2 (G2); H feeds one G1 MSM.
+  // G16HIP_C_WITNESS / G16HIP_C_H override the widths (2..16).
+  P.cw = merged ? msm_choose_c_merged((uint32_t)nv, 3, 1) : msm_choose_c((uint32_t)nv, 3, 1);
+  P.ch = merged ? msm_choose_c_merged((uint32_t)nh, 1, 0) : msm_default_c((uint32_t)nh);
+  if (const char* e = std::getenv("G16HIP_C_WITNESS")) P.cw = std::max(2, std::min(16, std::atoi(e)));
+  if (const char* e = std::getenv("G16HIP_C_H")) P.ch = std::max(2, std::min(16, std::atoi(e)));
+  const size_t tw = merged ? (255 + P.cw - 1) / P.cw : 1, th = merged ? (255 + P.ch - 1) / P.ch : 1;
+  size_t b = 0;
+  if (k->row_ptr[0]) {
+    for (int m = 0; m < 2; ++m) {
+      const size_t nnz = k->row_ptr[m][n];
+      b += dev_pages(4 * (n + 1)) + 2 * dev_pages(4 * nnz);
+    }
+    b += dev_pages(32 * (size_t)k->n_values);
+  }
+  b += 3 * dev_pages(64 * nv * tw) + dev_pages(128 * nv * tw) + dev_pages(64 * nh * th);
+  // NTT tables: two twiddle tables, the coset table, the scalars, gnark's table, and the
+  // powers built on the way.
+  b += 2 * dev_pages(32 * (n > 1 ? n - 1 : 1)) + dev_pages(32 * n) + dev_pages(256) +
+       (k->h_basis == 1 ? dev_pages(32 * n) : 0) + dev_pages(32 * (n / 2 ? n / 2 : 1));
+  P.key_bytes = b;
+  P.slot_bytes = slot_bytes_for((uint32_t)nv, (uint32_t)n, (uint32_t)nh, P.cw, P.ch, merged);
+  return P;
+}
+
+// Window tables take W times the bases' memory; they are used when the device keeps 8 GiB of
+// its budget free after the key and a slot (G16HIP_WINDOW_TABLES=0 turns them off). A key
+// that does not fit even without them is refused before anything is allocated.
+static KeyPlan choose_plan(const g16hip_device* d, const g16hip_key* k) {
+  const size_t budget = device_budget(d);
+  const char* wt = std::getenv("G16HIP_WINDOW_TABLES");
+  if (!(wt && wt[0] == '0')) {
+    KeyPlan P = plan_key(k, true);
+    if (P.total() + (8ull << 30) <= budget) return P;
+  }
+  KeyPlan P = plan_key(k, false);
+  if (P.total() > budget)
+    throw too_big(P.total(), budget, "this key's resident data and one proof's stage scratch");
+  return P;
+}
+
+static void upload_key(g16hip_circuit* C, const g16hip_key* k, const KeyPlan& plan, hipStream_t s) {
   uint32_t n = k->domain_size;
   C->n_vars = k->n_vars;
   C->n_public = k->n_public;
@@ -164,30 +227,13 @@ static void upload_key(g16hip_circuit* C, const g16hip_key* k, hipStream_t s) {
   if (C->has_matrices) upload(C->values, k->values, 32 * (size_t)k->n_values, s);
   const size_t nh = C->n_h;
   size_t nv = k->n_vars;
-  // The witness digits feed A, B1, L (G1) and B2 (G2); H feeds one G1 MSM. Widths are chosen
-  // for window tables first and again per window if the tables do not fit.
-  // G16HIP_C_WITNESS / G16HIP_C_H override the widths (2..16).
-  C->cw = msm_choose_c_merged((uint32_t)nv, 3, 1);
-  C->ch = msm_choose_c_merged((uint32_t)nh, 1, 0);
-  if (const char* e = std::getenv("G16HIP_C_WITNESS")) C->cw = std::max(2, std::min(16, std::atoi(e)));
-  if (const char* e = std::getenv("G16HIP_C_H")) C->ch = std::max(2, std::min(16, std::atoi(e)));
-  int Ww = (255 + C->cw - 1) / C->cw, Wh = (255 + C->ch - 1) / C->ch;
   // Window tables: with entry w n + i = 2^(c w) P_i precomputed (the key's bases are fixed),
   // an MSM accumulates the digits of every window into one set of buckets, so the merges and
-  // the window reduction run once instead of once per window. They take W times the bases'
-  // memory and are used when the device keeps 8 GiB free after them (G16HIP_WINDOW_TABLES=0
-  // turns them off).
-  const size_t table_bytes = (3 * 64 + 128) * nv * Ww + 64 * nh * Wh;
-  size_t free_b = 0, total_b = 0;
-  G16_CHECK(hipMemGetInfo(&free_b, &total_b));
-  const char* wt = std::getenv("G16HIP_WINDOW_TABLES");
-  C->merged = !(wt && wt[0] == '0') && free_b > table_bytes + (8ull << 30);
-  if (!C->merged) {
-    if (!std::getenv("G16HIP_C_WITNESS")) C->cw = msm_choose_c((uint32_t)nv, 3, 1);
-    if (!std::getenv("G16HIP_C_H")) C->ch = msm_default_c((uint32_t)nh);
-    Ww = (255 + C->cw - 1) / C->cw;
-    Wh = (255 + C->ch - 1) / C->ch;
-  }
+  // the window reduction run once instead of once per window.
+  C->cw = plan.cw;
+  C->ch = plan.ch;
+  C->merged = plan.merged;
+  const int Ww = (255 + C->cw - 1) / C->cw, Wh = (255 + C->ch - 1) / C->ch;
   const size_t tw = C->merged ? (size_t)Ww : 1, th = C->merged ? (size_t)Wh : 1;
   auto put = [&](DevBuf& b, const void* src, size_t npts, size_t pt, size_t windows) {
     b.alloc(pt * npts * windows);
@@ -272,7 +318,10 @@ static void size_slot(const g16hip_circuit* C, Slot* S) {
   G16_CHECK(hipHostMalloc((void**)&S->host_wout, wbytes + 64,
                           hipHostMallocMapped | hipHostMallocCoherent));
   G16_CHECK(hipHostGetDevicePointer((void**)&S->host_wout_dev, S->host_wout, 0));
-  DevBuf* bufs[] = {&S->witness, &S->abc, &S->hbuf, &S->sort_w, &S->sort_h, &S->accum_w1,
+  S->host_witness_bytes = 32 * nv;
+  G16_CHECK(hipHostMalloc((void**)&S->host_witness, S->host_witness_bytes ? S->host_witness_bytes : 32,
+                          hipHostMallocDefault));
+  DevBuf* bufs[] ={&S->witness, &S->abc, &S->hbuf, &S->sort_w, &S->sort_h, &S->accum_w1,
                     &S->accum_w2, &S->accum_h};
   S->bytes = 0;
   for (auto b : bufs) S->bytes += b->bytes;
@@ -284,25 +333,20 @@ static std::unique_ptr<Slot> make_slot(const g16hip_circuit* C) {
   return S;
 }
 
-static size_t slot_bytes_estimate(const g16hip_circuit* C) {
-  size_t nv = C->n_vars, n = C->domain;
+// Device bytes of one slot (what size_slot allocates), rounded up to pages.
+static size_t slot_bytes_for(uint32_t nv, uint32_t n, uint32_t nh, int cw, int ch, bool merged) {
   MsmPlan pw, ph;
-  pw.c = C->cw;
-  pw.windows = (255 + C->cw - 1) / C->cw;
-  pw.n = (uint32_t)nv;
-  pw.nb = 1u << (C->cw - 1);
-  pw.max_entries = nv * pw.windows;
-  pw.merged = C->merged;
-  ph.c = C->ch;
-  ph.windows = (255 + C->ch - 1) / C->ch;
-  ph.n = C->n_h;
-  ph.nb = 1u << (C->ch - 1);
-  ph.max_entries = (size_t)C->n_h * ph.windows;
-  ph.merged = C->merged;
-  return 32 * nv + 32 * 4 * n + msm_sort_scratch_bytes((uint32_t)nv, C->cw) +
-         msm_sort_scratch_bytes(C->n_h, C->ch) + msm_accum_scratch_bytes(pw, 1) +
-         msm_accum_scratch_bytes(pw, 2) + msm_accum_scratch_bytes(ph, 1) +
-         256 * (4 * msm_out_points(pw) + msm_out_points(ph));
+  static char anchor;  // the plans are bound only to size their pieces
+  msm_plan_bind(pw, &anchor, nv, cw, merged);
+  msm_plan_bind(ph, &anchor, nh, ch, merged);
+  return dev_pages(32 * (size_t)nv) + dev_pages(32 * 3 * (size_t)n) + dev_pages(32 * (size_t)n) +
+         dev_pages(msm_sort_scratch_bytes(nv, cw)) + dev_pages(msm_sort_scratch_bytes(nh, ch)) +
+         dev_pages(msm_accum_scratch_bytes(pw, 1)) + dev_pages(msm_accum_scratch_bytes(pw, 2)) +
+         dev_pages(msm_accum_scratch_bytes(ph, 1));
+}
+
+static size_t slot_bytes_estimate(const g16hip_circuit* C) {
+  return slot_bytes_for(C->n_vars, C->domain, C->n_h, C->cw, C->ch, C->merged);
 }
 
 static bool try_add_slot(g16hip_circuit* C, size_t reserve);
@@ -360,12 +404,61 @@ static void check_witness_len(const g16hip_circuit* C, size_t n) {
                                                  std::to_string(C->n_vars)};
 }
 
+// A host witness goes up through the slot's page-locked staging in chunks: helper threads
+// copy chunks in, and the calling thread enqueues each chunk's DMA in order as soon as it is
+// in (copying chunks itself while it waits), so the host copy overlaps the transfer. The
+// staging is free at the start of a proof: a slot is leased by one proof at a time, and a
+// proof returns only after its streams have finished.
+static void stage_witness(Slot* S, const uint8_t* w, size_t bytes) {
+  constexpr size_t CHUNK = 2u << 20;
+  uint8_t* dst = S->host_witness;
+  char* dev = S->witness.as<char>();
+  const size_t nch = (bytes + CHUNK - 1) / CHUNK;
+  auto len = [&](size_t c) { return std::min(CHUNK, bytes - c * CHUNK); };
+  if (nch <= 2) {
+    std::memcpy(dst, w, bytes);
+    G16_CHECK(hipMemcpyAsync(dev, dst, bytes, hipMemcpyHostToDevice, S->stream));
+    return;
+  }
+  std::atomic<size_t> next{0};
+  std::unique_ptr<std::atomic<uint8_t>[]> ready(new std::atomic<uint8_t>[nch]);
+  for (size_t c = 0; c < nch; ++c) ready[c].store(0, std::memory_order_relaxed);
+  auto copy_one = [&]() -> bool {
+    const size_t c = next.fetch_add(1, std::memory_order_relaxed);
+    if (c >= nch) return false;
+    std::memcpy(dst + c * CHUNK, w + c * CHUNK, len(c));
+    ready[c].store(1, std::memory_order_release);
+    return true;
+  };
+  std::vector<std::thread> helpers;
+  const size_t nh = std::min<size_t>(3, nch - 1);
+  for (size_t h = 0; h < nh; ++h)
+    helpers.emplace_back([&] {
+      while (copy_one()) {
+      }
+    });
+  std::exception_ptr err;
+  try {
+    for (size_t c = 0; c < nch; ++c) {
+      while (!ready[c].load(std::memory_order_acquire))
+        if (!copy_one()) std::this_thread::yield();
+      G16_CHECK(hipMemcpyAsync(dev + c * CHUNK, dst + c * CHUNK, len(c), hipMemcpyHostToDevice,
+                               S->stream));
+    }
+  } catch (...) {
+    err = std::current_exception();
+  }
+  for (auto& t : helpers) t.join();
+  if (err) std::rethrow_exception(err);
+}
+
 static void load_witness(const g16hip_circuit* C, Slot* S, const void* w, size_t n,
                          bool on_device) {
   check_witness_len(C, n);
-  G16_CHECK(hipMemcpyAsync(S->witness.p, w, 32 * n,
-                           on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                           S->stream));
+  if (on_device)
+    G16_CHECK(hipMemcpyAsync(S->witness.p, w, 32 * n, hipMemcpyDeviceToDevice, S->stream));
+  else
+    stage_witness(S, static_cast<const uint8_t*>(w), 32 * n);
 }
 
 // Stage 0 handed over by the caller: constraint i's A.w and B.w (host memory), i < n.
@@ -764,9 +857,7 @@ namespace g16 {
 // Adds one slot to `C` if the device keeps `reserve` bytes free after it (admission by free
 // device memory); returns whether it did.
 static bool try_add_slot(g16hip_circuit* C, size_t reserve) {
-  size_t free_b = 0, total_b = 0;
-  G16_CHECK(hipMemGetInfo(&free_b, &total_b));
-  if (free_b < slot_bytes_estimate(C) + reserve) return false;
+  if (device_budget(C->dev) < slot_bytes_estimate(C) + reserve) return false;
   auto S = make_slot(C);
   std::lock_guard<std::mutex> lk(C->mu);
   C->slots.push_back(std::move(S));
@@ -842,14 +933,25 @@ g16hip_status g16hip_prepare(g16hip_device* d, const g16hip_key* key, g16hip_cir
       std::lock_guard<std::mutex> lk(d->mu);
       C->constant = d->constant_work;
     }
+    // Refused for device memory before anything is allocated (streams included).
+    const KeyPlan plan = choose_plan(d, key);
     // The key goes up on slot 0's main stream (a failure destroys the slot, which waits for
     // its streams first).
     auto S = new_slot();
-    upload_key(C.get(), key, S->stream);
+    upload_key(C.get(), key, plan, S->stream);
     size_slot(C.get(), S.get());
     C->slots.push_back(std::move(S));
     C->busy.push_back(false);
     *out = C.release();
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_key_resident_bytes(const g16hip_key* key, int window_tables, size_t* bytes) {
+  return guarded([&] {
+    if (!bytes) throw ArgError{"null argument"};
+    validate_key(key);
+    *bytes = plan_key(key, window_tables != 0).total();
     return G16HIP_OK;
   });
 }

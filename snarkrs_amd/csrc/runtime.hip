@@ -10,7 +10,11 @@ namespace g16 {
 
 void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess) {
-    throw HipError{std::string(what) + ": " + hipGetErrorString(e)};
+    // HIP keeps the last error a runtime call returned and hipGetLastError reports it; read it
+    // here so that a failure this library caught (an allocation refused, then handled) is not
+    // reported again by the next launch check on this thread.
+    (void)hipGetLastError();
+    throw HipError{std::string(what) + ": " + hipGetErrorString(e), e};
   }
 }
 
@@ -68,24 +72,40 @@ NttTables& ntt_tables_for(g16hip_device* d, int L) {
   return ref;
 }
 
-// One MSM on device-resident bases and scalars with scratch from `d`.
-template <class F>
-static Xyzz<F> run_msm(g16hip_device* d, int group, const void* bases, const void* scalars,
-                       uint32_t n, int c) {
+static size_t round256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+MsmScratchSizes msm_scratch_sizes(int group, uint32_t n, int c) {
+  MsmPlan plan;
+  static char anchor;  // the plan is bound only to size its pieces
+  msm_plan_bind(plan, &anchor, n, c);
+  const size_t pt = group == 1 ? 128 : 256;  // XYZZ point
+  MsmScratchSizes z;
+  z.sort = round256(msm_sort_scratch_bytes(n, c));
+  z.accum = round256(msm_accum_scratch_bytes(plan, group) + pt * msm_out_points(plan));
+  z.bases = round256(32 * (size_t)n * (group == 1 ? 2 : 4));
+  return z;
+}
+
+int msm_width(uint32_t n, int c) {
   if (c <= 0) c = msm_default_c(n);
   if (c < 2 || c > 16) throw ArgError{"window width " + std::to_string(c) + " is outside [2, 16]"};
+  return c;
+}
+
+// One MSM on device-resident bases and scalars with the given scratch (d's stream, events,
+// statistics).
+template <class F>
+static Xyzz<F> run_msm_on(g16hip_device* d, int group, const void* bases, const void* scalars,
+                          uint32_t n, int c, const MsmScratch& sc) {
   MsmPlan plan;
-  d->scratch_d.alloc(msm_sort_scratch_bytes(n, c));
-  msm_plan_bind(plan, d->scratch_d.p, n, c, false, d->constant_work);
+  msm_plan_bind(plan, sc.sort, n, c, false, d->constant_work);
   const size_t npts = msm_out_points(plan);
-  d->scratch_e.alloc(msm_accum_scratch_bytes(plan, group) + sizeof(Xyzz<F>) * npts);
   const size_t coords = (size_t)n * (group == 1 ? 2 : 4);
-  d->msm_bases.alloc(32 * coords);
   G16_CHECK(hipEventRecord(d->ev[0], d->stream));
-  msm_to_msm_form(bases, d->msm_bases.p, coords, d->stream);
+  msm_to_msm_form(bases, sc.bases, coords, d->stream);
   msm_sort(plan, scalars, d->stream);
-  char* wout = static_cast<char*>(d->scratch_e.p) + msm_accum_scratch_bytes(plan, group);
-  msm_accumulate(plan, group, d->msm_bases.p, d->scratch_e.p, wout, d->stream, d->ev + 1);
+  char* wout = static_cast<char*>(sc.accum) + msm_accum_scratch_bytes(plan, group);
+  msm_accumulate(plan, group, sc.bases, sc.accum, wout, d->stream, d->ev + 1);
   std::vector<Xyzz<F>> S(npts);
   uint32_t entries = 0;
   G16_CHECK(hipMemcpyAsync(S.data(), wout, sizeof(Xyzz<F>) * npts, hipMemcpyDeviceToHost,
@@ -104,6 +124,27 @@ static Xyzz<F> run_msm(g16hip_device* d, int group, const void* bases, const voi
   st.windows = (uint32_t)plan.windows;
   note_accumulate(d, group, entries, st.accumulate_ms);
   return msm_combine(S.data(), plan);
+}
+
+// The same with scratch the device owns and grows as needed.
+template <class F>
+static Xyzz<F> run_msm(g16hip_device* d, int group, const void* bases, const void* scalars,
+                       uint32_t n, int c) {
+  c = msm_width(n, c);
+  const MsmScratchSizes z = msm_scratch_sizes(group, n, c);
+  d->scratch_d.alloc(z.sort);
+  d->scratch_e.alloc(z.accum);
+  d->msm_bases.alloc(z.bases);
+  return run_msm_on<F>(d, group, bases, scalars, n, c,
+                       MsmScratch{d->scratch_d.p, d->scratch_e.p, d->msm_bases.p});
+}
+
+// One device MSM on caller-provided scratch, result affine in the wire encoding (the caller
+// holds d->mu and has made d's device current).
+void msm_device_on_scratch(g16hip_device* d, int group, const void* bases, const void* scalars,
+                           uint32_t n, int c, const MsmScratch& sc, uint8_t* out) {
+  if (group == 1) g1_store(out, host_to_aff(run_msm_on<Fq>(d, 1, bases, scalars, n, c, sc)));
+  else g2_store(out, host_to_aff(run_msm_on<Fq2>(d, 2, bases, scalars, n, c, sc)));
 }
 
 
@@ -223,8 +264,7 @@ static Xyzz<F> ceremony_msm(g16hip_ceremony* cer, int group, const uint8_t* base
       if (e.st != G16HIP_DEVICE_MEMORY || m <= CEREMONY_MIN_RUN) throw;
       len = std::max(m / 2, CEREMONY_MIN_RUN);
     } catch (const HipError& e) {
-      bool oom = e.msg.find("out of memory") != std::string::npos;
-      if (!oom || m <= CEREMONY_MIN_RUN) throw;
+      if (e.code != hipErrorOutOfMemory || m <= CEREMONY_MIN_RUN) throw;
       len = std::max(m / 2, CEREMONY_MIN_RUN);
     }
   }
@@ -321,8 +361,19 @@ g16hip_status g16hip_open(int ordinal, g16hip_device** out) {
     if (ordinal < 0 || ordinal >= n)
       throw ArgError{"device ordinal " + std::to_string(ordinal) + " out of range (" +
                      std::to_string(n) + " devices)"};
+    auto bytes_env = [](const char* name) -> size_t {
+      const char* v = std::getenv(name);
+      if (!v || !*v) return 0;
+      char* end = nullptr;
+      unsigned long long x = std::strtoull(v, &end, 10);
+      if (*end) throw ArgError{std::string(name) + "=" + v + " is not a byte count"};
+      return (size_t)x;
+    };
+    const size_t limit = bytes_env("G16HIP_VRAM_LIMIT"), reserve = bytes_env("G16HIP_VRAM_RESERVE");
     auto* d = new g16hip_device();
     d->ordinal = ordinal;
+    d->vram_limit = limit;
+    d->vram_reserve = reserve;
     G16_CHECK(hipSetDevice(ordinal));
     G16_CHECK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
     for (auto& e : d->ev) G16_CHECK(hipEventCreate(&e));
@@ -350,8 +401,26 @@ void g16hip_close(g16hip_device* d) {
 
 g16hip_status g16hip_mem_info(g16hip_device* d, size_t* free_b, size_t* total_b) {
   return guarded([&] {
+    if (!d || !free_b || !total_b) throw ArgError{"null argument"};
     set_device(d);
     G16_CHECK(hipMemGetInfo(free_b, total_b));
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_set_vram_limit(g16hip_device* d, size_t bytes) {
+  return guarded([&] {
+    if (!d) throw ArgError{"null device"};
+    d->vram_limit = bytes;
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_device_budget(g16hip_device* d, size_t* bytes) {
+  return guarded([&] {
+    if (!d || !bytes) throw ArgError{"null argument"};
+    set_device(d);
+    *bytes = device_budget(d);
     return G16HIP_OK;
   });
 }

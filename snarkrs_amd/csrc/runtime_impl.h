@@ -1,6 +1,7 @@
 // Host-side types shared by runtime.hip (devices, standalone entry points) and prove.hip
 // (prepared circuits, proofs, batches, the server).
 #pragma once
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <exception>
@@ -35,8 +36,10 @@ g16hip_status guarded(F&& f) {
     set_last_error("");
     return f();
   } catch (const HipError& e) {
-    bool oom = e.msg.find("out of memory") != std::string::npos;
-    return fail(oom ? G16HIP_DEVICE_MEMORY : G16HIP_DEVICE, e.msg);
+    // Sizes are checked against the device budget before anything is allocated (a refusal
+    // there is a StatusError); an allocation the driver still refuses afterwards (memory
+    // taken by another process in between) is reported as a memory refusal too.
+    return fail(e.code == hipErrorOutOfMemory ? G16HIP_DEVICE_MEMORY : G16HIP_DEVICE, e.msg);
   } catch (const ArgError& e) {
     return fail(G16HIP_BACKEND, e.msg);
   } catch (const StatusError& e) {
@@ -378,9 +381,37 @@ struct g16hip_device {
   std::mutex stats_mu;
   g16hip_kernel_stats kstats = {};
   bool constant_work = false;  // MsmPlan::constant for standalone MSMs and new circuits
+  // Cap on the device memory a circuit prepared afterwards (its key and first slot) and a
+  // further slot may plan against; 0 = none (g16hip_set_vram_limit, G16HIP_VRAM_LIMIT).
+  std::atomic<size_t> vram_limit{0};
+  size_t vram_reserve = 0;  // bytes left to other users (G16HIP_VRAM_RESERVE)
 };
 
 namespace g16 {
+
+// Bytes an allocation may take now: free device memory less a headroom for the runtime and
+// the reserve, capped by the device's limit (the reference's device_budget,
+// crates/cuda/src/msm.rs:1057-1091).
+constexpr size_t VRAM_HEADROOM = 256ull << 20;
+inline size_t device_budget(const g16hip_device* d) {
+  size_t free_b = 0, total_b = 0;
+  G16_CHECK(hipMemGetInfo(&free_b, &total_b));
+  size_t budget = free_b > VRAM_HEADROOM + d->vram_reserve ? free_b - VRAM_HEADROOM - d->vram_reserve : 0;
+  const size_t limit = d->vram_limit.load();
+  return limit && limit < budget ? limit : budget;
+}
+
+// Device allocations are rounded up to the pages the driver maps them in when planned.
+inline size_t dev_pages(size_t bytes) {
+  constexpr size_t PAGE = 2ull << 20;
+  return (bytes + PAGE - 1) / PAGE * PAGE;
+}
+
+inline StatusError too_big(size_t need, size_t budget, const std::string& what) {
+  return StatusError{G16HIP_DEVICE_MEMORY, what + " needs " + std::to_string(need) +
+                                               " bytes of device memory and " +
+                                               std::to_string(budget) + " are available to it"};
+}
 
 inline void note_accumulate(g16hip_device* d, int group, uint64_t entries, double ms) {
   std::lock_guard<std::mutex> lk(d->stats_mu);
@@ -390,5 +421,26 @@ inline void note_accumulate(g16hip_device* d, int group, uint64_t entries, doubl
 }
 
 inline void set_device(const g16hip_device* d) { G16_CHECK(hipSetDevice(d->ordinal)); }
+
+// Device scratch of one standalone MSM: the sort's, the accumulation's (window sums behind
+// it) and the bases in the MSM kernels' form.
+struct MsmScratch {
+  void* sort = nullptr;
+  void* accum = nullptr;
+  void* bases = nullptr;
+};
+
+// Bytes of each piece for n points at width c, each rounded up to 256 bytes.
+struct MsmScratchSizes {
+  size_t sort = 0, accum = 0, bases = 0;
+  size_t total() const { return sort + accum + bases; }
+};
+MsmScratchSizes msm_scratch_sizes(int group, uint32_t n, int c);
+// c, or the library's width for n when c <= 0; throws outside [2, 16].
+int msm_width(uint32_t n, int c);
+// One device MSM (group 1 or 2) on caller-provided scratch, result affine in the wire
+// encoding; the caller holds d->mu with d's device current.
+void msm_device_on_scratch(g16hip_device* d, int group, const void* bases, const void* scalars,
+                           uint32_t n, int c, const MsmScratch& sc, uint8_t* out);
 
 }  // namespace g16

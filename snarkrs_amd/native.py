@@ -199,6 +199,10 @@ EXPORTS = {
     "g16hip_close": (None, [_vp]),
     "g16hip_mem_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t),
                                        ctypes.POINTER(ctypes.c_size_t)]),
+    "g16hip_set_vram_limit": (ctypes.c_int, [_vp, ctypes.c_size_t]),
+    "g16hip_device_budget": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
+    "g16hip_key_resident_bytes": (ctypes.c_int, [ctypes.POINTER(KeyStruct), ctypes.c_int,
+                                                 ctypes.POINTER(ctypes.c_size_t)]),
     "g16hip_prepare": (ctypes.c_int, [_vp, ctypes.POINTER(KeyStruct), ctypes.POINTER(_vp)]),
     "g16hip_circuit_release": (None, [_vp]),
     "g16hip_circuit_dims": (ctypes.c_int, [_vp] + [ctypes.POINTER(ctypes.c_uint32)] * 3),
@@ -338,6 +342,17 @@ class Device:
         _check(self.lib.g16hip_mem_info(self.h, ctypes.byref(f), ctypes.byref(t)))
         return f.value, t.value
 
+    def set_vram_limit(self, nbytes: int) -> None:
+        """``with_vram_limit`` (crates/cuda/src/backend.rs:120-130): cap what circuits
+        prepared afterwards may plan against (0 = no cap)."""
+        _check(self.lib.g16hip_set_vram_limit(self.h, nbytes))
+
+    def budget(self) -> int:
+        """Bytes an allocation may plan against now (free less headroom, capped)."""
+        b = ctypes.c_size_t()
+        _check(self.lib.g16hip_device_budget(self.h, ctypes.byref(b)))
+        return b.value
+
     def field_probe(self, field: int, op: int, a: np.ndarray, b: np.ndarray | None = None):
         w = 64 if field == 2 else 32
         a = _c8(a).reshape(-1, w)
@@ -408,6 +423,15 @@ class Device:
 
     def prepare(self, key: ProvingKey) -> "Circuit":
         return Circuit(self, key)
+
+
+def key_resident_bytes(key: ProvingKey, window_tables: bool = False) -> int:
+    """Device bytes ``key`` and one proof slot take once prepared (``resident_bytes``,
+    crates/cuda/src/backend.rs:626-643), with or without the per-window base tables."""
+    st = key.struct()
+    b = ctypes.c_size_t()
+    _check(load().g16hip_key_resident_bytes(ctypes.byref(st), int(window_tables), ctypes.byref(b)))
+    return b.value
 
 
 class HandleH:

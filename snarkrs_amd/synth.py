@@ -67,6 +67,28 @@ LARGE_1_1M = Workload(n_constraints=1_100_000, n_public=2, n_private_in=100_000,
                       witness_seed=23)
 
 
+def mixed_workloads(count: int, lo: float, hi: float, seed: int, base_seed: int) -> list:
+    """`count` circuits whose log2 constraint count is drawn uniformly in [lo, hi] (seeded
+    SplitMix64): 40% boolean witness, 3 nonzeros per A/B row, about 9% of the wires private
+    inputs. The batch workload is (4, 14, 20, seed=3, base_seed=500), the serving one
+    (8, 16, 22, seed=4, base_seed=700)."""
+    from .wire import SplitMix64
+
+    rng = SplitMix64(seed)
+    out = []
+    for i in range(count):
+        x = lo + (hi - lo) * (rng.next_u64() / 2.0**64)
+        nc = int(2.0**x)
+        out.append(Workload(n_constraints=nc, n_public=2, n_private_in=max(8, nc // 11),
+                            bool_permille=400, input_bool_permille=400, seed=base_seed + i,
+                            setup_seed=base_seed + 100 + i, witness_seed=base_seed + 200 + i))
+    return out
+
+
+BATCH_WORKLOADS = dict(count=4, lo=14.0, hi=20.0, seed=3, base_seed=500)
+SERVE_WORKLOADS = dict(count=8, lo=16.0, hi=22.0, seed=4, base_seed=700)
+
+
 class Synth:
     def __init__(self, w: Workload):
         self.lib = load()
