@@ -299,11 +299,30 @@ typedef struct {
  * also be set with G16HIP_GRAPHS=1. Results are bit-identical either way. */
 void g16hip_set_graph_replay(int on);
 
-/* Proves every item (any mix of circuits) with all slots of each circuit busy at once.
- * proofs[i] is bit-identical to g16hip_prove_with_blinders on item i. statuses (optional)
- * receives each item's status; the return value is the first failure's. */
+/* Proves every item (any mix of circuits). Each circuit's items run in waves of up to its
+ * batch cap: a wave is up to cap proofs of that circuit proved together, every stage one launch
+ * over all of them (their witnesses staged and uploaded together, one fused gather, the
+ * transforms of all their vectors, each MSM's sort / accumulation / merge / reduction over all
+ * their scalar vectors with a bucket set per proof), on one stream per circuit, the host
+ * finishing wave i while the device runs wave i + 1. Circuits run concurrently. Keys without
+ * matrices or with gnark's H basis prove item by item. proofs[i] is bit-identical to
+ * g16hip_prove_with_blinders on item i. statuses (optional) receives each item's status; the
+ * return value is the first failure's. A wave larger than the device budget is refused as
+ * G16HIP_DEVICE_MEMORY before anything is allocated. */
 g16hip_status g16hip_prove_batch(const g16hip_batch_item* items, size_t n, g16hip_proof* proofs,
                                  g16hip_status* statuses);
+/* The batch cap (proofs per wave) of circuit c; 0 = the library's default (about 2^21 transform
+ * elements per wave, at most 16 proofs). At most 64. */
+g16hip_status g16hip_set_batch_cap(g16hip_circuit* c, uint32_t cap);
+/* Measures batched throughput of circuit c on the given witness (Fr[n_witness] Montgomery,
+ * host memory) for caps 1, 2, 4, ... up to max_cap (0 = 32) while the device budget holds a wave,
+ * stopping after two caps that do not improve on the best by 2%, and sets the best as c's cap
+ * (*best). caps[i] / rates[i] (optional, len entries) receive each cap tried and its proofs/s;
+ * *tried (optional) the number tried. The reference has no batching; this replaces its fixed
+ * launch shapes (crates/cuda/src/msm.rs:744-853) for batches by a measurement per (n, batch). */
+g16hip_status g16hip_batch_autotune(g16hip_circuit* c, const uint8_t* witness, size_t n_witness,
+                                    uint32_t max_cap, uint32_t* caps, double* rates, size_t len,
+                                    size_t* tried, uint32_t* best);
 
 /* A serving loop with continuous batching. `lanes` proofs are in flight at once (a host
  * thread each; 1..64): a free lane takes the oldest request of any circuit, leases a slot of

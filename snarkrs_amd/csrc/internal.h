@@ -48,6 +48,11 @@ void launch_fr_to_rp(void* data, size_t n, hipStream_t s);  // R form -> R' form
 void ntt_dif_inverse(void* data, int nvec, int log_n, const void* tw_inv, hipStream_t s);
 void ntt_dit_forward(void* data, int nvec, int log_n, const void* tw_fwd, const void* prescale,
                      hipStream_t s);
+// ntt_dif_inverse then ntt_dit_forward with `coset` as the prescale, the inverse's lowest pass
+// and the forward's first pass fused into one (the same tiles, in LDS). With h_out (nvec = 3:
+// A, B, C back to back), H = A B - C goes to h_out from the last pass and C is not stored.
+void ntt_coset_pipeline(void* data, int nvec, int log_n, const void* tw_inv, const void* tw_fwd,
+                        const void* coset, hipStream_t s, void* h_out = nullptr);
 void launch_bitrev_permute(void* dst, const void* src, int log_n, hipStream_t s);
 void launch_scale(void* data, size_t n, const void* factor, hipStream_t s);
 void launch_scale_rp(void* data, size_t n, const void* factor_rp, hipStream_t s);
@@ -57,6 +62,12 @@ void launch_mul_table_rp(void* data, size_t n, const void* table_rp, hipStream_t
 void launch_gather(const uint32_t* row_ptr, const uint32_t* signal, const uint32_t* value_id,
                    const void* values, const void* witness, void* out, uint32_t n_rows,
                    hipStream_t s);
+// A = A.w, B = B.w and C = A o B row by row in one kernel (matrices 0 = A, 1 = B), for
+// `batch` witnesses of nv elements back to back; proof k's (A, B, C) go to abc + 3 k n_rows.
+void launch_gather_abc(const uint32_t* const row_ptr[2], const uint32_t* const signal[2],
+                       const uint32_t* const value_id[2], const void* values,
+                       const void* witness, size_t nv, void* abc, uint32_t n_rows,
+                       uint32_t batch, hipStream_t s);
 void launch_mul_into(void* out, const void* a, const void* b, size_t n, hipStream_t s);
 void launch_h(void* h, const void* a, const void* b, const void* c, size_t n, hipStream_t s);
 void launch_rows_abc(void* a, void* b, void* c, size_t rows, size_t n, hipStream_t s);
@@ -65,9 +76,13 @@ void launch_rows_abc(void* a, void* b, void* c, size_t rows, size_t n, hipStream
 struct MsmPlan {
   int c = 0;              // window width
   int windows = 0;        // ceil(255 / c)
-  uint32_t n = 0;         // scalars
+  uint32_t n = 0;         // scalars per vector
   uint32_t nb = 0;        // buckets per window, 2^(c-1)
-  size_t max_entries = 0; // n * windows
+  // Scalar vectors sorted and accumulated together against the same bases (the proofs of one
+  // batch): vector k is scalars [k n, (k + 1) n), digits are laid out per (k, window), and
+  // every vector has bucket sets of its own (set index k, or k windows + w per window).
+  uint32_t batch = 1;
+  size_t max_entries = 0; // batch * n * windows
   uint32_t chunks = 0;    // scalar chunks of the counting sort
   // Merged windows: the bases come as window tables (entry w n + i = 2^(c w) P_i), so the
   // digits of every window fall into one set of nb buckets and the window sum is the MSM.
@@ -78,21 +93,31 @@ struct MsmPlan {
   // Entry values are then index << 2 | discard << 1 | sign (index < 2^30).
   bool constant = false;
   // device scratch (owned by the caller)
-  uint32_t* count = nullptr;    // [windows*nb + 1] entries per bucket
-  uint32_t* offset = nullptr;   // [windows*nb + 1] first entry of each bucket; [K] = total
+  uint32_t* count = nullptr;    // [batch*windows*nb + 1] entries per (vector, window, bucket)
+  uint32_t* offset = nullptr;   // [K + 1] first entry of each bucket (K = sets * nb); [K] = total
   uint32_t* block_sums = nullptr;
   uint2* ent = nullptr;         // [max_entries] sorted entries: {key = window * nb + bucket,
                                 //                 value = index << 1 | sign}
-  uint32_t* digits = nullptr;   // [windows * n] encoded signed digits, window-major
-  uint32_t* chunk_count = nullptr;  // [windows][chunks][nb] per-chunk counts, then prefixes
-  uint32_t* btotal = nullptr;   // [nb + 1] merged: entries per bucket over all windows
+  uint32_t* digits = nullptr;   // [windows * n] encoded signed digits (16 or 32-bit), window-major
+  uint32_t* cursor = nullptr;   // [batch*windows*nb] the scatter's next position per key
+  uint32_t* btotal = nullptr;   // [batch*nb + 1] merged: entries per bucket over all windows
   uint32_t* total = nullptr;    // [4] entry count
 };
-// Bucket sets the accumulation and reduction work on: one per window, or one when merged.
-inline int msm_bucket_sets(const MsmPlan& p) { return p.merged ? 1 : p.windows; }
-size_t msm_sort_scratch_bytes(uint32_t n, int c);
+// Bucket sets the accumulation and reduction work on: one per window, or one when merged, for
+// each scalar vector.
+inline int msm_bucket_sets(const MsmPlan& p) {
+  return (int)p.batch * (p.merged ? 1 : p.windows);
+}
+// The plan as seen by one vector's host finish (its rows are msm_out_points of this view).
+inline MsmPlan msm_vector_view(const MsmPlan& p) {
+  MsmPlan v = p;
+  v.batch = 1;
+  v.max_entries = (size_t)p.n * p.windows;
+  return v;
+}
+size_t msm_sort_scratch_bytes(uint32_t n, int c, uint32_t batch = 1);
 void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c, bool merged = false,
-                   bool constant = false);
+                   bool constant = false, uint32_t batch = 1);
 // Window tables for merged MSMs: `table` holds n affine bases (MSM form) at [0, n); fills
 // entries w n + i = 2^(c w) P_i for 1 <= w < windows. group 1 = G1, 2 = G2.
 void msm_window_tables(int group, void* table, size_t n, int c, int windows, hipStream_t s);
@@ -110,6 +135,13 @@ size_t msm_accum_scratch_bytes(const MsmPlan& p, int group);
 // merges and after the window reduction.
 void msm_accumulate(const MsmPlan& p, int group, const void* bases, void* scratch,
                     void* window_out, hipStream_t s, hipEvent_t* ev = nullptr);
+// The G1 bucket accumulations of n <= 4 MSMs (plans[k] sorted, bases[k] in the MSM form,
+// scratch[k] of msm_accum_scratch_bytes) as ONE persistent launch whose workgroups take tiles
+// of segments of every job from a device-side counter (a device word of the caller, 16 bytes,
+// cleared by a kernel here), then each MSM's merges and reduction into window_out[k].
+void msm_accumulate_g1_queue(const MsmPlan* const* plans, const void* const* bases,
+                             void* const* scratch, void* const* window_out, int n,
+                             uint32_t* counter, hipStream_t s);
 void msm_reduce_shape(const MsmPlan& p, int* rows, int* lg);
 // The MSM kernels keep coordinates in Montgomery form with R' = 2^261 (bn254.cuh Fm): bases go
 // in as 32 x mod q. dst may equal src; n_fq counts 32-byte coordinates.

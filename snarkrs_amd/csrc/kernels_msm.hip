@@ -53,35 +53,65 @@ __device__ __forceinline__ int signed_digit(const uint32_t s[8], int i, int c) {
   return d + carry;
 }
 
-// Digit encoding: (|d| - 1) | sign << 16, or DIGIT_ZERO for d = 0; under constant work a zero
-// digit is (i mod nb) | DIGIT_DISCARD instead, an entry whose addition is computed and dropped.
+// Digit codes, one per (window, scalar), window-major. Two widths:
+//   32-bit (c = 16, or constant work): (|d| - 1) | sign << 16, DIGIT_ZERO for d = 0; under
+//     constant work a zero digit is (i mod nb) | DIGIT_DISCARD instead, an entry whose addition
+//     is computed and dropped;
+//   16-bit (c <= 15, variable work): (|d| - 1) | sign << 15, 0x7FFF for d = 0 (|d| - 1 is at
+//     most 2^14 - 1 there), half the digit traffic.
 static constexpr uint32_t DIGIT_ZERO = 1u << 17;
 static constexpr uint32_t DIGIT_DISCARD = 1u << 18;
+
+template <class D> struct Digit;
+template <> struct Digit<uint32_t> {
+  static __device__ __forceinline__ uint32_t make(int d, uint32_t zero) {
+    return d == 0 ? zero : ((uint32_t)(d < 0 ? -d : d) - 1u) | (d < 0 ? 1u << 16 : 0u);
+  }
+  static __device__ __forceinline__ bool zero(uint32_t e) { return (e & DIGIT_ZERO) != 0; }
+  static __device__ __forceinline__ uint32_t bucket(uint32_t e) { return e & 0xFFFFu; }
+  static __device__ __forceinline__ uint32_t sign(uint32_t e) { return (e >> 16) & 1u; }
+  static __device__ __forceinline__ uint32_t discard(uint32_t e) {
+    return (e & DIGIT_DISCARD) ? 2u : 0u;
+  }
+};
+template <> struct Digit<uint16_t> {
+  static __device__ __forceinline__ uint16_t make(int d, uint32_t) {
+    return (uint16_t)(d == 0 ? 0x7FFFu
+                             : ((uint32_t)(d < 0 ? -d : d) - 1u) | (d < 0 ? 1u << 15 : 0u));
+  }
+  static __device__ __forceinline__ bool zero(uint16_t e) { return e == 0x7FFFu; }
+  static __device__ __forceinline__ uint32_t bucket(uint16_t e) { return e & 0x7FFFu; }
+  static __device__ __forceinline__ uint32_t sign(uint16_t e) { return (uint32_t)e >> 15; }
+  static __device__ __forceinline__ uint32_t discard(uint16_t) { return 0; }
+};
+
+// Whether a plan's digits are 16-bit.
+static bool short_digits(const MsmPlan& p) { return p.c <= 15 && !p.constant; }
+
 // Scalars per counting-sort workgroup: a bucket's count within a chunk fits 16 bits, so the
-// LDS histogram packs two buckets per word (64 KB at c = 16).
+// count kernel's LDS histogram packs two buckets per word (64 KB at c = 16).
 static constexpr uint32_t SORT_CHUNK = 32768;
 static constexpr int SORT_THREADS = 1024;
 static constexpr uint32_t SCATTER_SPAN = 16384;  // buckets per scatter workgroup (64 KB LDS)
 
-__global__ void __launch_bounds__(256) k_digits(const uint4* scalars, uint32_t n, int c, int W,
-                                                uint32_t* digits, int constant) {
+template <class D>
+__global__ void __launch_bounds__(256) k_digits(const uint4* scalars, uint32_t n, uint32_t total,
+                                                int c, int W, D* digits, int constant) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= total) return;
+  // scalar i is index j of vector k: its digits go to rows (k W + w) of n
+  const uint32_t k = i / n, j = i - k * n;
   Fr s = to_std(ld_fr(scalars + 2 * (size_t)i));
-  const uint32_t zero = constant ? (i & ((1u << (c - 1)) - 1u)) | DIGIT_DISCARD : DIGIT_ZERO;
-  for (int w = 0; w < W; ++w) {
-    int d = signed_digit(s.v, w, c);
-    uint32_t e = d == 0 ? zero
-                        : ((uint32_t)(d < 0 ? -d : d) - 1u) | (d < 0 ? 1u << 16 : 0u);
-    digits[(size_t)w * n + i] = e;
-  }
+  const uint32_t zero = constant ? (j & ((1u << (c - 1)) - 1u)) | DIGIT_DISCARD : DIGIT_ZERO;
+  D* row = digits + (size_t)k * W * n + j;
+  for (int w = 0; w < W; ++w) row[(size_t)w * n] = Digit<D>::make(signed_digit(s.v, w, c), zero);
 }
 
-// Counts of chunk blockIdx.x of window blockIdx.y: chunk_count[w][p][b].
-__global__ void __launch_bounds__(SORT_THREADS) k_chunk_count(const uint32_t* digits,
-                                                              uint32_t n, uint32_t nb,
-                                                              uint32_t chunks,
-                                                              uint32_t* chunk_count) {
+// Entries per (window, bucket): chunk blockIdx.x of window blockIdx.y counts its digits in an
+// LDS histogram and adds each nonzero bin to count[w nb + b] (cleared beforehand).
+template <class D>
+__global__ void __launch_bounds__(SORT_THREADS) k_chunk_count(const D* digits, uint32_t n,
+                                                              uint32_t nb, uint32_t* count) {
   extern __shared__ uint32_t hist[];  // nb / 2 words
   uint32_t p = blockIdx.x, w = blockIdx.y;
   uint32_t words = nb / 2;
@@ -89,94 +119,97 @@ __global__ void __launch_bounds__(SORT_THREADS) k_chunk_count(const uint32_t* di
   __syncthreads();
   size_t lo = (size_t)p * SORT_CHUNK;
   size_t hi = lo + SORT_CHUNK < n ? lo + SORT_CHUNK : n;
-  const uint32_t* d = digits + (size_t)w * n;
+  const D* d = digits + (size_t)w * n;
   for (size_t i = lo + threadIdx.x; i < hi; i += SORT_THREADS) {
-    uint32_t e = d[i];
-    if (e & DIGIT_ZERO) continue;
-    uint32_t b = e & 0xFFFFu;
+    const D e = d[i];
+    if (Digit<D>::zero(e)) continue;
+    const uint32_t b = Digit<D>::bucket(e);
     atomicAdd(&hist[b >> 1], 1u << ((b & 1u) << 4));
   }
   __syncthreads();
-  uint2* out = reinterpret_cast<uint2*>(chunk_count + ((size_t)w * chunks + p) * nb);
+  uint32_t* out = count + (size_t)w * nb;
   for (uint32_t j = threadIdx.x; j < words; j += SORT_THREADS) {
-    uint32_t v = hist[j];
-    out[j] = make_uint2(v & 0xFFFFu, v >> 16);
+    const uint32_t v = hist[j];
+    if (v & 0xFFFFu) atomicAdd(&out[2 * j], v & 0xFFFFu);
+    if (v >> 16) atomicAdd(&out[2 * j + 1], v >> 16);
   }
 }
 
-// Per bucket (w, b): chunk_count[w][p][b] becomes the bucket's entries in chunks before p,
-// count[w * nb + b] its total.
-__global__ void __launch_bounds__(256) k_chunk_scan(uint32_t* chunk_count, uint32_t nb,
-                                                    uint32_t chunks, uint32_t K,
-                                                    uint32_t* count) {
+template <int = 0>
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// The scatter's global cursors: the first position of key t = (vector k, window w) nb + b,
+// from the bucket offsets and, merged (wpre given), the entries of the vector's earlier windows
+// in that bucket (whose offsets are per (k, b)).
+__global__ void k_cursor_init(uint32_t* cursor, const uint32_t* offset, const uint32_t* wpre,
+                              uint32_t nb, int W, uint32_t K) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= K) return;
-  uint32_t w = t / nb, b = t % nb;
-  uint32_t* col = chunk_count + (size_t)w * chunks * nb + b;
-  uint32_t run = 0;
-  uint32_t p = 0;
-  for (; p + 4 <= chunks; p += 4) {
-    uint32_t v0 = col[(size_t)p * nb], v1 = col[(size_t)(p + 1) * nb];
-    uint32_t v2 = col[(size_t)(p + 2) * nb], v3 = col[(size_t)(p + 3) * nb];
-    col[(size_t)p * nb] = run;
-    col[(size_t)(p + 1) * nb] = run + v0;
-    col[(size_t)(p + 2) * nb] = run + v0 + v1;
-    col[(size_t)(p + 3) * nb] = run + v0 + v1 + v2;
-    run += v0 + v1 + v2 + v3;
-  }
-  for (; p < chunks; ++p) {
-    uint32_t v = col[(size_t)p * nb];
-    col[(size_t)p * nb] = run;
-    run += v;
-  }
-  count[t] = run;
+  const uint32_t vw = t / nb, b = t - vw * nb;
+  cursor[t] = wpre ? offset[(vw / (uint32_t)W) * nb + b] + wpre[t] : offset[t];
 }
 
-// Merged windows: per bucket b, count[w * nb + b] becomes the entries of windows before w and
-// btotal[b] the bucket's total over all windows.
+// Merged windows: per (vector k, bucket b), count[(k W + w) nb + b] becomes the entries of the
+// vector's windows before w and btotal[k nb + b] the bucket's total over all its windows.
 __global__ void __launch_bounds__(256) k_window_scan(uint32_t* count, uint32_t nb, int W,
-                                                     uint32_t* btotal) {
-  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
+                                                     uint32_t sets, uint32_t* btotal) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= sets * nb) return;
+  const uint32_t k = t / nb, b = t - k * nb;
+  uint32_t* col = count + (size_t)k * W * nb + b;
   uint32_t run = 0;
   for (int w = 0; w < W; ++w) {
-    uint32_t c = count[(size_t)w * nb + b];
-    count[(size_t)w * nb + b] = run;
+    uint32_t c = col[(size_t)w * nb];
+    col[(size_t)w * nb] = run;
     run += c;
   }
-  btotal[b] = run;
+  btotal[t] = run;
 }
 
-// Entries of chunk blockIdx.x of window blockIdx.y whose bucket lies in
-// [blockIdx.z * span, + span): positions from the bucket offset, the chunk prefix and an LDS
-// cursor. Merged (wpre given): buckets are shared by every window, offsets are per bucket,
-// wpre[w * nb + b] places window w after the earlier windows, and an entry's value indexes the
-// window table (w n + i).
+// Entries of chunk blockIdx.x of digit row blockIdx.y = (vector k, window w) whose bucket lies
+// in [blockIdx.z * span, + span), in three steps over one LDS array: a histogram of the chunk's
+// digits; each nonzero bin reserves its run of positions with one atomic on the global cursor
+// of its key (the runs of a bucket's chunks land in any order; the bucket's sum does not depend
+// on it); then each entry takes the next position of its bin. Merged: a vector's windows share
+// its one bucket set (key k nb + b) and an entry's value indexes the window table (w n + i).
+template <class D>
 __global__ void __launch_bounds__(SORT_THREADS) k_chunk_scatter(
-    const uint32_t* digits, uint32_t n, uint32_t nb, uint32_t chunks, uint32_t span,
-    const uint32_t* chunk_count, const uint32_t* offset, const uint32_t* wpre, uint2* ent,
-    int constant) {
+    const D* digits, uint32_t n, uint32_t nb, int W, uint32_t span, uint32_t* cursor, int merged,
+    uint2* ent, int constant) {
   extern __shared__ uint32_t cur[];  // span words
-  uint32_t p = blockIdx.x, w = blockIdx.y, b0 = blockIdx.z * span;
-  const uint32_t* cc = chunk_count + ((size_t)w * chunks + p) * nb + b0;
-  const uint32_t* off = offset + (wpre ? 0 : (size_t)w * nb) + b0;
-  const uint32_t* wp = wpre ? wpre + (size_t)w * nb + b0 : nullptr;
-  for (uint32_t j = threadIdx.x; j < span; j += SORT_THREADS)
-    cur[j] = off[j] + cc[j] + (wp ? wp[j] : 0u);
+  const uint32_t p = blockIdx.x, w = blockIdx.y, b0 = blockIdx.z * span;  // w: digit row
+  const uint32_t k = w / (uint32_t)W, win = w - k * (uint32_t)W;
+  for (uint32_t j = threadIdx.x; j < span; j += SORT_THREADS) cur[j] = 0;
   __syncthreads();
   size_t lo = (size_t)p * SORT_CHUNK;
   size_t hi = lo + SORT_CHUNK < n ? lo + SORT_CHUNK : n;
-  const uint32_t* d = digits + (size_t)w * n;
+  const D* d = digits + (size_t)w * n;
   for (size_t i = lo + threadIdx.x; i < hi; i += SORT_THREADS) {
-    uint32_t e = d[i];
-    if (e & DIGIT_ZERO) continue;
-    uint32_t b = (e & 0xFFFFu) - b0;
+    const D e = d[i];
+    if (Digit<D>::zero(e)) continue;
+    const uint32_t b = Digit<D>::bucket(e) - b0;
+    if (b < span) atomicAdd(&cur[b], 1u);
+  }
+  __syncthreads();
+  uint32_t* gc = cursor + (size_t)w * nb + b0;
+  for (uint32_t j = threadIdx.x; j < span; j += SORT_THREADS) {
+    const uint32_t c = cur[j];
+    if (c) cur[j] = atomicAdd(&gc[j], c);
+  }
+  __syncthreads();
+  for (size_t i = lo + threadIdx.x; i < hi; i += SORT_THREADS) {
+    const D e = d[i];
+    if (Digit<D>::zero(e)) continue;
+    const uint32_t b = Digit<D>::bucket(e) - b0;
     if (b >= span) continue;
-    uint32_t pos = atomicAdd(&cur[b], 1u);
-    const uint32_t key = (wpre ? 0u : w * nb) + b0 + b;
-    const uint32_t idx = wpre ? w * n + (uint32_t)i : (uint32_t)i;
-    const uint32_t sign = (e >> 16) & 1u;
-    ent[pos] = make_uint2(key, constant ? (idx << 2) | ((e & DIGIT_DISCARD) ? 2u : 0u) | sign
+    const uint32_t pos = atomicAdd(&cur[b], 1u);
+    const uint32_t key = (merged ? k : w) * nb + b0 + b;
+    const uint32_t idx = merged ? win * n + (uint32_t)i : (uint32_t)i;
+    const uint32_t sign = Digit<D>::sign(e);
+    ent[pos] = make_uint2(key, constant ? (idx << 2) | Digit<D>::discard(e) | sign
                                         : (idx << 1) | sign);
   }
 }
@@ -282,30 +315,35 @@ static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 static uint32_t sort_chunks(uint32_t n) { return (n + SORT_CHUNK - 1) / SORT_CHUNK; }
 
-size_t msm_sort_scratch_bytes(uint32_t n, int c) {
+size_t msm_sort_scratch_bytes(uint32_t n, int c, uint32_t batch) {
   int W = (RECODE_BITS + c - 1) / c;
   size_t nb = (size_t)1 << (c - 1);
-  size_t K = (size_t)W * nb;
-  size_t E = (size_t)n * W;
+  size_t K = (size_t)batch * W * nb;
+  size_t E = (size_t)batch * n * W;
+  // count, offset, block sums, entries, digits (32-bit: the widest), cursors, btotal, total
   return align_up(4 * (K + 1)) * 2 + align_up(4 * 1025) + align_up(8 * E) + align_up(4 * E) +
-         align_up(4 * K * sort_chunks(n)) + align_up(4 * (nb + 1)) + align_up(16);
+         align_up(4 * K) + align_up(4 * (batch * nb + 1)) + align_up(16);
 }
 
-void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c, bool merged, bool constant) {
+void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c, bool merged, bool constant,
+                   uint32_t batch) {
   if (c < 2 || c > 16) throw ArgError{"MSM window width must be in [2, 16]"};
+  if (batch == 0) throw ArgError{"MSM batch of zero vectors"};
   p.merged = merged;
   p.constant = constant;
   p.c = c;
   p.windows = (RECODE_BITS + c - 1) / c;
   p.n = n;
   p.nb = 1u << (c - 1);
-  p.max_entries = (size_t)n * p.windows;
+  p.batch = batch;
+  p.max_entries = (size_t)batch * n * p.windows;
   p.chunks = sort_chunks(n);
-  size_t K = (size_t)p.windows * p.nb;
-  if ((K + 1023) / 1024 > 1024) throw ArgError{"MSM window too wide for the scan"};
-  if ((size_t)n >= (1u << 31)) throw ArgError{"MSM larger than 2^31 points"};
+  size_t K = (size_t)batch * p.windows * p.nb;
+  const size_t KS = (size_t)msm_bucket_sets(p) * p.nb;  // keys the offsets scan covers
+  if ((KS + 1023) / 1024 > 1024) throw ArgError{"MSM window too wide for the scan"};
+  if ((size_t)n * batch >= (1u << 31)) throw ArgError{"MSM larger than 2^31 points"};
   if (p.max_entries >= (1ull << 31)) throw ArgError{"MSM entry count exceeds 2^31"};
-  if (constant && (merged ? p.max_entries : (size_t)n) >= (1ull << 30))
+  if (constant && (merged ? (size_t)n * p.windows : (size_t)n) >= (1ull << 30))
     throw ArgError{"constant-work MSM: base indices must stay below 2^30"};
   char* q = static_cast<char*>(scratch);
   auto take = [&](size_t bytes) {
@@ -318,34 +356,52 @@ void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c, bool merged, bo
   p.block_sums = reinterpret_cast<uint32_t*>(take(4 * 1025));
   p.ent = reinterpret_cast<uint2*>(take(8 * p.max_entries));
   p.digits = reinterpret_cast<uint32_t*>(take(4 * p.max_entries));
-  p.chunk_count = reinterpret_cast<uint32_t*>(take(4 * K * p.chunks));
-  p.btotal = reinterpret_cast<uint32_t*>(take(4 * ((size_t)p.nb + 1)));
+  p.cursor = reinterpret_cast<uint32_t*>(take(4 * K));
+  p.btotal = reinterpret_cast<uint32_t*>(take(4 * ((size_t)batch * p.nb + 1)));
   p.total = reinterpret_cast<uint32_t*>(take(16));
 }
 
+template <class D>
+static void sort_count(MsmPlan& p, const void* scalars, hipStream_t s) {
+  D* digits = reinterpret_cast<D*>(p.digits);
+  const uint32_t total = p.batch * p.n;
+  hipLaunchKernelGGL(k_digits<D>, dim3((total + 255) / 256), dim3(256), 0, s,
+                     static_cast<const uint4*>(scalars), p.n, total, p.c, p.windows, digits,
+                     (int)p.constant);
+  G16_LAUNCH_CHECK("msm digits");
+  hipLaunchKernelGGL(k_chunk_count<D>, dim3(p.chunks, p.batch * p.windows), dim3(SORT_THREADS),
+                     4 * (p.nb / 2), s, digits, p.n, p.nb, p.count);
+  G16_LAUNCH_CHECK("msm chunk count");
+}
+
+template <class D>
+static void sort_scatter(MsmPlan& p, hipStream_t s) {
+  const uint32_t span = p.nb < SCATTER_SPAN ? p.nb : SCATTER_SPAN;
+  hipLaunchKernelGGL(k_chunk_scatter<D>, dim3(p.chunks, p.batch * p.windows, p.nb / span),
+                     dim3(SORT_THREADS), 4 * span, s, reinterpret_cast<const D*>(p.digits), p.n,
+                     p.nb, p.windows, span, p.cursor, (int)p.merged, p.ent, (int)p.constant);
+  G16_LAUNCH_CHECK("msm chunk scatter");
+}
+
 void msm_sort(MsmPlan& p, const void* scalars, hipStream_t s) {
-  uint32_t K = (uint32_t)p.windows * p.nb;
+  const uint32_t K = p.batch * (uint32_t)p.windows * p.nb;
+  hipLaunchKernelGGL(k_fill_u32<0>, dim3((K + 255) / 256), dim3(256), 0, s, p.count, (size_t)K,
+                     0u);
+  G16_LAUNCH_CHECK("msm clear counts");
   if (p.n) {
-    hipLaunchKernelGGL(k_digits, dim3((p.n + 255) / 256), dim3(256), 0, s,
-                       static_cast<const uint4*>(scalars), p.n, p.c, p.windows, p.digits,
-                       (int)p.constant);
-    G16_LAUNCH_CHECK("msm digits");
-    hipLaunchKernelGGL(k_chunk_count, dim3(p.chunks, p.windows), dim3(SORT_THREADS),
-                       4 * (p.nb / 2), s, p.digits, p.n, p.nb, p.chunks, p.chunk_count);
-    G16_LAUNCH_CHECK("msm chunk count");
+    if (short_digits(p)) sort_count<uint16_t>(p, scalars, s);
+    else sort_count<uint32_t>(p, scalars, s);
   }
-  hipLaunchKernelGGL(k_chunk_scan, dim3((K + 255) / 256), dim3(256), 0, s, p.chunk_count, p.nb,
-                     p.chunks, K, p.count);
-  G16_LAUNCH_CHECK("msm chunk scan");
   // The bucket totals to scan: per (window, bucket), or per bucket over all windows.
   const uint32_t* totals = p.count;
   uint32_t KS = K;
   if (p.merged) {
-    hipLaunchKernelGGL(k_window_scan, dim3((p.nb + 255) / 256), dim3(256), 0, s, p.count, p.nb,
-                       p.windows, p.btotal);
+    const uint32_t KB = p.batch * p.nb;
+    hipLaunchKernelGGL(k_window_scan, dim3((KB + 255) / 256), dim3(256), 0, s, p.count, p.nb,
+                       p.windows, p.batch, p.btotal);
     G16_LAUNCH_CHECK("msm window scan");
     totals = p.btotal;
-    KS = p.nb;
+    KS = KB;
   }
   uint32_t nblk = (KS + 1023) / 1024;
   hipLaunchKernelGGL(k_scan_blocks, dim3(nblk), dim3(256), 0, s, totals, KS, p.offset,
@@ -357,12 +413,11 @@ void msm_sort(MsmPlan& p, const void* scalars, hipStream_t s) {
                      p.block_sums, KS, p.total);
   G16_LAUNCH_CHECK("msm scan fix");
   if (p.n) {
-    uint32_t span = p.nb < SCATTER_SPAN ? p.nb : SCATTER_SPAN;
-    hipLaunchKernelGGL(k_chunk_scatter, dim3(p.chunks, p.windows, p.nb / span),
-                       dim3(SORT_THREADS), 4 * span, s, p.digits, p.n, p.nb, p.chunks, span,
-                       p.chunk_count, p.offset, p.merged ? p.count : nullptr, p.ent,
-                       (int)p.constant);
-    G16_LAUNCH_CHECK("msm chunk scatter");
+    hipLaunchKernelGGL(k_cursor_init, dim3((K + 255) / 256), dim3(256), 0, s, p.cursor,
+                       p.offset, p.merged ? p.count : nullptr, p.nb, p.windows, K);
+    G16_LAUNCH_CHECK("msm cursors");
+    if (short_digits(p)) sort_scatter<uint16_t>(p, s);
+    else sort_scatter<uint32_t>(p, s);
   }
 }
 
@@ -425,6 +480,26 @@ void msm_to_msm_form(const void* src, void* dst, size_t n_fq, hipStream_t s) {
 // G2 lives in kernels_msm_g2*.hip, compiled on their own (the slow ones to compile).
 void msm_accumulate_g2(const MsmPlan& p, const void* bases, void* scratch, void* window_out,
                        hipStream_t s, hipEvent_t* ev);
+
+// Workgroups of k_accumulate_queue<Fm> the device holds at once (every CU's share), measured
+// once per process.
+static int queue_grid() {
+  static const int grid = [] {
+    int dev = 0, cus = 0, per = 0;
+    G16_CHECK(hipGetDevice(&dev));
+    G16_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    G16_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_accumulate_queue<Fm>,
+                                                           QUEUE_TILE, 0));
+    return std::max(1, cus) * std::max(1, per);
+  }();
+  return grid;
+}
+
+void msm_accumulate_g1_queue(const MsmPlan* const* plans, const void* const* bases,
+                             void* const* scratch, void* const* window_out, int n,
+                             uint32_t* counter, hipStream_t s) {
+  accumulate_queue_impl<Fm>(plans, bases, scratch, window_out, n, counter, queue_grid(), s);
+}
 
 void msm_accumulate(const MsmPlan& p, int group, const void* bases, void* scratch,
                     void* window_out, hipStream_t s, hipEvent_t* ev) {

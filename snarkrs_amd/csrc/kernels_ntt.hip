@@ -33,99 +33,212 @@ __device__ __forceinline__ void stfr(uint4* p, const Fr& x) {
 
 // One pass = B consecutive radix-2 stages s0 .. s0+B-1, done in LDS. For those stages the
 // elements i = hi 2^(s0+B) + k 2^s0 + lo (k < 2^B) interact only among themselves; a
-// workgroup takes one hi and TLO consecutive lo (2^B TLO <= 2048 elements, 64 KB of LDS held
-// as eight 32-bit planes, so neighbouring lanes hit neighbouring banks), loads them with
-// consecutive lanes on consecutive lo, runs the B stages with a barrier between stages, and
-// stores them back. The pass over the lowest stages (s0 = 0) uses TLO = 1 and reads 2^11
-// contiguous elements; higher passes use B <= 6, so every load is a run of >= 32 elements.
-static constexpr int NTT_TILE = 2048;  // elements per workgroup
-static constexpr int NTT_THREADS = 256;
+// workgroup takes one hi and TLO consecutive lo (2^B TLO = NTT_TILE elements), loads them
+// with consecutive lanes on consecutive lo, runs the B stages and stores them back. The tile is
+// 4096 elements (128 KB of LDS, one workgroup of 512 threads per CU on gfx950's 160 KB), so a
+// 2^21 transform is two passes: the lowest 12 stages over contiguous tiles, then 9 stages over
+// runs of 8 consecutive elements (256 bytes). The LDS holds the tile as two planes of 16-byte
+// halves, so consecutive lanes read consecutive 16-byte words.
+//
+// Stages go two at a time (radix 4): a thread loads 4 elements, does the 2 x 2 butterflies of
+// both stages in registers and stores them, one barrier per pair; an odd stage count ends (DIT)
+// or starts (DIF) with a radix-2 step. The products are the same as radix 2's.
+//
+// The prover's iNTT -> coset shift -> NTT runs as three passes: the inverse's high stages, one
+// fused pass that does the inverse's low stages, multiplies by the coset table and does the
+// forward's low stages on the same tile in LDS, then the forward's high stages.
+static constexpr int NTT_LOG_TILE = 12;
+static constexpr int NTT_TILE = 1 << NTT_LOG_TILE;  // elements per workgroup
+static constexpr int NTT_THREADS = 512;
+static constexpr int NTT_HIGH_PASS = 10;  // stages per pass above the low pass: runs >= 4
 
-__device__ __forceinline__ void lds_put(uint32_t* lds, int e, const Fr& x) {
-#pragma unroll
-  for (int l = 0; l < 8; ++l) lds[l * NTT_TILE + e] = x.v[l];
+struct NttLds {
+  uint4 lo[NTT_TILE];  // words 0-3 of each element
+  uint4 hi[NTT_TILE];  // words 4-7
+};
+
+__device__ __forceinline__ void lds_put(NttLds& t, int e, const Fr& x) {
+  t.lo[e] = make_uint4(x.v[0], x.v[1], x.v[2], x.v[3]);
+  t.hi[e] = make_uint4(x.v[4], x.v[5], x.v[6], x.v[7]);
 }
-__device__ __forceinline__ Fr lds_get(const uint32_t* lds, int e) {
-  Fr x;
-#pragma unroll
-  for (int l = 0; l < 8; ++l) x.v[l] = lds[l * NTT_TILE + e];
-  return x;
+__device__ __forceinline__ Fr lds_get(const NttLds& t, int e) {
+  const uint4 a = t.lo[e], b = t.hi[e];
+  return {{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w}};
 }
 
+// Stage s0 + m's twiddle for tile-local k (bits above m ignored) and lo = lo0 + l.
+__device__ __forceinline__ Fr twiddle(const uint4* tw, int s0, int m, uint32_t k, uint32_t lo) {
+  const uint4* T = tw + 2 * (((size_t)1 << (s0 + m)) - 1);
+  const uint32_t j = ((k & ((1u << m) - 1u)) << s0) + lo;
+  return ldfr(T + 2 * (size_t)j);
+}
+
+// The B stages of one tile in LDS (element e = k tlo + l). DIT: stages ascending,
+// (a, b) -> (a + b w, a - b w). DIF: descending, (a, b) -> (a + b, (a - b) w).
 template <bool DIF>
-__global__ void __launch_bounds__(NTT_THREADS) k_ntt_pass(uint4* data, int L, int s0, int B,
-                                                          int tlo, const uint4* tw,
-                                                          const uint4* prescale) {
-  __shared__ uint32_t lds[8 * NTT_TILE];
-  const int E = tlo << B;  // elements in this tile
-  const uint32_t tiles_lo = (1u << s0) / (uint32_t)tlo;
-  const uint32_t hi = blockIdx.x / tiles_lo;
-  const uint32_t lo0 = (blockIdx.x % tiles_lo) * (uint32_t)tlo;
-  uint4* v = data + (((size_t)blockIdx.y << L) * 2);
-  const size_t base = ((size_t)hi << (s0 + B)) + lo0;
-  for (int e = threadIdx.x; e < E; e += NTT_THREADS) {
-    const int k = e / tlo, l = e % tlo;
-    const size_t idx = base + ((size_t)k << s0) + l;
-    Fr x = ldfr(v + 2 * idx);
-    if (!DIF && prescale) x = mul_rp(x, ldfr(prescale + 2 * idx));
-    lds_put(lds, e, x);
-  }
-  __syncthreads();
-  for (int step = 0; step < B; ++step) {
-    const int m = DIF ? B - 1 - step : step;  // stage s0 + m
-    const int s = s0 + m;
-    const uint4* T = tw + 2 * (((size_t)1 << s) - 1);
+__device__ __forceinline__ void tile_stages(NttLds& t, int lt, int s0, int B, uint32_t lo0,
+                                            const uint4* tw) {
+  const int tlo = 1 << lt, E = tlo << B;
+  auto radix2 = [&](int m) {
     for (int u = threadIdx.x; u < E / 2; u += NTT_THREADS) {
-      const int q = u / tlo, l = u % tlo;
+      const int q = u >> lt, l = u & (tlo - 1);
       const int k = ((q >> m) << (m + 1)) | (q & ((1 << m) - 1));  // bit m of k clear
-      const int e0 = k * tlo + l, e1 = e0 + (tlo << m);
-      const uint32_t j = ((uint32_t)(k & ((1 << m) - 1)) << s0) + lo0 + (uint32_t)l;
-      const Fr w = ldfr(T + 2 * (size_t)j);
-      const Fr a = lds_get(lds, e0), c = lds_get(lds, e1);
+      const int e0 = (k << lt) + l, e1 = e0 + (tlo << m);
+      const Fr w = twiddle(tw, s0, m, (uint32_t)k, lo0 + (uint32_t)l);
+      const Fr a = lds_get(t, e0), c = lds_get(t, e1);
       if (DIF) {
-        lds_put(lds, e0, add(a, c));
-        lds_put(lds, e1, mul_rp(sub(a, c), w));
+        lds_put(t, e0, add(a, c));
+        lds_put(t, e1, mul_rp(sub(a, c), w));
       } else {
         const Fr d = mul_rp(c, w);
-        lds_put(lds, e0, add(a, d));
-        lds_put(lds, e1, sub(a, d));
+        lds_put(t, e0, add(a, d));
+        lds_put(t, e1, sub(a, d));
       }
     }
     __syncthreads();
-  }
-  for (int e = threadIdx.x; e < E; e += NTT_THREADS) {
-    const int k = e / tlo, l = e % tlo;
-    stfr(v + 2 * (base + ((size_t)k << s0) + l), lds_get(lds, e));
+  };
+  // Stages m and m + 1 on e0..e3 = k0, k0 + 2^m, k0 + 2^(m+1), k0 + 2^m + 2^(m+1) (times tlo,
+  // plus l): stage m pairs (e0, e1), (e2, e3) with one twiddle, stage m + 1 pairs (e0, e2) with
+  // the same low bits and (e1, e3) with bit m set.
+  auto radix4 = [&](int m) {
+    for (int u = threadIdx.x; u < E / 4; u += NTT_THREADS) {
+      const int q = u >> lt, l = u & (tlo - 1);
+      const int k0 = ((q >> m) << (m + 2)) | (q & ((1 << m) - 1));
+      const int e0 = (k0 << lt) + l, e1 = e0 + (tlo << m), e2 = e0 + (tlo << (m + 1)),
+                e3 = e2 + (tlo << m);
+      const uint32_t lo = lo0 + (uint32_t)l;
+      const Fr wa = twiddle(tw, s0, m, (uint32_t)k0, lo);
+      const Fr wb = twiddle(tw, s0, m + 1, (uint32_t)k0, lo);
+      const Fr wc = twiddle(tw, s0, m + 1, (uint32_t)k0 | (1u << m), lo);
+      Fr x0 = lds_get(t, e0), x1 = lds_get(t, e1), x2 = lds_get(t, e2), x3 = lds_get(t, e3);
+      if (DIF) {  // stage m + 1, then m
+        Fr y0 = add(x0, x2), y2 = mul_rp(sub(x0, x2), wb);
+        Fr y1 = add(x1, x3), y3 = mul_rp(sub(x1, x3), wc);
+        x0 = add(y0, y1);
+        x1 = mul_rp(sub(y0, y1), wa);
+        x2 = add(y2, y3);
+        x3 = mul_rp(sub(y2, y3), wa);
+      } else {  // stage m, then m + 1
+        const Fr d1 = mul_rp(x1, wa), d3 = mul_rp(x3, wa);
+        Fr y0 = add(x0, d1), y1 = sub(x0, d1), y2 = add(x2, d3), y3 = sub(x2, d3);
+        const Fr f2 = mul_rp(y2, wb), f3 = mul_rp(y3, wc);
+        x0 = add(y0, f2);
+        x2 = sub(y0, f2);
+        x1 = add(y1, f3);
+        x3 = sub(y1, f3);
+      }
+      lds_put(t, e0, x0);
+      lds_put(t, e1, x1);
+      lds_put(t, e2, x2);
+      lds_put(t, e3, x3);
+    }
+    __syncthreads();
+  };
+  if (DIF) {
+    int m = B - 1;
+    if (B & 1) radix2(m--);
+    for (; m > 0; m -= 2) radix4(m - 1);
+  } else {
+    int m = 0;
+    for (; m + 1 < B; m += 2) radix4(m);
+    if (m < B) radix2(m);
   }
 }
 
-template <bool DIF>
+// Which vectors a launch covers: grid row y is vector (y / g) 3 + y % g + o of the buffer, so
+// (g, o) = (3, 0) is every vector, (2, 0) the A and B of each proof's (A, B, C) triple and
+// (1, 2) the C. The last forward pass over C can write H = A B - C to h (vector y) instead of C,
+// with A and B already final.
+struct NttVecs {
+  int g = 3, o = 0;
+  uint4* h = nullptr;
+};
+
+// MODE 0: DIF stages (inverse twiddles tw). MODE 1: DIT stages (forward twiddles tw), each
+// element first multiplied by prescale[idx] when given. MODE 2 (s0 = 0 only): DIF stages with
+// tw, the prescale product, then DIT stages with tw2, on the same tile. MODE 3: DIT stages, then
+// h[idx] = a[idx] b[idx] - x stored instead of x (H = A B - C, no division by Z:
+// crates/groth16/src/cpu.rs:230-245, fused as crates/cuda/src/kernels/ntt.cu:268-275 does).
+template <int MODE>
+__global__ void __launch_bounds__(NTT_THREADS) k_ntt_pass(uint4* data, int L, int s0, int B,
+                                                          int lt, const uint4* tw,
+                                                          const uint4* prescale,
+                                                          const uint4* tw2, NttVecs vs) {
+  __shared__ NttLds t;
+  const int tlo = 1 << lt, E = tlo << B;  // elements in this tile
+  const uint32_t tiles_lo = 1u << (s0 - lt);
+  const uint32_t hi = blockIdx.x >> (s0 - lt);
+  const uint32_t lo0 = (blockIdx.x & (tiles_lo - 1)) << lt;
+  const uint32_t y = blockIdx.y;
+  uint4* v = data + (((size_t)((y / vs.g) * 3 + y % vs.g + vs.o) << L) * 2);
+  const size_t base = ((size_t)hi << (s0 + B)) + lo0;
+  for (int e = threadIdx.x; e < E; e += NTT_THREADS) {
+    const int k = e >> lt, l = e & (tlo - 1);
+    const size_t idx = base + ((size_t)k << s0) + l;
+    Fr x = ldfr(v + 2 * idx);
+    if (MODE == 1 && prescale) x = mul_rp(x, ldfr(prescale + 2 * idx));
+    lds_put(t, e, x);
+  }
+  __syncthreads();
+  if (MODE == 0) tile_stages<true>(t, lt, s0, B, lo0, tw);
+  if (MODE == 1 || MODE == 3) tile_stages<false>(t, lt, s0, B, lo0, tw);
+  if (MODE == 2) {
+    tile_stages<true>(t, lt, s0, B, lo0, tw);
+    for (int e = threadIdx.x; e < E; e += NTT_THREADS) {
+      const size_t idx = base + ((size_t)(e >> lt) << s0) + (e & (tlo - 1));
+      lds_put(t, e, mul_rp(lds_get(t, e), ldfr(prescale + 2 * idx)));
+    }
+    __syncthreads();
+    tile_stages<false>(t, lt, s0, B, lo0, tw2);
+  }
+  for (int e = threadIdx.x; e < E; e += NTT_THREADS) {
+    const int k = e >> lt, l = e & (tlo - 1);
+    const size_t idx = base + ((size_t)k << s0) + l;
+    if (MODE == 3) {
+      const uint4* a = data + (((size_t)(3 * y) << L) * 2);
+      const uint4* b = a + (((size_t)1 << L) * 2);
+      uint4* h = vs.h + (((size_t)y << L) * 2);
+      stfr(h + 2 * idx, sub(mul(ldfr(a + 2 * idx), ldfr(b + 2 * idx)), lds_get(t, e)));
+    } else
+
+      stfr(v + 2 * idx, lds_get(t, e));
+  }
+}
+
+template <int MODE>
 static void launch_pass(void* data, int nvec, int L, int s0, int B, const void* tw,
-                        const void* prescale, hipStream_t s) {
-  int tlo = NTT_TILE >> B;
-  if (tlo > (1 << s0)) tlo = 1 << s0;
-  const uint32_t tiles = (1u << (L - B)) / (uint32_t)tlo;
-  hipLaunchKernelGGL(k_ntt_pass<DIF>, dim3(tiles, nvec), dim3(NTT_THREADS), 0, s,
-                     static_cast<uint4*>(data), L, s0, B, tlo, static_cast<const uint4*>(tw),
-                     static_cast<const uint4*>(prescale));
+                        const void* prescale, const void* tw2, hipStream_t s,
+                        NttVecs vs = NttVecs{}) {
+  int lt = NTT_LOG_TILE - B;  // log2 of the consecutive lo per tile
+  if (lt > s0) lt = s0;
+  if (lt < 0) throw ArgError{"NTT pass of more stages than a tile holds"};
+  const uint32_t tiles = 1u << (L - B - lt);
+  hipLaunchKernelGGL(k_ntt_pass<MODE>, dim3(tiles, nvec), dim3(NTT_THREADS), 0, s,
+                     static_cast<uint4*>(data), L, s0, B, lt, static_cast<const uint4*>(tw),
+                     static_cast<const uint4*>(prescale), static_cast<const uint4*>(tw2), vs);
   G16_LAUNCH_CHECK("ntt pass");
 }
 
-// Pass plan: the lowest 11 stages in one pass (contiguous tiles), the rest in passes of at
-// most 6 stages.
-static constexpr int LOW_PASS = 11, HIGH_PASS = 6;
+// Pass plan: the lowest min(L, 12) stages in one pass over contiguous tiles; the stages above
+// in passes of at most NTT_HIGH_PASS, as equal as they can be.
+static int high_passes(int rest) { return (rest + NTT_HIGH_PASS - 1) / NTT_HIGH_PASS; }
+
+static int low_stages(int L) { return L < NTT_LOG_TILE ? L : NTT_LOG_TILE; }
+
+// Stages [low, L) in ascending passes: pass p covers [start(p), start(p + 1)).
+static int high_start(int L, int p) {
+  const int low = low_stages(L), rest = L - low, np = high_passes(rest);
+  return low + (rest * p) / np;
+}
 
 void ntt_dif_inverse(void* data, int nvec, int L, const void* tw, hipStream_t s) {
   if (L == 0) return;
-  const int low = L < LOW_PASS ? L : LOW_PASS;
-  int top = L;
-  while (top > low) {
-    const int rest = top - low;
-    const int B = rest < HIGH_PASS ? rest : HIGH_PASS;
-    launch_pass<true>(data, nvec, L, top - B, B, tw, nullptr, s);
-    top -= B;
+  const int np = high_passes(L - low_stages(L));
+  for (int p = np - 1; p >= 0; --p) {
+    const int a = high_start(L, p), b = high_start(L, p + 1);
+    launch_pass<0>(data, nvec, L, a, b - a, tw, nullptr, nullptr, s);
   }
-  launch_pass<true>(data, nvec, L, 0, low, tw, nullptr, s);
+  launch_pass<0>(data, nvec, L, 0, low_stages(L), tw, nullptr, nullptr, s);
 }
 
 void ntt_dit_forward(void* data, int nvec, int L, const void* tw, const void* prescale,
@@ -134,14 +247,45 @@ void ntt_dit_forward(void* data, int nvec, int L, const void* tw, const void* pr
     if (prescale) launch_scale_rp(data, (size_t)nvec, prescale, s);  // n == 1: one factor
     return;
   }
-  const int low = L < LOW_PASS ? L : LOW_PASS;
-  launch_pass<false>(data, nvec, L, 0, low, tw, prescale, s);
-  int s0 = low;
-  while (s0 < L) {
-    const int B = (L - s0) < HIGH_PASS ? (L - s0) : HIGH_PASS;
-    launch_pass<false>(data, nvec, L, s0, B, tw, nullptr, s);
-    s0 += B;
+  launch_pass<1>(data, nvec, L, 0, low_stages(L), tw, prescale, nullptr, s);
+  const int np = high_passes(L - low_stages(L));
+  for (int p = 0; p < np; ++p) {
+    const int a = high_start(L, p), b = high_start(L, p + 1);
+    launch_pass<1>(data, nvec, L, a, b - a, tw, nullptr, nullptr, s);
   }
+}
+
+void ntt_coset_pipeline(void* data, int nvec, int L, const void* tw_inv, const void* tw_fwd,
+                        const void* coset, hipStream_t s, void* h_out) {
+  const size_t n = (size_t)1 << L;
+  if (h_out && nvec % 3) throw ArgError{"H from the transforms needs (A, B, C) triples"};
+  uint4* v = static_cast<uint4*>(data);
+  const int np = L == 0 ? 0 : high_passes(L - low_stages(L));
+  if (L == 0) {
+    launch_scale_rp(data, (size_t)nvec, coset, s);
+  } else {
+    for (int p = np - 1; p >= 0; --p) {
+      const int a = high_start(L, p), b = high_start(L, p + 1);
+      launch_pass<0>(data, nvec, L, a, b - a, tw_inv, nullptr, nullptr, s);
+    }
+    launch_pass<2>(data, nvec, L, 0, low_stages(L), tw_inv, coset, tw_fwd, s);
+    for (int p = 0; p < np; ++p) {
+      const int a = high_start(L, p), b = high_start(L, p + 1);
+      if (h_out && p == np - 1) {
+        // Every A and B first, then the C passes store H from the final A and B.
+        const int m = nvec / 3;
+        launch_pass<1>(data, 2 * m, L, a, b - a, tw_fwd, nullptr, nullptr, s, NttVecs{2, 0});
+        launch_pass<3>(data, m, L, a, b - a, tw_fwd, nullptr, nullptr, s,
+                       NttVecs{1, 2, static_cast<uint4*>(h_out)});
+      } else {
+        launch_pass<1>(data, nvec, L, a, b - a, tw_fwd, nullptr, nullptr, s);
+      }
+    }
+  }
+  if (h_out && np == 0)
+    for (int k = 0; k < nvec / 3; ++k)
+      launch_h(static_cast<uint4*>(h_out) + 2 * n * k, v + 6 * n * k, v + 6 * n * k + 2 * n,
+               v + 6 * n * k + 4 * n, n, s);
 }
 
 // out[i] = base^i, 64 consecutive elements per thread re-entering the ladder at its start.

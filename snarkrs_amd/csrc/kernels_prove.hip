@@ -45,6 +45,56 @@ void launch_gather(const uint32_t* row_ptr, const uint32_t* signal, const uint32
   G16_LAUNCH_CHECK("gather");
 }
 
+// Row c of A.w and of B.w (the shortcuts of k_gather), and C = A o B, in one pass: stage 0
+// as the reference's fused gather (crates/cuda/src/kernels/gather.cu:83-129) computes it.
+__device__ __forceinline__ Fr gather_row(const uint32_t* __restrict__ signal,
+                                         const uint32_t* __restrict__ value_id,
+                                         const uint4* __restrict__ values,
+                                         const uint4* __restrict__ w, uint32_t lo, uint32_t hi) {
+  Fr acc = fp_zero<FrCfg>();
+  for (uint32_t k = lo; k < hi; ++k) {
+    Fr x = ldfr4(w + 2 * (size_t)signal[k]);
+    if (is_zero(x)) continue;
+    Fr v = ldfr4(values + 2 * (size_t)value_id[k]);
+    acc = add(acc, is_one(x) ? v : mul(v, x));
+  }
+  return acc;
+}
+
+// Grid row y is proof y of a batch: witness y at w + y nv, its (A, B, C) at abc + 3 y n_rows.
+__global__ void __launch_bounds__(256) k_gather_abc(const uint32_t* __restrict__ rp_a,
+                                                    const uint32_t* __restrict__ sig_a,
+                                                    const uint32_t* __restrict__ vid_a,
+                                                    const uint32_t* __restrict__ rp_b,
+                                                    const uint32_t* __restrict__ sig_b,
+                                                    const uint32_t* __restrict__ vid_b,
+                                                    const uint4* __restrict__ values,
+                                                    const uint4* __restrict__ w, size_t nv,
+                                                    uint4* abc, uint32_t n_rows) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_rows) return;
+  w += 2 * nv * blockIdx.y;
+  uint4* a = abc + 2 * (size_t)n_rows * 3 * blockIdx.y;
+  uint4* b = a + 2 * (size_t)n_rows;
+  uint4* c = b + 2 * (size_t)n_rows;
+  const Fr x = gather_row(sig_a, vid_a, values, w, rp_a[r], rp_a[r + 1]);
+  const Fr y = gather_row(sig_b, vid_b, values, w, rp_b[r], rp_b[r + 1]);
+  stfr4(a + 2 * (size_t)r, x);
+  stfr4(b + 2 * (size_t)r, y);
+  stfr4(c + 2 * (size_t)r, mul(x, y));
+}
+
+void launch_gather_abc(const uint32_t* const row_ptr[2], const uint32_t* const signal[2],
+                       const uint32_t* const value_id[2], const void* values,
+                       const void* witness, size_t nv, void* abc, uint32_t n_rows,
+                       uint32_t batch, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_abc, dim3((n_rows + 255) / 256, batch), dim3(256), 0, s,
+                     row_ptr[0], signal[0], value_id[0], row_ptr[1], signal[1], value_id[1],
+                     static_cast<const uint4*>(values), static_cast<const uint4*>(witness), nv,
+                     static_cast<uint4*>(abc), n_rows);
+  G16_LAUNCH_CHECK("gather A, B, C");
+}
+
 __global__ void k_mul_into(uint4* out, const uint4* a, const uint4* b, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;

@@ -2,6 +2,9 @@
 // shared by kernels_msm.hip (G1) and kernels_msm_g2.hip (G2). See kernels_msm.hip for the
 // pipeline.
 #pragma once
+#include <algorithm>
+#include <string>
+
 #include "bn254.cuh"
 #include "internal.h"
 
@@ -217,16 +220,16 @@ struct Acc<Fm> {
   }
 };
 
+// Segment t of the sorted entries: its runs added into XYZZ accumulators, a run whole inside
+// the segment stored as its bucket, the end pieces to pfirst / plast.
 // CONST: constant-work entries (index << 2 | discard << 1 | sign): the addition of a
 // discarded entry is computed and its result dropped by a select.
 template <class F, bool CONST>
-__global__ void __launch_bounds__(128) k_accumulate(const uint2* __restrict__ ent,
-                                                    const uint32_t* total, size_t nseg,
-                                                    uint32_t seg,
-                                                    const uint4* __restrict__ bases,
-                                                    uint4* buckets, uint4* pfirst,
-                                                    uint4* plast) {
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void accumulate_segment(size_t t, const uint2* __restrict__ ent,
+                                                   const uint32_t* total, size_t nseg,
+                                                   uint32_t seg,
+                                                   const uint4* __restrict__ bases,
+                                                   uint4* buckets, uint4* pfirst, uint4* plast) {
   if (t >= nseg) return;
   uint32_t T = total[0];
   size_t b = t * seg;
@@ -283,6 +286,57 @@ __global__ void __launch_bounds__(128) k_accumulate(const uint2* __restrict__ en
   } else {
     if (rs == b) st_xyzz(pfirst, t, sum);
     st_xyzz(plast, t, sum);
+  }
+}
+
+template <class F, bool CONST>
+__global__ void __launch_bounds__(128) k_accumulate(const uint2* __restrict__ ent,
+                                                    const uint32_t* total, size_t nseg,
+                                                    uint32_t seg,
+                                                    const uint4* __restrict__ bases,
+                                                    uint4* buckets, uint4* pfirst,
+                                                    uint4* plast) {
+  accumulate_segment<F, CONST>((size_t)blockIdx.x * blockDim.x + threadIdx.x, ent, total, nseg,
+                               seg, bases, buckets, pfirst, plast);
+}
+
+// Several MSMs' bucket accumulations as one persistent launch over a device-side work queue:
+// the grid is sized to what the device holds at once, and each workgroup takes tiles of 128
+// segments from a shared counter until every job's segments are done. Tile t belongs to job
+// d with start[d] <= t < start[d + 1]; segments past a job's device-side entry count end at once.
+static constexpr int QUEUE_JOBS = 4;
+static constexpr uint32_t QUEUE_TILE = 128;  // segments per tile = threads per workgroup
+
+struct AccQueueJob {
+  const uint2* ent;
+  const uint32_t* total;
+  const uint4* bases;
+  uint4 *buckets, *pfirst, *plast;
+  uint64_t nseg;
+  uint32_t seg;
+};
+
+struct AccQueue {
+  AccQueueJob job[QUEUE_JOBS];
+  uint32_t start[QUEUE_JOBS + 1];
+  int n;
+};
+
+template <class F>
+__global__ void __launch_bounds__(QUEUE_TILE) k_accumulate_queue(AccQueue q, uint32_t* counter) {
+  __shared__ uint32_t tile;
+  const uint32_t tiles = q.start[q.n];
+  for (;;) {
+    if (threadIdx.x == 0) tile = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t t = tile;
+    __syncthreads();  // every lane has read `tile` before lane 0 overwrites it
+    if (t >= tiles) return;
+    int d = 0;
+    while (d + 1 < q.n && t >= q.start[d + 1]) ++d;
+    const AccQueueJob& j = q.job[d];
+    accumulate_segment<F, false>((size_t)(t - q.start[d]) * QUEUE_TILE + threadIdx.x, j.ent,
+                                 j.total, j.nseg, j.seg, j.bases, j.buckets, j.pfirst, j.plast);
   }
 }
 
@@ -642,6 +696,60 @@ inline void tail_reduce_impl(const MsmPlan& p, const AccumPtrs& a, uint32_t B, v
 
 // F: the coordinate field in the MSM's Montgomery form R' = 2^261 (Fm, or Fm2i for G2); the
 // bases must already be in that form (msm_bases_to_msm_form).
+template <class F>
+inline AccumPtrs accum_ptrs(const MsmPlan& p, void* scratch) {
+  constexpr size_t PT = sizeof(Xyzz<F>);
+  AccumLayout L = accum_layout(p, PT);
+  char* q = static_cast<char*>(scratch);
+  AccumPtrs a;
+  a.buckets = reinterpret_cast<uint4*>(q + L.buckets);
+  a.pfirst = reinterpret_cast<uint4*>(q + L.pfirst);
+  a.plast = reinterpret_cast<uint4*>(q + L.plast);
+  a.heavy = reinterpret_cast<uint32_t*>(q + L.heavy);
+  a.tasks = reinterpret_cast<uint32_t*>(q + L.tasks);
+  a.task_sum = reinterpret_cast<uint4*>(q + L.task_sum);
+  a.gT = reinterpret_cast<uint4*>(q + L.gT);
+  a.gU = reinterpret_cast<uint4*>(q + L.gU);
+  a.parts = reinterpret_cast<uint4*>(q + L.parts);
+  a.cnt = reinterpret_cast<uint32_t*>(q + L.cnt);
+  a.seg = accum_seg(p, PT);
+  return a;
+}
+
+// The G1 bucket accumulations of up to QUEUE_JOBS MSMs (each: plan, bases, scratch) as one
+// persistent launch, then each job's merges and reduction into its window_out. `counter` is a
+// device word the launch takes tiles from (cleared here by a kernel first).
+template <class F>
+inline void accumulate_queue_impl(const MsmPlan* const* plans, const void* const* bases,
+                                  void* const* scratch, void* const* window_out, int n,
+                                  uint32_t* counter, int grid, hipStream_t s) {
+  if (n < 1 || n > QUEUE_JOBS) throw ArgError{"accumulation queue of " + std::to_string(n) + " jobs"};
+  AccQueue q{};
+  q.n = n;
+  q.start[0] = 0;
+  AccumPtrs a[QUEUE_JOBS];
+  for (int d = 0; d < n; ++d) {
+    a[d] = accum_ptrs<F>(*plans[d], scratch[d]);
+    const size_t nseg = (plans[d]->max_entries + a[d].seg - 1) / a[d].seg;
+    q.job[d] = AccQueueJob{plans[d]->ent, plans[d]->total, static_cast<const uint4*>(bases[d]),
+                           a[d].buckets, a[d].pfirst, a[d].plast, nseg, a[d].seg};
+    const size_t tiles = (nseg + QUEUE_TILE - 1) / QUEUE_TILE;
+    if (q.start[d] + tiles >= (1ull << 32)) throw ArgError{"accumulation queue too long"};
+    q.start[d + 1] = q.start[d] + (uint32_t)tiles;
+  }
+  hipLaunchKernelGGL(k_clear_cnt<0>, dim3(1), dim3(64), 0, s, counter);
+  G16_LAUNCH_CHECK("msm queue counter");
+  if (q.start[n]) {
+    const unsigned blocks = (unsigned)std::min<size_t>((size_t)grid, q.start[n]);
+    hipLaunchKernelGGL(k_accumulate_queue<F>, dim3(blocks), dim3(QUEUE_TILE), 0, s, q, counter);
+    G16_LAUNCH_CHECK("msm accumulate (queue)");
+  }
+  for (int d = 0; d < n; ++d) {
+    Tail<F>::merge(*plans[d], a[d], s);
+    Tail<F>::reduce(*plans[d], a[d], reduce_group_size(*plans[d]), window_out[d], s);
+  }
+}
+
 template <class F>
 inline void accumulate_impl(const MsmPlan& p, const void* bases, void* scratch, void* window_out,
                             hipStream_t s, hipEvent_t* ev) {

@@ -69,6 +69,43 @@ struct Slot {
   }
 };
 
+// A wave: up to `cap` proofs of one circuit proved together. Every stage is one launch over all
+// of them (the fused gather over cap witnesses, the transforms of 3 cap vectors, each MSM's
+// sort, accumulation, merge and reduction over all cap scalar vectors with a bucket set per
+// proof), on one stream, so a batch keeps one hardware queue per circuit. Host buffers are
+// double-buffered: the host finishes wave i (Horner passes, blinders, assembly) while the
+// device runs wave i + 1.
+struct Wave {
+  uint32_t cap = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t done[2] = {};
+  DevBuf witness, abc, hbuf, sort_w, sort_h;
+  DevBuf accum_g1[4], accum_w2;    // A, B1, L, H (one accumulation queue), B2
+  DevBuf queue;                    // the accumulation queue's counter
+  uint8_t* host_wout[2] = {};      // reduction rows, pinned and device-mapped
+  uint8_t* host_wout_dev[2] = {};
+  uint8_t* staging[2] = {};        // page-locked witnesses, cap * 32 n_vars bytes each
+  size_t wout_bytes = 0, bytes = 0;
+  // The device part of a wave of m proofs on buffer b, captured once and replayed (graph
+  // replay on): key 2 m + b.
+  std::map<uint32_t, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+
+  ~Wave() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& g : graphs) {
+      if (g.second.second) (void)hipGraphExecDestroy(g.second.second);
+      if (g.second.first) (void)hipGraphDestroy(g.second.first);
+    }
+    for (auto& e : done)
+      if (e) (void)hipEventDestroy(e);
+    for (int b = 0; b < 2; ++b) {
+      if (host_wout[b]) (void)hipHostFree(host_wout[b]);
+      if (staging[b]) (void)hipHostFree(staging[b]);
+    }
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
 }  // namespace g16
 
 struct g16hip_circuit {
@@ -93,6 +130,11 @@ struct g16hip_circuit {
   std::vector<std::unique_ptr<Slot>> slots;
   std::vector<bool> busy;
   uint32_t growing = 0;  // slots being added outside the lock
+  // Batched proving: the wave (built on first use, rebuilt for another cap) and the proofs per
+  // wave g16hip_prove_batch uses (0: the default; g16hip_batch_autotune measures one).
+  std::mutex wave_mu;
+  std::unique_ptr<g16::Wave> wave;
+  std::atomic<uint32_t> wave_cap{0};
 };
 
 struct g16hip_h {
@@ -498,17 +540,16 @@ static void stages_0_4(const g16hip_circuit* C, Slot* S, void* h_out, bool timed
     }
     launch_rows_abc(A, B, Cv, rows->n, n, s);
   } else {
-    launch_gather(C->row_ptr[0].as<uint32_t>(), C->signal[0].as<uint32_t>(),
-                  C->value_id[0].as<uint32_t>(), C->values.p, S->witness.p, A, C->domain, s);
-    launch_gather(C->row_ptr[1].as<uint32_t>(), C->signal[1].as<uint32_t>(),
-                  C->value_id[1].as<uint32_t>(), C->values.p, S->witness.p, B, C->domain, s);
-    launch_mul_into(Cv, A, B, n, s);
+    const uint32_t* rp[2] = {C->row_ptr[0].as<uint32_t>(), C->row_ptr[1].as<uint32_t>()};
+    const uint32_t* sg[2] = {C->signal[0].as<uint32_t>(), C->signal[1].as<uint32_t>()};
+    const uint32_t* vi[2] = {C->value_id[0].as<uint32_t>(), C->value_id[1].as<uint32_t>()};
+    launch_gather_abc(rp, sg, vi, C->values.p, S->witness.p, C->n_vars, abc, C->domain, 1, s);
   }
   if (timed) G16_CHECK(hipEventRecord(S->ev[1], s));
-  ntt_dif_inverse(abc, 3, C->log_n, C->tables.inv.p, s);
-  ntt_dit_forward(abc, 3, C->log_n, C->tables.fwd.p, C->tables.coset.p, s);
+  // The transforms, with H = A*B - C written by the last pass.
+  ntt_coset_pipeline(abc, 3, C->log_n, C->tables.inv.p, C->tables.fwd.p, C->tables.coset.p, s,
+                     h_out);
   if (timed) G16_CHECK(hipEventRecord(S->ev[2], s));
-  launch_h(h_out, A, B, Cv, n, s);
   if (C->h_basis == 1) {
     // Coefficients in bit-reversed order, the order gnark's computeH leaves and pk.G1.Z
     // expects (crates/groth16/src/cpu.rs:250-286).
@@ -817,6 +858,279 @@ static void prove_on_slot(g16hip_circuit* C, Slot* S, const void* w, size_t n, b
   if (t) t->assemble_us += ta.us();
 }
 
+// Waves (and the server's slot proofs) launch their kernels one by one (the eager path) or
+// replay the pipeline captured once as a hipGraph: g16hip_set_graph_replay, or G16HIP_GRAPHS=1
+// at start-up. Eager is the default.
+static std::atomic<int> g_graph_replay{-1};
+
+static bool replay_graphs() {
+  int v = g_graph_replay.load(std::memory_order_relaxed);
+  if (v < 0) {
+    const char* e = std::getenv("G16HIP_GRAPHS");
+    v = (e && e[0] == '1') ? 1 : 0;
+    int expected = -1;
+    g_graph_replay.compare_exchange_strong(expected, v);
+    v = g_graph_replay.load(std::memory_order_relaxed);
+  }
+  return v == 1;
+}
+
+// ---------------------------------------------------------------------------------------
+// batched proving: waves
+// ---------------------------------------------------------------------------------------
+// Waves need the gather path and snarkjs' H basis; other keys prove item by item on a slot.
+static bool can_wave(const g16hip_circuit* C) { return C->has_matrices && C->h_basis == 0; }
+
+struct WavePlans {
+  MsmPlan w, h;      // bound to the wave's scratch for m proofs
+  MsmPlan vw, vh;    // one proof's view (its rows)
+  size_t rw = 0, rh = 0;  // reduction rows per proof
+};
+
+static WavePlans wave_plans(const g16hip_circuit* C, void* sort_w, void* sort_h, uint32_t m) {
+  WavePlans P;
+  msm_plan_bind(P.w, sort_w, C->n_vars, C->cw, C->merged, C->constant, m);
+  msm_plan_bind(P.h, sort_h, C->n_h, C->ch, C->merged, C->constant, m);
+  P.vw = msm_vector_view(P.w);
+  P.vh = msm_vector_view(P.h);
+  P.rw = msm_out_points(P.vw);
+  P.rh = msm_out_points(P.vh);
+  return P;
+}
+
+// Rows of MSM j (A, B1, L, B2, H) for proof k of a wave of cap proofs, in bytes from the start
+// of the rows buffer: every MSM's block holds cap proofs' rows.
+static size_t wave_rows_offset(const WavePlans& P, uint32_t cap, int j, uint32_t k) {
+  const size_t g1 = 128 * P.rw * cap;
+  const size_t base[5] = {0, g1, 2 * g1, 3 * g1, 3 * g1 + 256 * P.rw * cap};
+  const size_t per[5] = {128 * P.rw, 128 * P.rw, 128 * P.rw, 256 * P.rw, 128 * P.rh};
+  return base[j] + per[j] * k;
+}
+
+static size_t wave_rows_bytes(const WavePlans& P, uint32_t cap) {
+  return wave_rows_offset(P, cap, 4, cap);
+}
+
+static size_t wave_bytes_for(const g16hip_circuit* C, uint32_t cap) {
+  static char anchor;  // bound only to size the pieces
+  const WavePlans P = wave_plans(C, &anchor, &anchor, cap);
+  const size_t nv = C->n_vars, n = C->domain;
+  return dev_pages(32 * nv * cap) + dev_pages(96 * n * cap) + dev_pages(32 * n * cap) +
+         dev_pages(msm_sort_scratch_bytes((uint32_t)nv, C->cw, cap)) +
+         dev_pages(msm_sort_scratch_bytes(C->n_h, C->ch, cap)) +
+         3 * dev_pages(msm_accum_scratch_bytes(P.w, 1)) + dev_pages(msm_accum_scratch_bytes(P.w, 2)) +
+         dev_pages(msm_accum_scratch_bytes(P.h, 1)) + dev_pages(256);
+}
+
+// Proofs per wave when none was measured: enough to give a transform about 2^21 elements per
+// vector set (small circuits fill the device together), at most 16.
+static uint32_t default_wave_cap(const g16hip_circuit* C) {
+  uint32_t cap = (1u << 21) / std::max<uint32_t>(C->domain, 1);
+  return std::max(1u, std::min(16u, cap));
+}
+
+static uint32_t wave_cap_of(const g16hip_circuit* C) {
+  const uint32_t cap = C->wave_cap.load();
+  return cap ? cap : default_wave_cap(C);
+}
+
+// Builds a wave for cap proofs, refused before allocating when the device budget cannot hold it.
+static std::unique_ptr<Wave> make_wave(g16hip_circuit* C, uint32_t cap) {
+  const size_t need = wave_bytes_for(C, cap);
+  const size_t budget = device_budget(C->dev);
+  if (need > budget) throw too_big(need, budget, "a wave of " + std::to_string(cap) + " proofs");
+  auto W = std::make_unique<Wave>();
+  W->cap = cap;
+  G16_CHECK(hipStreamCreateWithFlags(&W->stream, hipStreamNonBlocking));
+  for (auto& e : W->done) G16_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  const size_t nv = C->n_vars, n = C->domain;
+  W->witness.alloc(32 * nv * cap);
+  W->abc.alloc(96 * n * cap);
+  W->hbuf.alloc(32 * n * cap);
+  W->sort_w.alloc(msm_sort_scratch_bytes((uint32_t)nv, C->cw, cap));
+  W->sort_h.alloc(msm_sort_scratch_bytes(C->n_h, C->ch, cap));
+  const WavePlans P = wave_plans(C, W->sort_w.p, W->sort_h.p, cap);
+  for (int j = 0; j < 3; ++j) W->accum_g1[j].alloc(msm_accum_scratch_bytes(P.w, 1));
+  W->accum_g1[3].alloc(msm_accum_scratch_bytes(P.h, 1));
+  W->accum_w2.alloc(msm_accum_scratch_bytes(P.w, 2));
+  W->queue.alloc(256);
+  W->wout_bytes = wave_rows_bytes(P, cap);
+  for (int b = 0; b < 2; ++b) {
+    G16_CHECK(hipHostMalloc((void**)&W->host_wout[b], W->wout_bytes + 64,
+                            hipHostMallocMapped | hipHostMallocCoherent));
+    G16_CHECK(hipHostGetDevicePointer((void**)&W->host_wout_dev[b], W->host_wout[b], 0));
+    G16_CHECK(hipHostMalloc((void**)&W->staging[b], std::max<size_t>(32 * nv * cap, 32),
+                            hipHostMallocDefault));
+  }
+  W->bytes = need;
+  return W;
+}
+
+struct WaveItem {
+  const uint8_t* witness;
+  Fr r, s;  // Montgomery
+  size_t index;  // in the caller's batch
+};
+
+// The device part of a wave of m proofs on buffer b (everything after the witnesses are in
+// staging), on the wave's stream: the upload, stages 0-4, both sorts and the five MSMs.
+static void wave_device(g16hip_circuit* C, Wave* W, uint32_t m, int b) {
+  const size_t nv = C->n_vars, wb = 32 * nv;
+  hipStream_t s = W->stream;
+  G16_CHECK(hipMemcpyAsync(W->witness.p, W->staging[b], wb * m, hipMemcpyHostToDevice, s));
+  WavePlans P = wave_plans(C, W->sort_w.p, W->sort_h.p, m);
+  const uint32_t* rp[2] = {C->row_ptr[0].as<uint32_t>(), C->row_ptr[1].as<uint32_t>()};
+  const uint32_t* sg[2] = {C->signal[0].as<uint32_t>(), C->signal[1].as<uint32_t>()};
+  const uint32_t* vi[2] = {C->value_id[0].as<uint32_t>(), C->value_id[1].as<uint32_t>()};
+  launch_gather_abc(rp, sg, vi, C->values.p, W->witness.p, nv, W->abc.p, C->domain, m, s);
+  ntt_coset_pipeline(W->abc.p, 3 * (int)m, C->log_n, C->tables.inv.p, C->tables.fwd.p,
+                     C->tables.coset.p, s, W->hbuf.p);
+  uint8_t* rows = W->host_wout_dev[b];
+  const uint32_t cap = W->cap;
+  msm_sort(P.w, W->witness.p, s);
+  msm_sort(P.h, W->hbuf.p, s);
+  // The four G1 MSMs (A, B1, L over the witness entries, H over its own) in one accumulation
+  // queue, then B2 (G2).
+  const MsmPlan* plans[4] = {&P.w, &P.w, &P.w, &P.h};
+  const void* bases[4] = {C->a_q.p, C->b1_q.p, C->l_q.p, C->h_q.p};
+  void* scratch[4] = {W->accum_g1[0].p, W->accum_g1[1].p, W->accum_g1[2].p, W->accum_g1[3].p};
+  void* outs[4] = {rows + wave_rows_offset(P, cap, 0, 0), rows + wave_rows_offset(P, cap, 1, 0),
+                   rows + wave_rows_offset(P, cap, 2, 0), rows + wave_rows_offset(P, cap, 4, 0)};
+  msm_accumulate_g1_queue(plans, bases, scratch, outs, 4, W->queue.as<uint32_t>(), s);
+  msm_accumulate(P.w, 2, C->b2_q.p, W->accum_w2.p, rows + wave_rows_offset(P, cap, 3, 0), s);
+}
+
+// Copies the wave's witnesses into page-locked staging with a few host threads and enqueues one
+// upload, then stages 0-9 for all m proofs on the wave's stream; done[b] follows the last rows.
+static void enqueue_wave(g16hip_circuit* C, Wave* W, const std::vector<WaveItem>& it, int b) {
+  const uint32_t m = (uint32_t)it.size();
+  const size_t nv = C->n_vars, wb = 32 * nv;
+  hipStream_t s = W->stream;
+  uint8_t* stage = W->staging[b];
+  {
+    std::atomic<uint32_t> next{0};
+    auto copy = [&] {
+      for (uint32_t k; (k = next.fetch_add(1)) < m;) std::memcpy(stage + wb * k, it[k].witness, wb);
+    };
+    std::vector<std::thread> th;
+    for (uint32_t h = 1; h < std::min<uint32_t>(m, 4); ++h) th.emplace_back(copy);
+    copy();
+    for (auto& t : th) t.join();
+  }
+  if (replay_graphs()) {
+    auto gi = W->graphs.find(2 * m + (uint32_t)b);
+    if (gi == W->graphs.end()) {
+      hipGraph_t g = nullptr;
+      hipGraphExec_t x = nullptr;
+      G16_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      try {
+        wave_device(C, W, m, b);
+      } catch (...) {
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      G16_CHECK(hipStreamEndCapture(s, &g));
+      const hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+      if (e != hipSuccess) (void)hipGraphDestroy(g);
+      G16_CHECK(e);
+      gi = W->graphs.emplace(2 * m + (uint32_t)b, std::make_pair(g, x)).first;
+    }
+    G16_CHECK(hipGraphLaunch(gi->second.second, s));
+  } else {
+    wave_device(C, W, m, b);
+  }
+  G16_CHECK(hipEventRecord(W->done[b], s));
+}
+
+// Waits for wave b and finishes its proofs on the host: each proof's Horner passes, blinding
+// products and stage 11, a few proofs per host thread.
+static void finish_wave(g16hip_circuit* C, Wave* W, const std::vector<WaveItem>& it, int b,
+                        g16hip_proof* proofs, g16hip_status* st, std::string* msg) {
+  G16_CHECK(hipEventSynchronize(W->done[b]));
+  const uint32_t m = (uint32_t)it.size(), cap = W->cap;
+  static char anchor;
+  const WavePlans P = wave_plans(C, &anchor, &anchor, m);
+  const uint8_t* rows = W->host_wout[b];
+  auto g1 = [&](int j, uint32_t k, const MsmPlan& v) {
+    return msm_combine(reinterpret_cast<const G1X*>(rows + wave_rows_offset(P, cap, j, k)), v);
+  };
+  auto one = [&](uint32_t k) {
+    const WaveItem& w = it[k];
+    st[w.index] = guarded([&] {
+      g16hip_msm_out mo;
+      BlindProducts bp;
+      bp.want = true;
+      bp.r_std = to_std(w.r);
+      bp.s_std = to_std(w.s);
+      const G1X A = g1(0, k, P.vw), B1 = g1(1, k, P.vw);
+      g1_store(mo.a_g1, host_to_aff(A));
+      g1_store(mo.b_g1, host_to_aff(B1));
+      g1_store(mo.l_g1, host_to_aff(g1(2, k, P.vw)));
+      g2_store(mo.b_g2, host_to_aff(msm_combine(
+                            reinterpret_cast<const G2X*>(rows + wave_rows_offset(P, cap, 3, k)),
+                            P.vw)));
+      g1_store(mo.h_g1, host_to_aff(g1(4, k, P.vh)));
+      bp.sA = host_mul(A, bp.s_std.v);
+      bp.rB1 = host_mul(B1, bp.r_std.v);
+      const BlinderTerms terms = blinder_terms(C, bp.r_std, bp.s_std, to_std(mul(w.r, w.s)));
+      assemble(C, mo, terms, bp, &proofs[w.index]);
+      return G16HIP_OK;
+    });
+    if (st[w.index] != G16HIP_OK) msg[w.index] = g16hip_last_error();
+  };
+  std::atomic<uint32_t> next{0};
+  auto work = [&] {
+    for (uint32_t k; (k = next.fetch_add(1)) < m;) one(k);
+  };
+  std::vector<std::thread> th;
+  for (uint32_t h = 1; h < std::min<uint32_t>(m, 8); ++h) th.emplace_back(work);
+  work();
+  for (auto& t : th) t.join();
+}
+
+// Proves the items of one circuit in waves of the circuit's cap, the host finishing wave i while
+// the device runs wave i + 1. Items that fail the witness checks get their status and no proof.
+static void prove_waves(g16hip_circuit* C, const g16hip_batch_item* items,
+                        const std::vector<size_t>& idx, g16hip_proof* proofs, g16hip_status* st,
+                        std::string* msg) {
+  std::vector<WaveItem> ok;
+  for (size_t i : idx) {
+    const g16hip_status v = guarded([&] {
+      const g16hip_batch_item& x = items[i];
+      if (!x.witness || !x.r || !x.s) throw ArgError{"null argument"};
+      check_witness_len(C, x.n_witness);
+      if (!is_one(fr_load(x.witness)))
+        throw StatusError{G16HIP_CONSTANT_WIRE, "w[0] is the constant-one wire and must be 1"};
+      const Fr r = fr_load(x.r), s = fr_load(x.s);
+      if (is_zero(r) || is_zero(s)) throw StatusError{G16HIP_ZERO_BLINDER, "a blinder is zero"};
+      ok.push_back(WaveItem{x.witness, r, s, i});
+      return G16HIP_OK;
+    });
+    if (v != G16HIP_OK) {
+      st[i] = v;
+      msg[i] = g16hip_last_error();
+    }
+  }
+  if (ok.empty()) return;
+  std::lock_guard<std::mutex> lk(C->wave_mu);
+  set_device(C->dev);
+  const uint32_t cap = wave_cap_of(C);
+  if (!C->wave || C->wave->cap != cap) {
+    C->wave.reset();
+    C->wave = make_wave(C, cap);
+  }
+  Wave* W = C->wave.get();
+  std::vector<std::vector<WaveItem>> waves;
+  for (size_t a = 0; a < ok.size(); a += cap)
+    waves.emplace_back(ok.begin() + a, ok.begin() + std::min(ok.size(), a + cap));
+  for (size_t i = 0; i < waves.size(); ++i) {
+    enqueue_wave(C, W, waves[i], (int)(i & 1));
+    if (i) finish_wave(C, W, waves[i - 1], (int)((i - 1) & 1), proofs, st, msg);
+  }
+  finish_wave(C, W, waves.back(), (int)((waves.size() - 1) & 1), proofs, st, msg);
+}
+
 }  // namespace g16
 
 // ---------------------------------------------------------------------------------------
@@ -866,50 +1180,77 @@ static bool try_add_slot(g16hip_circuit* C, size_t reserve) {
   return true;
 }
 
-// Batches and the server launch each proof's kernels one by one on the slot's three streams
-// (the eager path) or replay the slot's captured graph: g16hip_set_graph_replay, or
-// G16HIP_GRAPHS=1 at start-up. Eager is the default: with many slots in flight it measured
-// faster on gfx950 (362 against 317 proofs/s in the 64-proof batch).
-static std::atomic<int> g_graph_replay{-1};
-
-static bool replay_graphs() {
-  int v = g_graph_replay.load(std::memory_order_relaxed);
-  if (v < 0) {
-    const char* e = std::getenv("G16HIP_GRAPHS");
-    v = (e && e[0] == '1') ? 1 : 0;
-    int expected = -1;
-    g_graph_replay.compare_exchange_strong(expected, v);
-    v = g_graph_replay.load(std::memory_order_relaxed);
-  }
-  return v == 1;
-}
-
-// A lane: takes the oldest request of any circuit, leases (or adds) a slot of its circuit and
-// replays that slot's proof graph.
+// A lane: takes the oldest request and every queued request of the same circuit, up to the
+// circuit's wave cap, and proves them as one wave (continuous batching: requests that arrive
+// while other waves run join the next one). Keys that cannot wave, or a wave the device budget
+// cannot hold, prove request by request on a slot, leasing (or adding) one as memory allows.
 static void server_lane(g16hip_server* srv) {
   (void)hipSetDevice(srv->dev->ordinal);
   for (;;) {
-    std::shared_ptr<Request> req;
+    std::vector<std::shared_ptr<Request>> group;
     {
       std::unique_lock<std::mutex> lk(srv->mu);
       srv->cv_work.wait(lk, [&] { return srv->closing || !srv->queue.empty(); });
       if (srv->queue.empty()) return;  // closing, nothing left
-      req = srv->queue.front();
-      srv->queue.pop_front();
+      g16hip_circuit* C = srv->queue.front()->circuit;
+      const uint32_t cap = can_wave(C) ? wave_cap_of(C) : 1;
+      for (auto it = srv->queue.begin(); it != srv->queue.end() && group.size() < cap;) {
+        if ((*it)->circuit == C) {
+          group.push_back(*it);
+          it = srv->queue.erase(it);
+        } else {
+          ++it;
+        }
+      }
     }
-    g16hip_status st = guarded([&] {
-      g16hip_circuit* C = req->circuit;
-      SlotLease lease(C, srv->lanes, srv->reserve);
-      prove_on_slot(C, lease.S, req->witness, req->n_witness, false, req->r, req->s, &req->proof,
-                    nullptr, replay_graphs());
-      return G16HIP_OK;
-    });
+    g16hip_circuit* C = group[0]->circuit;
+    const size_t m = group.size();
+    std::vector<g16hip_batch_item> items(m);
+    std::vector<g16hip_proof> proofs(m);
+    std::vector<g16hip_status> st(m, G16HIP_DEVICE);
+    std::vector<std::string> msg(m);
+    std::vector<size_t> idx(m);
+    for (size_t k = 0; k < m; ++k) {
+      items[k] = g16hip_batch_item{C, group[k]->witness, group[k]->n_witness, group[k]->r,
+                                   group[k]->s};
+      idx[k] = k;
+    }
+    bool waved = false;
+    if (can_wave(C)) {
+      const g16hip_status w = guarded([&] {
+        prove_waves(C, items.data(), idx, proofs.data(), st.data(), msg.data());
+        return G16HIP_OK;
+      });
+      waved = w == G16HIP_OK || w != G16HIP_DEVICE_MEMORY;
+      if (!waved) std::fill(st.begin(), st.end(), G16HIP_DEVICE);
+      else if (w != G16HIP_OK)
+        for (size_t k = 0; k < m; ++k)
+          if (st[k] == G16HIP_DEVICE && msg[k].empty()) {
+            st[k] = w;
+            msg[k] = g16hip_last_error();
+          }
+    }
+    if (!waved) {
+      for (size_t k = 0; k < m; ++k) {
+        st[k] = guarded([&] {
+          SlotLease lease(C, srv->lanes, srv->reserve);
+          prove_on_slot(C, lease.S, items[k].witness, items[k].n_witness, false, items[k].r,
+                        items[k].s, &proofs[k], nullptr, replay_graphs());
+          return G16HIP_OK;
+        });
+        if (st[k] != G16HIP_OK) msg[k] = g16hip_last_error();
+      }
+    }
     {
       std::lock_guard<std::mutex> lk(srv->mu);
-      req->status = st;
-      if (st != G16HIP_OK) req->error = g16hip_last_error();
-      req->latency_ms = req->submitted.ms();
-      req->done = true;
+      for (size_t k = 0; k < m; ++k) {
+        Request& req = *group[k];
+        req.status = st[k];
+        req.error = msg[k];
+        if (st[k] == G16HIP_OK) req.proof = proofs[k];
+        req.latency_ms = req.submitted.ms();
+        req.done = true;
+      }
     }
     srv->cv_done.notify_all();
   }
@@ -1180,35 +1521,41 @@ g16hip_status g16hip_prove_batch(const g16hip_batch_item* items, size_t n, g16hi
       if (!items[i].circuit) throw ArgError{"null circuit in batch item " + std::to_string(i)};
       by_circuit[items[i].circuit].push_back(i);
     }
-    std::vector<g16hip_status> st(n, G16HIP_OK);
+    // Every item starts as not proved; a circuit's worker overwrites each status it settles.
+    std::vector<g16hip_status> st(n, G16HIP_DEVICE);
     std::vector<std::string> msg(n);
     std::vector<std::unique_ptr<HostTask>> workers;
-    std::vector<std::unique_ptr<std::atomic<size_t>>> cursors;
     for (auto& kv : by_circuit) {
       g16hip_circuit* C = kv.first;
       const std::vector<size_t>* list = &kv.second;
-      cursors.push_back(std::make_unique<std::atomic<size_t>>(0));
-      std::atomic<size_t>* cur = cursors.back().get();
-      size_t nslots;
-      {
-        std::lock_guard<std::mutex> lk(C->mu);
-        nslots = C->slots.size();
-      }
-      size_t nw = std::min(nslots, list->size());
-      for (size_t k = 0; k < nw; ++k)
-        workers.push_back(std::make_unique<HostTask>([=, &st, &msg] {
-          (void)hipSetDevice(C->dev->ordinal);
-          for (size_t j; (j = cur->fetch_add(1)) < list->size();) {
-            size_t i = (*list)[j];
-            st[i] = guarded([&] {
-              SlotLease lease(C);
-              prove_on_slot(C, lease.S, items[i].witness, items[i].n_witness, false, items[i].r,
-                            items[i].s, &proofs[i], nullptr, replay_graphs());
-              return G16HIP_OK;
-            });
-            if (st[i] != G16HIP_OK) msg[i] = g16hip_last_error();
+      workers.push_back(std::make_unique<HostTask>([=, &st, &msg] {
+        (void)hipSetDevice(C->dev->ordinal);
+        if (can_wave(C)) {
+          // One worker per circuit: its waves, on the circuit's own stream.
+          const g16hip_status w = guarded([&] {
+            prove_waves(C, items, *list, proofs, st.data(), msg.data());
+            return G16HIP_OK;
+          });
+          if (w != G16HIP_OK) {
+            const std::string why = g16hip_last_error();
+            for (size_t i : *list)
+              if (st[i] == G16HIP_DEVICE && msg[i].empty()) {
+                st[i] = w;
+                msg[i] = why;
+              }
           }
-        }));
+          return;
+        }
+        for (size_t i : *list) {  // rows-only and gnark-basis keys: proof by proof on a slot
+          st[i] = guarded([&] {
+            SlotLease lease(C);
+            prove_on_slot(C, lease.S, items[i].witness, items[i].n_witness, false, items[i].r,
+                          items[i].s, &proofs[i], nullptr, false);
+            return G16HIP_OK;
+          });
+          if (st[i] != G16HIP_OK) msg[i] = g16hip_last_error();
+        }
+      }));
     }
     for (auto& w : workers) w->join();
     g16hip_status first = G16HIP_OK;
@@ -1221,6 +1568,73 @@ g16hip_status g16hip_prove_batch(const g16hip_batch_item* items, size_t n, g16hi
       }
     }
     if (first != G16HIP_OK) throw StatusError{first, first_msg};
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_set_batch_cap(g16hip_circuit* C, uint32_t cap) {
+  return guarded([&] {
+    if (!C) throw ArgError{"null circuit"};
+    if (cap > 64) throw ArgError{"at most 64 proofs per wave"};
+    std::lock_guard<std::mutex> lk(C->wave_mu);
+    C->wave_cap = cap;
+    return G16HIP_OK;
+  });
+}
+
+g16hip_status g16hip_batch_autotune(g16hip_circuit* C, const uint8_t* w, size_t n,
+                                    uint32_t max_cap, uint32_t* caps, double* rates,
+                                    size_t len, size_t* tried, uint32_t* best) {
+  return guarded([&] {
+    if (!C || !w || !best) throw ArgError{"null argument"};
+    if (!can_wave(C)) throw ArgError{"batched proving needs the key's matrices and snarkjs' H basis"};
+    check_witness_len(C, n);
+    if (!is_one(fr_load(w))) throw StatusError{G16HIP_CONSTANT_WIRE, "w[0] is the constant-one wire and must be 1"};
+    set_device(C->dev);
+    max_cap = std::max(1u, std::min(max_cap ? max_cap : 32u, 64u));
+    uint8_t r8[32] = {}, s8[32] = {};
+    const Fr r = fr_from_u64(0x5eed1), s = fr_from_u64(0x5eed2);
+    std::memcpy(r8, r.v, 32);
+    std::memcpy(s8, s.v, 32);
+    uint32_t best_cap = 1;
+    double best_rate = 0;
+    size_t k = 0;
+    int worse = 0;
+    for (uint32_t cap = 1; cap <= max_cap; cap *= 2) {
+      {
+        std::lock_guard<std::mutex> lk(C->wave_mu);
+        if (wave_bytes_for(C, cap) > device_budget(C->dev)) break;
+        C->wave_cap = cap;
+      }
+      // one warm wave, then two timed ones, all of the caller's witness
+      std::vector<g16hip_batch_item> items(2 * cap, g16hip_batch_item{C, w, n, r8, s8});
+      std::vector<g16hip_proof> pr(2 * cap);
+      std::vector<g16hip_status> stv(2 * cap);
+      if (g16hip_prove_batch(items.data(), cap, pr.data(), stv.data()) != G16HIP_OK)
+        throw StatusError{stv[0], g16hip_last_error()};
+      Timer t;
+      if (g16hip_prove_batch(items.data(), 2 * cap, pr.data(), stv.data()) != G16HIP_OK)
+        throw StatusError{stv[0], g16hip_last_error()};
+      const double rate = 2.0 * cap / (t.us() * 1e-6);
+      if (caps && rates && k < len) {
+        caps[k] = cap;
+        rates[k] = rate;
+      }
+      ++k;
+      if (rate > best_rate * 1.02) {
+        best_rate = rate;
+        best_cap = cap;
+        worse = 0;
+      } else if (++worse >= 2) {
+        break;
+      }
+    }
+    {
+      std::lock_guard<std::mutex> lk(C->wave_mu);
+      C->wave_cap = best_cap;
+    }
+    if (tried) *tried = k;
+    *best = best_cap;
     return G16HIP_OK;
   });
 }

@@ -587,8 +587,7 @@ static void ntt_on_device(g16hip_device* d, void* data, int L, int kind) {
     G16_CHECK(hipMemcpyAsync(data, d->scratch_c.p, 32 * n, hipMemcpyDeviceToDevice, s));
     launch_scale(data, n, sc + 3, s);
   } else {
-    ntt_dif_inverse(data, 1, L, t.inv.p, s);
-    ntt_dit_forward(data, 1, L, t.fwd.p, t.coset.p, s);
+    ntt_coset_pipeline(data, 1, L, t.inv.p, t.fwd.p, t.coset.p, s);
   }
 }
 

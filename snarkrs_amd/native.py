@@ -227,6 +227,12 @@ EXPORTS = {
     "g16hip_set_graph_replay": (None, [ctypes.c_int]),
     "g16hip_prove_batch": (ctypes.c_int, [ctypes.POINTER(BatchItem), ctypes.c_size_t,
                                           ctypes.POINTER(ProofOut), ctypes.POINTER(ctypes.c_int)]),
+    "g16hip_set_batch_cap": (ctypes.c_int, [_vp, ctypes.c_uint32]),
+    "g16hip_batch_autotune": (ctypes.c_int, [_vp, _u8p, ctypes.c_size_t, ctypes.c_uint32,
+                                             ctypes.POINTER(ctypes.c_uint32),
+                                             ctypes.POINTER(ctypes.c_double), ctypes.c_size_t,
+                                             ctypes.POINTER(ctypes.c_size_t),
+                                             ctypes.POINTER(ctypes.c_uint32)]),
     "g16hip_server_open": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_uint32,
                                           ctypes.POINTER(_vp)]),
     "g16hip_server_submit": (ctypes.c_int, [_vp, _vp, _u8p, ctypes.c_size_t, _u8p, _u8p,
@@ -478,6 +484,21 @@ class Circuit:
     def resident_bytes(self) -> int:
         return int(self.lib.g16hip_circuit_resident_bytes(self.h))
 
+    def set_batch_cap(self, cap: int) -> None:
+        """Proofs per wave in g16hip_prove_batch (0 = the library's default)."""
+        _check(self.lib.g16hip_set_batch_cap(self.h, cap))
+
+    def batch_autotune(self, witness: np.ndarray, max_cap: int = 32) -> dict:
+        """Measures batched proofs/s per wave size on this circuit and keeps the best."""
+        w = _c8(witness).reshape(-1, 32)
+        caps = (ctypes.c_uint32 * 8)()
+        rates = (ctypes.c_double * 8)()
+        tried, best = ctypes.c_size_t(), ctypes.c_uint32()
+        _check(self.lib.g16hip_batch_autotune(self.h, _p8(w), len(w), max_cap, caps, rates, 8,
+                                              ctypes.byref(tried), ctypes.byref(best)))
+        return {"best": best.value,
+                "tried": {int(caps[i]): float(rates[i]) for i in range(min(tried.value, 8))}}
+
     def add_slots(self, count: int, reserve_bytes: int = 8 << 30) -> int:
         """More proofs in flight on this circuit, admitted while device memory allows."""
         total = ctypes.c_uint32()
@@ -653,9 +674,10 @@ def set_graph_replay(on: bool) -> None:
     load().g16hip_set_graph_replay(1 if on else 0)
 
 
-def prove_batch(items) -> list:
+def prove_batch(items, statuses: bool = False):
     """items: (circuit, witness, r, s) tuples (arrays as for prove_with_blinders). Returns the
-    proofs in order; raises on the first failure."""
+    proofs in order and raises on the first failure; with statuses=True returns (proofs,
+    per-item statuses) instead, a failed item's proof left zero."""
     lib = load()
     keep = []
     arr = (BatchItem * len(items))()
@@ -665,8 +687,12 @@ def prove_batch(items) -> list:
         arr[i] = BatchItem(c.h, _p8(w), len(w), _p8(r), _p8(s))
     out = (ProofOut * len(items))()
     st = (ctypes.c_int * len(items))()
-    _check(lib.g16hip_prove_batch(arr, len(items), out, st))
-    return [out[i].as_array() for i in range(len(items))]
+    rc = lib.g16hip_prove_batch(arr, len(items), out, st)
+    proofs = [out[i].as_array() for i in range(len(items))]
+    if statuses:
+        return proofs, [int(st[i]) for i in range(len(items))]
+    _check(rc)
+    return proofs
 
 
 class Server:

@@ -102,3 +102,58 @@ def test_batch_reports_bad_items(dev, circuits):
     r, s = _blind(1)
     with pytest.raises(native.WitnessLength):
         native.prove_batch([(circ, wit[:-1], r, s)])
+
+
+@pytest.mark.parametrize("cap", [1, 3, 16])
+def test_waves_of_every_size_match_single_proofs(dev, circuits, cap):
+    # 7 witnesses of one circuit: full waves and a partial last wave (cap 3: 3 + 3 + 1)
+    syn, key, _, circ = circuits[1]
+    circ.set_batch_cap(cap)
+    try:
+        items, want = [], []
+        for i in range(7):
+            w = syn.witness(3100 + i)
+            r, s = _blind(300 + i)
+            items.append((circ, w, r, s))
+            want.append(circ.prove_with_blinders(w, r, s))
+        for rep in range(2):  # the wave's buffers are reused (dirty) the second time
+            got = native.prove_batch(items)
+            for i, (g, w) in enumerate(zip(got, want)):
+                assert (g == w).all(), (cap, rep, i)
+    finally:
+        circ.set_batch_cap(0)
+
+
+def test_bad_items_inside_a_wave_keep_their_status(dev, circuits):
+    syn, key, wit, circ = circuits[2]
+    circ.set_batch_cap(4)
+    try:
+        bad = wit.copy()
+        bad[0] = fr_array([2])[0]
+        items = []
+        for i in range(5):
+            r, s = _blind(400 + i)
+            items.append((circ, bad if i == 1 else syn.witness(4100 + i),
+                          fr_array([0]) if i == 3 else r, s))
+        proofs, st = native.prove_batch(items, statuses=True)
+        assert st[1] == native.CONSTANT_WIRE and st[3] == native.ZERO_BLINDER
+        for i in (0, 2, 4):
+            assert st[i] == native.OK
+            c, w, r, s = items[i]
+            assert (proofs[i] == circ.prove_with_blinders(w, r, s)).all(), i
+        with pytest.raises(native.ConstantWire):
+            native.prove_batch(items)
+    finally:
+        circ.set_batch_cap(0)
+
+
+def test_autotune_picks_a_measured_wave_size(dev, orc, circuits):
+    syn, key, wit, circ = circuits[2]
+    res = circ.batch_autotune(wit, 8)
+    assert res["best"] in res["tried"] and set(res["tried"]) <= {1, 2, 4, 8}
+    assert all(v > 0 for v in res["tried"].values())
+    items = [(circ, syn.witness(5100 + i), *_blind(500 + i)) for i in range(6)]
+    got = native.prove_batch(items)
+    for i, (c, w, r, s) in enumerate(items):
+        assert (got[i] == orc.prove(key, w, fr_ints(r)[0], fr_ints(s)[0])[1]).all(), i
+    circ.set_batch_cap(0)
