@@ -357,6 +357,21 @@ g16hip_status g16hip_msm_g1_device(g16hip_device* dev, const void* d_bases, cons
 g16hip_status g16hip_msm_g2_device(g16hip_device* dev, const void* d_bases, const void* d_scalars,
                                    size_t n, int c, uint8_t* out);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc). The device MSM of
+ * g16hip_msm_g1_device / g16hip_msm_g2_device (group 1 or 2, same arguments, same result bit
+ * for bit) with every piece of its scratch (the sort's digits, counts and entries, the
+ * accumulation's buckets and segment pieces, the bases in the MSM form, the window sums) in a
+ * static device array of G16HIP_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds
+ * until the process ends; it calls no hipMalloc or hipFree. An MSM whose scratch exceeds the
+ * array is G16HIP_DEVICE_MEMORY (2^17 G1 or 2^16 G2 points at the library's width fit). Calls
+ * through one device handle serialise; calls that overlap on one GPU through different handles
+ * share the array and race. The array keeps the last call's contents; nothing in it needs to
+ * start at zero. */
+#define G16HIP_NOALLOC_SCRATCH_BYTES ((size_t)72 << 20)
+g16hip_status g16hip_msm_device_noalloc(g16hip_device* dev, int group, const void* d_bases,
+                                        const void* d_scalars, size_t n, int c, uint8_t* out);
+
 /* Work::Constant (crates/cuda/src/msm.rs:143-164, CudaBackend::with_constant_work,
  * crates/cuda/src/backend.rs:100-118), opt in: for standalone MSMs on dev and circuits
  * prepared on dev after the call, every (scalar, window) emits an entry, zero digits into

@@ -244,6 +244,8 @@ EXPORTS = {
     "g16hip_msm_g2": (ctypes.c_int, [_vp, _u8p, _u8p, ctypes.c_size_t, _u8p]),
     "g16hip_msm_g1_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t, ctypes.c_int, _u8p]),
     "g16hip_msm_g2_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t, ctypes.c_int, _u8p]),
+    "g16hip_msm_device_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_size_t,
+                                                 ctypes.c_int, _u8p]),
     "g16hip_msm_last_stats": (ctypes.c_int, [_vp, ctypes.POINTER(MsmStats)]),
     "g16hip_kernel_stats_get": (ctypes.c_int, [_vp, ctypes.POINTER(KernelStats)]),
     "g16hip_kernel_stats_reset": (ctypes.c_int, [_vp]),
@@ -396,6 +398,14 @@ class Device:
         out = np.zeros(w, dtype=np.uint8)
         fn = self.lib.g16hip_msm_g1_device if group == 1 else self.lib.g16hip_msm_g2_device
         _check(fn(self.h, _vp(d_bases), _vp(d_scalars), n, c, _p8(out)))
+        return out
+
+    def msm_device_noalloc(self, group: int, d_bases: int, d_scalars: int, n: int,
+                           c: int = 0) -> np.ndarray:
+        """msm_device with all scratch in the library's static device array (no allocation)."""
+        out = np.zeros(64 if group == 1 else 128, dtype=np.uint8)
+        _check(self.lib.g16hip_msm_device_noalloc(self.h, group, _vp(d_bases), _vp(d_scalars), n,
+                                                  c, _p8(out)))
         return out
 
     def set_constant_work(self, on: bool) -> None:
