@@ -368,7 +368,7 @@ g16hip_status g16hip_msm_g2_device(g16hip_device* dev, const void* d_bases, cons
  * through one device handle serialise; calls that overlap on one GPU through different handles
  * share the array and race. The array keeps the last call's contents; nothing in it needs to
  * start at zero. */
-#define G16HIP_NOALLOC_SCRATCH_BYTES ((size_t)72 << 20)
+#define G16HIP_NOALLOC_SCRATCH_BYTES ((size_t)96 << 20)
 g16hip_status g16hip_msm_device_noalloc(g16hip_device* dev, int group, const void* d_bases,
                                         const void* d_scalars, size_t n, int c, uint8_t* out);
 

@@ -83,7 +83,13 @@ struct MsmPlan {
   // every vector has bucket sets of its own (set index k, or k windows + w per window).
   uint32_t batch = 1;
   size_t max_entries = 0; // batch * n * windows
-  uint32_t chunks = 0;    // scalar chunks of the counting sort
+  // The two-level sort: coarse bins (bucket >> shift) per set, digit rows per set (windows
+  // when merged, else 1), scalars per chunk, chunks, and the count array's length.
+  // The fine level: coarse groups (sets * cbins) and an upper bound on their slices.
+  int shift = 0;
+  uint32_t cbins = 1, rows_per_set = 1, chunk = 0, chunks = 0;
+  size_t q = 0;
+  uint32_t groups = 0, max_slices = 0;
   // Merged windows: the bases come as window tables (entry w n + i = 2^(c w) P_i), so the
   // digits of every window fall into one set of nb buckets and the window sum is the MSM.
   bool merged = false;
@@ -93,14 +99,16 @@ struct MsmPlan {
   // Entry values are then index << 2 | discard << 1 | sign (index < 2^30).
   bool constant = false;
   // device scratch (owned by the caller)
-  uint32_t* count = nullptr;    // [batch*windows*nb + 1] entries per (vector, window, bucket)
   uint32_t* offset = nullptr;   // [K + 1] first entry of each bucket (K = sets * nb); [K] = total
   uint32_t* block_sums = nullptr;
-  uint2* ent = nullptr;         // [max_entries] sorted entries: {key = window * nb + bucket,
+  uint32_t* ccount = nullptr;   // [q + 1] entries per (set, coarse bin, row of the set, chunk)
+  uint32_t* cpos = nullptr;     // [q + 1] their exclusive prefix: each run's first position
+  uint2* ent = nullptr;         // [max_entries] sorted entries: {key = set * nb + bucket,
                                 //                 value = index << 1 | sign}
-  uint32_t* digits = nullptr;   // [windows * n] encoded signed digits (16 or 32-bit), window-major
-  uint32_t* cursor = nullptr;   // [batch*windows*nb] the scatter's next position per key
-  uint32_t* btotal = nullptr;   // [batch*nb + 1] merged: entries per bucket over all windows
+  uint2* tmp = nullptr;         // [max_entries] coarse-sorted entries {fine bits, value}
+  uint32_t* sprefix = nullptr;  // [groups + 1] slices before each coarse group
+  uint32_t* fcnt = nullptr;     // [2^shift * max_slices] fine counts, then positions
+  uint32_t* digits = nullptr;   // [rows * n] encoded signed digits (16 or 32-bit), row-major
   uint32_t* total = nullptr;    // [4] entry count
 };
 // Bucket sets the accumulation and reduction work on: one per window, or one when merged, for

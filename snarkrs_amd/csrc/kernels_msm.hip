@@ -3,11 +3,9 @@
 // Pipeline per scalar vector (shared by every base vector multiplied by the same scalars,
 // so the witness MSMs A, B1, B2 and L sort once):
 //   1. digits:  Montgomery -> standard scalar, carry-free signed digits per window
-//               (crates/msm/src/lib.rs:275-331), stored window-major;
-//   2. sort:    a counting sort of (window, |digit|) keys without global atomics: each
-//               workgroup counts one chunk of one window in LDS, a column scan over the
-//               chunks and a scan over the buckets place every (bucket, chunk) pair, and the
-//               scatter ranks a chunk's entries with LDS atomics;
+//               (crates/msm/src/lib.rs:275-331), stored window-major (16-bit up to c = 15);
+//   2. sort:    a two-level counting sort of (set, bucket) keys, coarse bins then fine bins,
+//               whose writes all land in runs (see "The sort has two levels" below);
 //   3. accumulate: one thread per fixed-length segment of the sorted entries adds its runs
 //               into XYZZ accumulators (madd-2008-s); a run that fills its bucket inside
 //               the segment is stored as the bucket, the partial runs at a segment's ends go
@@ -68,6 +66,7 @@ template <> struct Digit<uint32_t> {
     return d == 0 ? zero : ((uint32_t)(d < 0 ? -d : d) - 1u) | (d < 0 ? 1u << 16 : 0u);
   }
   static __device__ __forceinline__ bool zero(uint32_t e) { return (e & DIGIT_ZERO) != 0; }
+  static __device__ __forceinline__ uint32_t none() { return DIGIT_ZERO; }
   static __device__ __forceinline__ uint32_t bucket(uint32_t e) { return e & 0xFFFFu; }
   static __device__ __forceinline__ uint32_t sign(uint32_t e) { return (e >> 16) & 1u; }
   static __device__ __forceinline__ uint32_t discard(uint32_t e) {
@@ -80,6 +79,7 @@ template <> struct Digit<uint16_t> {
                              : ((uint32_t)(d < 0 ? -d : d) - 1u) | (d < 0 ? 1u << 15 : 0u));
   }
   static __device__ __forceinline__ bool zero(uint16_t e) { return e == 0x7FFFu; }
+  static __device__ __forceinline__ uint16_t none() { return (uint16_t)0x7FFFu; }
   static __device__ __forceinline__ uint32_t bucket(uint16_t e) { return e & 0x7FFFu; }
   static __device__ __forceinline__ uint32_t sign(uint16_t e) { return (uint32_t)e >> 15; }
   static __device__ __forceinline__ uint32_t discard(uint16_t) { return 0; }
@@ -88,11 +88,47 @@ template <> struct Digit<uint16_t> {
 // Whether a plan's digits are 16-bit.
 static bool short_digits(const MsmPlan& p) { return p.c <= 15 && !p.constant; }
 
-// Scalars per counting-sort workgroup: a bucket's count within a chunk fits 16 bits, so the
-// count kernel's LDS histogram packs two buckets per word (64 KB at c = 16).
-static constexpr uint32_t SORT_CHUNK = 32768;
+// The sort has two levels, so that every write lands in a run instead of at a random place:
+//   1. coarse: each (chunk of SORT_CHUNK scalars, digit row) workgroup counts its digits per
+//      coarse bin (bucket >> shift) in LDS; one exclusive scan over all the counts, laid out
+//      group-major ([set][coarse bin][row of the set][chunk]), gives every (chunk, row, bin) its
+//      run in a staging array and every coarse group its range; the scatter writes each entry
+//      (fine bits, value) into its run;
+//   2. fine: each coarse group is cut into slices of FINE_TILE entries; a workgroup per slice
+//      counts its entries per fine bin (bucket & mask), a workgroup per group scans its
+//      slices' counts into positions and bucket offsets, and a workgroup per slice writes its
+//      entries with their full keys, in order of bucket, inside the group's range.
+// A set is a digit row (per-window buckets) or a vector's W rows (merged windows). Entries of a
+// bucket come out in any order; its sum does not depend on it.
+static constexpr uint32_t SORT_CHUNK = 65536;  // scalars per coarse workgroup (at least)
 static constexpr int SORT_THREADS = 1024;
-static constexpr uint32_t SCATTER_SPAN = 16384;  // buckets per scatter workgroup (64 KB LDS)
+static constexpr int SORT_ITEMS = 8;           // digits per thread per round (loads first)
+static constexpr int FINE_THREADS = 256;
+static constexpr int FINE_ITEMS = 16;
+static constexpr uint32_t FINE_TILE = FINE_THREADS * FINE_ITEMS;  // entries per fine slice
+static constexpr uint32_t SCAN_BLOCK = 4096;   // elements per scan workgroup (256 x 16)
+static constexpr uint32_t GROUP_TARGET = 8192; // entries per coarse group at full density
+static constexpr uint32_t MAX_CBINS = 4096;    // coarse bins per set (LDS counters)
+static constexpr int MAX_FINE_SHIFT = 12;      // fine bins per coarse group <= 4096
+
+// ctr[key] += 1 from every active lane, returning each lane's old value plus its rank among
+// the lanes that share its key. When at least a quarter of the wave shares the first active
+// lane's key (a heavy bucket: the ones of a boolean witness all land in bucket 0), those lanes
+// take one atomic together instead of 16+ serialised ones on the same LDS word.
+__device__ __forceinline__ uint32_t lds_take(uint32_t* ctr, uint32_t key) {
+  const uint32_t lane = __lane_id();
+  const uint64_t active = __ballot(1);
+  const int first = __ffsll((unsigned long long)active) - 1;
+  const uint32_t k0 = __shfl(key, first);
+  const uint64_t same = __ballot(key == k0);
+  if (__popcll(same) >= 16) {
+    uint32_t base = 0;
+    if ((int)lane == first) base = atomicAdd(&ctr[k0], (uint32_t)__popcll(same));
+    base = __shfl(base, first);
+    if (key == k0) return base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+  }
+  return atomicAdd(&ctr[key], 1u);
+}
 
 template <class D>
 __global__ void __launch_bounds__(256) k_digits(const uint4* scalars, uint32_t n, uint32_t total,
@@ -107,122 +143,252 @@ __global__ void __launch_bounds__(256) k_digits(const uint4* scalars, uint32_t n
   for (int w = 0; w < W; ++w) row[(size_t)w * n] = Digit<D>::make(signed_digit(s.v, w, c), zero);
 }
 
-// Entries per (window, bucket): chunk blockIdx.x of window blockIdx.y counts its digits in an
-// LDS histogram and adds each nonzero bin to count[w nb + b] (cleared beforehand).
 template <class D>
-__global__ void __launch_bounds__(SORT_THREADS) k_chunk_count(const D* digits, uint32_t n,
-                                                              uint32_t nb, uint32_t* count) {
-  extern __shared__ uint32_t hist[];  // nb / 2 words
-  uint32_t p = blockIdx.x, w = blockIdx.y;
-  uint32_t words = nb / 2;
-  for (uint32_t j = threadIdx.x; j < words; j += SORT_THREADS) hist[j] = 0;
+__global__ void __launch_bounds__(SORT_THREADS) k_coarse_count(const D* digits, uint32_t n,
+                                                               uint32_t cs, int shift,
+                                                               uint32_t cbins,
+                                                               uint32_t rows_per_set,
+                                                               uint32_t chunks,
+                                                               uint32_t* ccount) {
+  extern __shared__ uint32_t hist[];  // cbins words
+  const uint32_t p = blockIdx.x, row = blockIdx.y;
+  for (uint32_t j = threadIdx.x; j < cbins; j += SORT_THREADS) hist[j] = 0;
   __syncthreads();
-  size_t lo = (size_t)p * SORT_CHUNK;
-  size_t hi = lo + SORT_CHUNK < n ? lo + SORT_CHUNK : n;
-  const D* d = digits + (size_t)w * n;
-  for (size_t i = lo + threadIdx.x; i < hi; i += SORT_THREADS) {
-    const D e = d[i];
-    if (Digit<D>::zero(e)) continue;
-    const uint32_t b = Digit<D>::bucket(e);
-    atomicAdd(&hist[b >> 1], 1u << ((b & 1u) << 4));
+  const size_t lo = (size_t)p * cs;
+  const size_t hi = lo + cs < n ? lo + cs : n;
+  const D* d = digits + (size_t)row * n;
+  for (size_t base = lo; base < hi; base += SORT_THREADS * SORT_ITEMS) {
+    D e[SORT_ITEMS];
+#pragma unroll
+    for (int k = 0; k < SORT_ITEMS; ++k) {
+      const size_t i = base + k * SORT_THREADS + threadIdx.x;
+      e[k] = i < hi ? d[i] : Digit<D>::none();
+    }
+#pragma unroll
+    for (int k = 0; k < SORT_ITEMS; ++k)
+      if (!Digit<D>::zero(e[k])) lds_take(hist, Digit<D>::bucket(e[k]) >> shift);
   }
   __syncthreads();
-  uint32_t* out = count + (size_t)w * nb;
-  for (uint32_t j = threadIdx.x; j < words; j += SORT_THREADS) {
-    const uint32_t v = hist[j];
-    if (v & 0xFFFFu) atomicAdd(&out[2 * j], v & 0xFFFFu);
-    if (v >> 16) atomicAdd(&out[2 * j + 1], v >> 16);
-  }
+  const uint32_t set = row / rows_per_set, r = row - set * rows_per_set;
+  for (uint32_t j = threadIdx.x; j < cbins; j += SORT_THREADS)
+    ccount[(((size_t)set * cbins + j) * rows_per_set + r) * chunks + p] = hist[j];
 }
 
-template <int = 0>
-__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
-
-// The scatter's global cursors: the first position of key t = (vector k, window w) nb + b,
-// from the bucket offsets and, merged (wpre given), the entries of the vector's earlier windows
-// in that bucket (whose offsets are per (k, b)).
-__global__ void k_cursor_init(uint32_t* cursor, const uint32_t* offset, const uint32_t* wpre,
-                              uint32_t nb, int W, uint32_t K) {
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= K) return;
-  const uint32_t vw = t / nb, b = t - vw * nb;
-  cursor[t] = wpre ? offset[(vw / (uint32_t)W) * nb + b] + wpre[t] : offset[t];
-}
-
-// Merged windows: per (vector k, bucket b), count[(k W + w) nb + b] becomes the entries of the
-// vector's windows before w and btotal[k nb + b] the bucket's total over all its windows.
-__global__ void __launch_bounds__(256) k_window_scan(uint32_t* count, uint32_t nb, int W,
-                                                     uint32_t sets, uint32_t* btotal) {
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= sets * nb) return;
-  const uint32_t k = t / nb, b = t - k * nb;
-  uint32_t* col = count + (size_t)k * W * nb + b;
-  uint32_t run = 0;
-  for (int w = 0; w < W; ++w) {
-    uint32_t c = col[(size_t)w * nb];
-    col[(size_t)w * nb] = run;
-    run += c;
-  }
-  btotal[t] = run;
-}
-
-// Entries of chunk blockIdx.x of digit row blockIdx.y = (vector k, window w) whose bucket lies
-// in [blockIdx.z * span, + span), in three steps over one LDS array: a histogram of the chunk's
-// digits; each nonzero bin reserves its run of positions with one atomic on the global cursor
-// of its key (the runs of a bucket's chunks land in any order; the bucket's sum does not depend
-// on it); then each entry takes the next position of its bin. Merged: a vector's windows share
-// its one bucket set (key k nb + b) and an entry's value indexes the window table (w n + i).
+// Each entry of (chunk p, row) into its coarse bin's run of the staging array:
+// tmp[pos] = {fine bits of the bucket, value}.
 template <class D>
-__global__ void __launch_bounds__(SORT_THREADS) k_chunk_scatter(
-    const D* digits, uint32_t n, uint32_t nb, int W, uint32_t span, uint32_t* cursor, int merged,
-    uint2* ent, int constant) {
-  extern __shared__ uint32_t cur[];  // span words
-  const uint32_t p = blockIdx.x, w = blockIdx.y, b0 = blockIdx.z * span;  // w: digit row
-  const uint32_t k = w / (uint32_t)W, win = w - k * (uint32_t)W;
-  for (uint32_t j = threadIdx.x; j < span; j += SORT_THREADS) cur[j] = 0;
+__global__ void __launch_bounds__(SORT_THREADS) k_coarse_scatter(
+    const D* digits, uint32_t n, uint32_t cs, int shift, uint32_t cbins, uint32_t rows_per_set,
+    uint32_t chunks, int W, const uint32_t* cpos, uint2* tmp, int merged, int constant) {
+  extern __shared__ uint32_t cur[];  // cbins words
+  const uint32_t p = blockIdx.x, row = blockIdx.y;
+  const uint32_t set = row / rows_per_set, r = row - set * rows_per_set;
+  for (uint32_t j = threadIdx.x; j < cbins; j += SORT_THREADS)
+    cur[j] = cpos[(((size_t)set * cbins + j) * rows_per_set + r) * chunks + p];
   __syncthreads();
-  size_t lo = (size_t)p * SORT_CHUNK;
-  size_t hi = lo + SORT_CHUNK < n ? lo + SORT_CHUNK : n;
-  const D* d = digits + (size_t)w * n;
-  for (size_t i = lo + threadIdx.x; i < hi; i += SORT_THREADS) {
-    const D e = d[i];
-    if (Digit<D>::zero(e)) continue;
-    const uint32_t b = Digit<D>::bucket(e) - b0;
-    if (b < span) atomicAdd(&cur[b], 1u);
-  }
-  __syncthreads();
-  uint32_t* gc = cursor + (size_t)w * nb + b0;
-  for (uint32_t j = threadIdx.x; j < span; j += SORT_THREADS) {
-    const uint32_t c = cur[j];
-    if (c) cur[j] = atomicAdd(&gc[j], c);
-  }
-  __syncthreads();
-  for (size_t i = lo + threadIdx.x; i < hi; i += SORT_THREADS) {
-    const D e = d[i];
-    if (Digit<D>::zero(e)) continue;
-    const uint32_t b = Digit<D>::bucket(e) - b0;
-    if (b >= span) continue;
-    const uint32_t pos = atomicAdd(&cur[b], 1u);
-    const uint32_t key = (merged ? k : w) * nb + b0 + b;
-    const uint32_t idx = merged ? win * n + (uint32_t)i : (uint32_t)i;
-    const uint32_t sign = Digit<D>::sign(e);
-    ent[pos] = make_uint2(key, constant ? (idx << 2) | Digit<D>::discard(e) | sign
-                                        : (idx << 1) | sign);
+  const uint32_t win = row % (uint32_t)W;  // window of this row (rows are vector-major)
+  const uint32_t mask = (1u << shift) - 1u;
+  const size_t lo = (size_t)p * cs;
+  const size_t hi = lo + cs < n ? lo + cs : n;
+  const D* d = digits + (size_t)row * n;
+  for (size_t base = lo; base < hi; base += SORT_THREADS * SORT_ITEMS) {
+    D e[SORT_ITEMS];
+#pragma unroll
+    for (int k = 0; k < SORT_ITEMS; ++k) {
+      const size_t i = base + k * SORT_THREADS + threadIdx.x;
+      e[k] = i < hi ? d[i] : Digit<D>::none();
+    }
+#pragma unroll
+    for (int k = 0; k < SORT_ITEMS; ++k) {
+      if (Digit<D>::zero(e[k])) continue;
+      const uint32_t i = (uint32_t)(base + k * SORT_THREADS + threadIdx.x);
+      const uint32_t b = Digit<D>::bucket(e[k]);
+      const uint32_t pos = lds_take(cur, b >> shift);
+      const uint32_t idx = merged ? win * n + i : i;
+      const uint32_t sign = Digit<D>::sign(e[k]);
+      tmp[pos] = make_uint2(b & mask, constant ? (idx << 2) | Digit<D>::discard(e[k]) | sign
+                                               : (idx << 1) | sign);
+    }
   }
 }
 
-// Exclusive scan of count[0..K) into offset[0..K], offset[K] = total.
-// Three launches: per-1024-element block scans, a scan of the block sums, the fix-up.
+// The fine level works on slices of at most FINE_TILE entries, so that a heavy coarse group
+// (the ones of a boolean witness all land in one) is spread over many workgroups: coarse group
+// g = set cbins + bin has max(1, ceil(size / FINE_TILE)) slices, slice s of it covers entries
+// [start + s FINE_TILE, min(start + (s + 1) FINE_TILE, end)) of the staging array, and its
+// fine-bin counts sit at fcnt[sprefix[g] fb + f ns + s] (ns = the group's slice count): laid
+// out [group][fine bin][slice], one exclusive scan per group turns them into positions.
+
+// Block-wide exclusive prefix of one value per thread (blockDim.x = T, a multiple of 64);
+// returns the thread's prefix and writes the block total to *sum.
+template <int T>
+__device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t* sh, uint32_t* sum) {
+  const uint32_t lane = __lane_id(), wave = threadIdx.x / 64;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off);
+    if (lane >= (uint32_t)off) x += y;
+  }
+  if (lane == 63) sh[wave] = x;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < T / 64; ++w) {
+    const uint32_t t = sh[w];
+    before += (uint32_t)w < wave ? t : 0u;
+    all += t;
+  }
+  __syncthreads();  // sh may be reused by the caller
+  *sum = all;
+  return before + x - v;
+}
+
+// sprefix[g] = slices before group g, sprefix[groups] = all slices (one workgroup).
+__global__ void __launch_bounds__(1024) k_slice_scan(const uint32_t* cpos, uint32_t per_group,
+                                                     uint32_t groups, uint32_t* sprefix) {
+  __shared__ uint32_t sh[16];
+  const uint32_t per = (groups + 1023) / 1024;
+  const uint32_t g0 = threadIdx.x * per;
+  uint32_t loc = 0;
+  for (uint32_t g = g0; g < g0 + per && g < groups; ++g) {
+    const uint32_t size = cpos[(size_t)(g + 1) * per_group] - cpos[(size_t)g * per_group];
+    loc += size ? (size + FINE_TILE - 1) / FINE_TILE : 1u;
+  }
+  uint32_t all;
+  uint32_t run = block_exclusive<1024>(loc, sh, &all);
+  for (uint32_t g = g0; g < g0 + per && g < groups; ++g) {
+    sprefix[g] = run;
+    const uint32_t size = cpos[(size_t)(g + 1) * per_group] - cpos[(size_t)g * per_group];
+    run += size ? (size + FINE_TILE - 1) / FINE_TILE : 1u;
+  }
+  if (threadIdx.x == 0) sprefix[groups] = all;
+}
+
+struct Slice {
+  uint32_t g, s, ns, start, end;  // group, slice, group's slices, slice's entry range
+};
+
+// The slice of workgroup b, or ns = 0 past the last slice.
+__device__ __forceinline__ Slice find_slice(uint32_t b, const uint32_t* sprefix, uint32_t groups,
+                                            const uint32_t* cpos, uint32_t per_group) {
+  Slice r{0, 0, 0, 0, 0};
+  if (b >= sprefix[groups]) return r;
+  uint32_t lo = 0, hi = groups;  // sprefix strictly increases: the last g with sprefix[g] <= b
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (sprefix[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  r.g = lo;
+  r.s = b - sprefix[lo];
+  r.ns = sprefix[lo + 1] - sprefix[lo];
+  const uint32_t start = cpos[(size_t)lo * per_group], end = cpos[(size_t)(lo + 1) * per_group];
+  r.start = start + r.s * FINE_TILE;
+  r.end = r.start + FINE_TILE < end ? r.start + FINE_TILE : end;
+  return r;
+}
+
+__global__ void __launch_bounds__(FINE_THREADS) k_fine_count(const uint2* tmp, const uint32_t* cpos,
+                                                             uint32_t per_group, uint32_t groups,
+                                                             const uint32_t* sprefix, int shift,
+                                                             uint32_t* fcnt) {
+  __shared__ uint32_t cnt[1u << MAX_FINE_SHIFT];
+  const Slice sl = find_slice(blockIdx.x, sprefix, groups, cpos, per_group);
+  if (!sl.ns) return;
+  const uint32_t fb = 1u << shift;
+  for (uint32_t j = threadIdx.x; j < fb; j += FINE_THREADS) cnt[j] = 0;
+  __syncthreads();
+  uint32_t f[FINE_ITEMS];
+#pragma unroll
+  for (int k = 0; k < FINE_ITEMS; ++k) {
+    const uint32_t i = sl.start + k * FINE_THREADS + threadIdx.x;
+    f[k] = i < sl.end ? tmp[i].x : ~0u;
+  }
+#pragma unroll
+  for (int k = 0; k < FINE_ITEMS; ++k)
+    if (f[k] != ~0u) lds_take(cnt, f[k]);
+  __syncthreads();
+  uint32_t* out = fcnt + (size_t)sprefix[sl.g] * fb + sl.s;
+  for (uint32_t j = threadIdx.x; j < fb; j += FINE_THREADS) out[(size_t)j * sl.ns] = cnt[j];
+}
+
+// Group g's counts [fine bin][slice] into positions (exclusive prefix from the group's start),
+// in place, and the offset of each of its buckets (the position of its slice 0). The last
+// group also writes offset[K] and the total.
+__global__ void __launch_bounds__(FINE_THREADS) k_fine_prefix(const uint32_t* cpos,
+                                                              uint32_t per_group, uint32_t groups,
+                                                              const uint32_t* sprefix,
+                                                              uint32_t cbins, int shift,
+                                                              uint32_t nb, uint32_t* fcnt,
+                                                              uint32_t* offset, uint32_t* total) {
+  __shared__ uint32_t sh[FINE_THREADS / 64];
+  const uint32_t g = blockIdx.x;
+  const uint32_t fb = 1u << shift;
+  const uint32_t ns = sprefix[g + 1] - sprefix[g];
+  const uint32_t m = fb * ns;
+  uint32_t* c = fcnt + (size_t)sprefix[g] * fb;
+  const uint32_t start = cpos[(size_t)g * per_group];
+  const uint32_t set = g / cbins, bin = g - set * cbins;
+  const uint32_t key0 = set * nb + (bin << shift);
+  const uint32_t per = (m + FINE_THREADS - 1) / FINE_THREADS;
+  const uint32_t j0 = threadIdx.x * per;
+  uint32_t loc = 0;
+  for (uint32_t j = j0; j < j0 + per && j < m; ++j) loc += c[j];
+  uint32_t all;
+  uint32_t run = start + block_exclusive<FINE_THREADS>(loc, sh, &all);
+  for (uint32_t j = j0; j < j0 + per && j < m; ++j) {
+    const uint32_t v = c[j];
+    c[j] = run;
+    const uint32_t f = j / ns;
+    if (j == f * ns) offset[key0 + f] = run;
+    run += v;
+  }
+  if (g + 1 == groups && threadIdx.x == 0) {  // key0 + fb = sets nb = K
+    offset[key0 + fb] = start + all;
+    total[0] = start + all;
+  }
+}
+
+// Each slice's entries ordered by bucket into ent, keys set nb + (bin << shift) + fine.
+__global__ void __launch_bounds__(FINE_THREADS) k_fine_scatter(const uint2* tmp,
+                                                               const uint32_t* cpos,
+                                                               uint32_t per_group,
+                                                               uint32_t groups,
+                                                               const uint32_t* sprefix,
+                                                               uint32_t cbins, int shift,
+                                                               uint32_t nb, const uint32_t* fcnt,
+                                                               uint2* ent) {
+  __shared__ uint32_t cur[1u << MAX_FINE_SHIFT];
+  const Slice sl = find_slice(blockIdx.x, sprefix, groups, cpos, per_group);
+  if (!sl.ns) return;
+  const uint32_t fb = 1u << shift;
+  const uint32_t* pos = fcnt + (size_t)sprefix[sl.g] * fb + sl.s;
+  for (uint32_t j = threadIdx.x; j < fb; j += FINE_THREADS) cur[j] = pos[(size_t)j * sl.ns];
+  const uint32_t set = sl.g / cbins, bin = sl.g - set * cbins;
+  const uint32_t key0 = set * nb + (bin << shift);
+  uint2 t[FINE_ITEMS];
+#pragma unroll
+  for (int k = 0; k < FINE_ITEMS; ++k) {
+    const uint32_t i = sl.start + k * FINE_THREADS + threadIdx.x;
+    t[k] = i < sl.end ? tmp[i] : make_uint2(~0u, 0u);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < FINE_ITEMS; ++k)
+    if (t[k].x != ~0u) ent[lds_take(cur, t[k].x)] = make_uint2(key0 + t[k].x, t[k].y);
+}
+
+// Exclusive scan of count[0..K) into offset[0..K], offset[K] = total (K <= 1024 SCAN_BLOCK).
+// Three launches: per-block scans, a scan of the block sums, the fix-up.
 __global__ void __launch_bounds__(256) k_scan_blocks(const uint32_t* count, uint32_t K,
                                                      uint32_t* offset, uint32_t* bsum) {
+  constexpr int PER = SCAN_BLOCK / 256;
   __shared__ uint32_t sh[256];
-  uint32_t base = blockIdx.x * 1024 + threadIdx.x * 4;
-  uint32_t v[4], loc = 0;
+  const uint32_t base = blockIdx.x * SCAN_BLOCK + threadIdx.x * PER;
+  uint32_t v[PER], loc = 0;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < PER; ++k) {
     v[k] = base + k < K ? count[base + k] : 0;
     loc += v[k];
   }
@@ -236,7 +402,7 @@ __global__ void __launch_bounds__(256) k_scan_blocks(const uint32_t* count, uint
   }
   uint32_t run = sh[threadIdx.x] - loc;  // exclusive prefix within the block
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < PER; ++k) {
     if (base + k < K) offset[base + k] = run;
     run += v[k];
   }
@@ -259,14 +425,16 @@ __global__ void __launch_bounds__(1024) k_scan_top(uint32_t* bsum, uint32_t nblk
   if (t == 0) bsum[1024] = sh[1023];   // grand total
 }
 
-__global__ void k_scan_fix(uint32_t* offset, const uint32_t* bsum, uint32_t K,
-                           uint32_t* total) {
+template <int = 0>
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_scan_fix(uint32_t* offset, const uint32_t* bsum, uint32_t K) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < K) offset[i] += bsum[i / 1024];
-  if (i == 0) {
-    offset[K] = bsum[1024];
-    total[0] = bsum[1024];
-  }
+  if (i < K) offset[i] += bsum[i / SCAN_BLOCK];
+  if (i == 0) offset[K] = bsum[1024];
 }
 
 int msm_choose_c(uint32_t n, int g1, int g2) {
@@ -313,16 +481,65 @@ int msm_choose_c_merged(uint32_t n, int g1, int g2) {
 
 static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
-static uint32_t sort_chunks(uint32_t n) { return (n + SORT_CHUNK - 1) / SORT_CHUNK; }
+// The sort's geometry: coarse bins per set about (entries of a set) / GROUP_TARGET (fine bins
+// per coarse bin at most 2^MAX_FINE_SHIFT, coarse bins at most MAX_CBINS), and chunks of at
+// least SORT_CHUNK scalars, doubled until the count array fits the scan (1024 SCAN_BLOCK).
+struct SortGeom {
+  int shift = 0;
+  uint32_t cbins = 1, rows_per_set = 1, sets = 1, rows = 1, cs = SORT_CHUNK, chunks = 1;
+  size_t q = 0;  // counts: rows * cbins * chunks
+  uint32_t groups = 1, max_slices = 1;  // coarse groups, and slices at most (one per group
+                                        // plus one per FINE_TILE entries)
+  size_t fine_words() const { return ((size_t)1 << shift) * max_slices; }
+};
+
+static SortGeom sort_geom(uint32_t n, int c, uint32_t batch, bool merged) {
+  SortGeom g;
+  const int W = (RECODE_BITS + c - 1) / c;
+  const uint32_t nb = 1u << (c - 1);
+  g.rows_per_set = merged ? (uint32_t)W : 1u;
+  g.rows = batch * (uint32_t)W;
+  g.sets = g.rows / g.rows_per_set;
+  const size_t per_set = (size_t)n * g.rows_per_set;
+  uint32_t lo = nb >> MAX_FINE_SHIFT;
+  if (lo < 1) lo = 1;
+  uint32_t cb = 1;
+  while ((size_t)cb * 2 * GROUP_TARGET <= per_set && cb * 2 <= nb && cb * 2 <= MAX_CBINS) cb *= 2;
+  if (cb < lo) cb = lo;
+  const size_t cap = (size_t)1024 * SCAN_BLOCK - 1;
+  uint32_t cs = SORT_CHUNK;
+  auto chunks_of = [&](uint32_t x) { return n ? (uint32_t)(((size_t)n + x - 1) / x) : 1u; };
+  while ((size_t)g.rows * cb * chunks_of(cs) > cap && cs < (1u << 30)) cs *= 2;
+  while ((size_t)g.rows * cb * chunks_of(cs) > cap && cb > lo) cb /= 2;
+  if ((size_t)g.rows * cb * chunks_of(cs) > cap) throw ArgError{"MSM too large for the sort's scan"};
+  g.cbins = cb;
+  g.shift = 0;
+  while ((cb << g.shift) < nb) ++g.shift;
+  g.cs = cs;
+  g.chunks = chunks_of(cs);
+  g.q = (size_t)g.rows * cb * g.chunks;
+  g.groups = g.sets * cb;
+  const size_t slices = g.groups + ((size_t)batch * n * W + FINE_TILE - 1) / FINE_TILE;
+  if (slices >= (1u << 31)) throw ArgError{"MSM too large for the sort's slices"};
+  g.max_slices = (uint32_t)slices;
+  return g;
+}
 
 size_t msm_sort_scratch_bytes(uint32_t n, int c, uint32_t batch) {
-  int W = (RECODE_BITS + c - 1) / c;
-  size_t nb = (size_t)1 << (c - 1);
-  size_t K = (size_t)batch * W * nb;
-  size_t E = (size_t)batch * n * W;
-  // count, offset, block sums, entries, digits (32-bit: the widest), cursors, btotal, total
-  return align_up(4 * (K + 1)) * 2 + align_up(4 * 1025) + align_up(8 * E) + align_up(4 * E) +
-         align_up(4 * K) + align_up(4 * (batch * nb + 1)) + align_up(16);
+  const int W = (RECODE_BITS + c - 1) / c;
+  const size_t nb = (size_t)1 << (c - 1);
+  const size_t E = (size_t)batch * n * W;
+  // Sized for per-window sets (the larger key count) and the geometry of either layout.
+  const size_t K = (size_t)batch * W * nb;
+  const SortGeom a = sort_geom(n, c, batch, false), b = sort_geom(n, c, batch, true);
+  const size_t q = std::max(a.q, b.q);
+  const size_t groups = std::max(a.groups, b.groups);
+  const size_t fine = std::max(a.fine_words(), b.fine_words());
+  // offset, block sums, counts, positions, entries, staging, slice prefix, fine counts,
+  // digits (32-bit: the widest), total
+  return align_up(4 * (K + 1)) + align_up(4 * 1025) + 2 * align_up(4 * (q + 1)) +
+         2 * align_up(8 * E) + align_up(4 * (groups + 1)) + align_up(4 * fine) +
+         align_up(4 * E) + align_up(16);
 }
 
 void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c, bool merged, bool constant,
@@ -337,88 +554,96 @@ void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c, bool merged, bo
   p.nb = 1u << (c - 1);
   p.batch = batch;
   p.max_entries = (size_t)batch * n * p.windows;
-  p.chunks = sort_chunks(n);
-  size_t K = (size_t)batch * p.windows * p.nb;
-  const size_t KS = (size_t)msm_bucket_sets(p) * p.nb;  // keys the offsets scan covers
-  if ((KS + 1023) / 1024 > 1024) throw ArgError{"MSM window too wide for the scan"};
   if ((size_t)n * batch >= (1u << 31)) throw ArgError{"MSM larger than 2^31 points"};
   if (p.max_entries >= (1ull << 31)) throw ArgError{"MSM entry count exceeds 2^31"};
   if (constant && (merged ? (size_t)n * p.windows : (size_t)n) >= (1ull << 30))
     throw ArgError{"constant-work MSM: base indices must stay below 2^30"};
+  const SortGeom g = sort_geom(n, c, batch, merged);
+  p.shift = g.shift;
+  p.cbins = g.cbins;
+  p.rows_per_set = g.rows_per_set;
+  p.chunk = g.cs;
+  p.chunks = g.chunks;
+  p.q = g.q;
+  p.groups = g.groups;
+  p.max_slices = g.max_slices;
+  const size_t K = (size_t)msm_bucket_sets(p) * p.nb;
   char* q = static_cast<char*>(scratch);
   auto take = [&](size_t bytes) {
     char* r = q;
     q += align_up(bytes);
     return r;
   };
-  p.count = reinterpret_cast<uint32_t*>(take(4 * (K + 1)));
   p.offset = reinterpret_cast<uint32_t*>(take(4 * (K + 1)));
   p.block_sums = reinterpret_cast<uint32_t*>(take(4 * 1025));
+  p.ccount = reinterpret_cast<uint32_t*>(take(4 * (p.q + 1)));
+  p.cpos = reinterpret_cast<uint32_t*>(take(4 * (p.q + 1)));
   p.ent = reinterpret_cast<uint2*>(take(8 * p.max_entries));
+  p.tmp = reinterpret_cast<uint2*>(take(8 * p.max_entries));
+  p.sprefix = reinterpret_cast<uint32_t*>(take(4 * ((size_t)p.groups + 1)));
+  p.fcnt = reinterpret_cast<uint32_t*>(take(4 * g.fine_words()));
   p.digits = reinterpret_cast<uint32_t*>(take(4 * p.max_entries));
-  p.cursor = reinterpret_cast<uint32_t*>(take(4 * K));
-  p.btotal = reinterpret_cast<uint32_t*>(take(4 * ((size_t)batch * p.nb + 1)));
   p.total = reinterpret_cast<uint32_t*>(take(16));
 }
 
 template <class D>
-static void sort_count(MsmPlan& p, const void* scalars, hipStream_t s) {
+static void sort_coarse_count(MsmPlan& p, const void* scalars, hipStream_t s) {
   D* digits = reinterpret_cast<D*>(p.digits);
   const uint32_t total = p.batch * p.n;
   hipLaunchKernelGGL(k_digits<D>, dim3((total + 255) / 256), dim3(256), 0, s,
                      static_cast<const uint4*>(scalars), p.n, total, p.c, p.windows, digits,
                      (int)p.constant);
   G16_LAUNCH_CHECK("msm digits");
-  hipLaunchKernelGGL(k_chunk_count<D>, dim3(p.chunks, p.batch * p.windows), dim3(SORT_THREADS),
-                     4 * (p.nb / 2), s, digits, p.n, p.nb, p.count);
-  G16_LAUNCH_CHECK("msm chunk count");
+  hipLaunchKernelGGL(k_coarse_count<D>, dim3(p.chunks, p.batch * p.windows), dim3(SORT_THREADS),
+                     4 * p.cbins, s, digits, p.n, p.chunk, p.shift, p.cbins, p.rows_per_set,
+                     p.chunks, p.ccount);
+  G16_LAUNCH_CHECK("msm coarse count");
 }
 
 template <class D>
-static void sort_scatter(MsmPlan& p, hipStream_t s) {
-  const uint32_t span = p.nb < SCATTER_SPAN ? p.nb : SCATTER_SPAN;
-  hipLaunchKernelGGL(k_chunk_scatter<D>, dim3(p.chunks, p.batch * p.windows, p.nb / span),
-                     dim3(SORT_THREADS), 4 * span, s, reinterpret_cast<const D*>(p.digits), p.n,
-                     p.nb, p.windows, span, p.cursor, (int)p.merged, p.ent, (int)p.constant);
-  G16_LAUNCH_CHECK("msm chunk scatter");
+static void sort_coarse_scatter(MsmPlan& p, hipStream_t s) {
+  hipLaunchKernelGGL(k_coarse_scatter<D>, dim3(p.chunks, p.batch * p.windows), dim3(SORT_THREADS),
+                     4 * p.cbins, s, reinterpret_cast<const D*>(p.digits), p.n, p.chunk, p.shift,
+                     p.cbins, p.rows_per_set, p.chunks, p.windows, p.cpos, p.tmp, (int)p.merged,
+                     (int)p.constant);
+  G16_LAUNCH_CHECK("msm coarse scatter");
 }
 
 void msm_sort(MsmPlan& p, const void* scalars, hipStream_t s) {
-  const uint32_t K = p.batch * (uint32_t)p.windows * p.nb;
-  hipLaunchKernelGGL(k_fill_u32<0>, dim3((K + 255) / 256), dim3(256), 0, s, p.count, (size_t)K,
-                     0u);
-  G16_LAUNCH_CHECK("msm clear counts");
+  const uint32_t Q = (uint32_t)p.q;
   if (p.n) {
-    if (short_digits(p)) sort_count<uint16_t>(p, scalars, s);
-    else sort_count<uint32_t>(p, scalars, s);
+    if (short_digits(p)) sort_coarse_count<uint16_t>(p, scalars, s);
+    else sort_coarse_count<uint32_t>(p, scalars, s);
+  } else {
+    hipLaunchKernelGGL(k_fill_u32<0>, dim3((Q + 255) / 256), dim3(256), 0, s, p.ccount,
+                       (size_t)Q, 0u);
+    G16_LAUNCH_CHECK("msm clear counts");
   }
-  // The bucket totals to scan: per (window, bucket), or per bucket over all windows.
-  const uint32_t* totals = p.count;
-  uint32_t KS = K;
-  if (p.merged) {
-    const uint32_t KB = p.batch * p.nb;
-    hipLaunchKernelGGL(k_window_scan, dim3((KB + 255) / 256), dim3(256), 0, s, p.count, p.nb,
-                       p.windows, p.batch, p.btotal);
-    G16_LAUNCH_CHECK("msm window scan");
-    totals = p.btotal;
-    KS = KB;
-  }
-  uint32_t nblk = (KS + 1023) / 1024;
-  hipLaunchKernelGGL(k_scan_blocks, dim3(nblk), dim3(256), 0, s, totals, KS, p.offset,
+  const uint32_t nblk = (Q + SCAN_BLOCK - 1) / SCAN_BLOCK;
+  hipLaunchKernelGGL(k_scan_blocks, dim3(nblk), dim3(256), 0, s, p.ccount, Q, p.cpos,
                      p.block_sums);
   G16_LAUNCH_CHECK("msm scan");
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, p.block_sums, nblk);
   G16_LAUNCH_CHECK("msm scan top");
-  hipLaunchKernelGGL(k_scan_fix, dim3((KS + 255) / 256), dim3(256), 0, s, p.offset,
-                     p.block_sums, KS, p.total);
+  hipLaunchKernelGGL(k_scan_fix, dim3((Q + 255) / 256), dim3(256), 0, s, p.cpos, p.block_sums, Q);
   G16_LAUNCH_CHECK("msm scan fix");
   if (p.n) {
-    hipLaunchKernelGGL(k_cursor_init, dim3((K + 255) / 256), dim3(256), 0, s, p.cursor,
-                       p.offset, p.merged ? p.count : nullptr, p.nb, p.windows, K);
-    G16_LAUNCH_CHECK("msm cursors");
-    if (short_digits(p)) sort_scatter<uint16_t>(p, s);
-    else sort_scatter<uint32_t>(p, s);
+    if (short_digits(p)) sort_coarse_scatter<uint16_t>(p, s);
+    else sort_coarse_scatter<uint32_t>(p, s);
   }
+  const uint32_t per_group = p.rows_per_set * p.chunks;
+  hipLaunchKernelGGL(k_slice_scan, dim3(1), dim3(1024), 0, s, p.cpos, per_group, p.groups,
+                     p.sprefix);
+  G16_LAUNCH_CHECK("msm slice scan");
+  hipLaunchKernelGGL(k_fine_count, dim3(p.max_slices), dim3(FINE_THREADS), 0, s, p.tmp, p.cpos,
+                     per_group, p.groups, p.sprefix, p.shift, p.fcnt);
+  G16_LAUNCH_CHECK("msm fine count");
+  hipLaunchKernelGGL(k_fine_prefix, dim3(p.groups), dim3(FINE_THREADS), 0, s, p.cpos, per_group,
+                     p.groups, p.sprefix, p.cbins, p.shift, p.nb, p.fcnt, p.offset, p.total);
+  G16_LAUNCH_CHECK("msm fine prefix");
+  hipLaunchKernelGGL(k_fine_scatter, dim3(p.max_slices), dim3(FINE_THREADS), 0, s, p.tmp, p.cpos,
+                     per_group, p.groups, p.sprefix, p.cbins, p.shift, p.nb, p.fcnt, p.ent);
+  G16_LAUNCH_CHECK("msm fine scatter");
 }
 
 void msm_reduce_shape(const MsmPlan& p, int* rows, int* lg) {
