@@ -311,6 +311,62 @@ HD void mont29_limbs(const uint32_t A[9], const uint32_t B[9], uint32_t R[9]) {
   }
 }
 
+// Lazily reduced sums of products. A wide value is 17 columns t[k] of weight 2^(29 k), kept
+// as 64-bit two's complement words (uint64_t arithmetic wraps, so a column may pass through
+// values outside int64 as long as its final value fits); wide_mac adds A B, wide_msub
+// subtracts it (one signed 32 x 32 multiply-add per partial product: limbs below 2^30). redc29 then divides the sum by 2^261 modulo the modulus once for all its
+// products.
+HD void wide_mac(uint64_t t[17], const uint32_t A[9], const uint32_t B[9]) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+#pragma unroll
+    for (int j = 0; j < 9; ++j) t[i + j] += (uint64_t)A[i] * B[j];
+}
+HD void wide_msub(uint64_t t[17], const uint32_t A[9], const uint32_t B[9]) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const int32_t na = -(int32_t)A[i];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) t[i + j] += (uint64_t)((int64_t)na * (int64_t)(int32_t)B[j]);
+  }
+}
+
+// t 2^-261 mod the modulus as nine 29-bit limbs in [0, 2 modulus), for a wide value
+// V = sum_k t[k] 2^(29 k) with -2^261 modulus < V < 2^261 modulus whose columns each hold at
+// most 18 partial products of limbs below 2^29 (|t[k]| <= 18 2^58; the reduction's nine
+// products and the carries keep every column below 27 2^58 + 2^34 < 2^63). The quotient
+// (V + M modulus) / 2^261, M < 2^261, lies in (-modulus, 2 modulus); a negative one gets the
+// modulus added (NEG = false: V >= 0, nothing to add).
+template <class M29, bool NEG = true>
+HD void redc29(uint64_t t[17], uint32_t R[9]) {
+  constexpr uint32_t MASK = (1u << 29) - 1u;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const uint32_t m = ((uint32_t)t[i] * M29::N29) & MASK;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) t[i + j] += (uint64_t)m * M29::p29(j);
+    if (i < 8) t[i + 1] += (uint64_t)((int64_t)t[i] >> 29);  // t[i] is now 0 mod 2^29
+  }
+  // The quotient's columns are t[9..16] plus the carry out of t[8].
+  int64_t c = (int64_t)t[8] >> 29;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    const int64_t x = (j < 8 ? (int64_t)t[9 + j] : 0) + c;
+    R[j] = (uint32_t)x & MASK;
+    c = x >> 29;
+  }
+  // c is 0, or -1 for a negative quotient: then add the modulus (the carry out cancels c).
+  if (!NEG) return;
+  const uint32_t neg = (uint32_t)c;
+  uint32_t k = 0;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    const uint32_t x = R[j] + (M29::p29(j) & neg) + k;
+    R[j] = x & MASK;
+    k = x >> 29;
+  }
+}
+
 // With FULL = false the result is left in [0, 2 modulus) (any 256-bit inputs): the MSM form
 // keeps its values there (see the FqmCfg operations below).
 template <class M29, bool FULL = true>
@@ -491,6 +547,106 @@ template <bool INL, class B> HD Fq2T<INL, B> mul(const Fq2T<INL, B>& a, const Fq
   return {sub(v0, v1), sub(sub(s, v0), v1)};
 }
 
+// In the MSM form (the G2 bucket accumulation) the Karatsuba products stay wide and each
+// coordinate is reduced once: c0 = a0 b0 - a1 b1 (|c0| < 4 q^2), c1 = (a0 + a1)(b0 + b1) -
+// a0 b0 - a1 b1 = a0 b1 + a1 b0 (each column a sum of at most 18 limb products): two
+// reductions instead of three, and each operand goes to limbs once.
+HD Fq2T<true, Fm> mul(const Fq2T<true, Fm>& a, const Fq2T<true, Fm>& b) {
+  uint32_t A0[9], A1[9], B0[9], B1[9], R[9];
+  limbs29_of(a.c0.v, A0);
+  limbs29_of(a.c1.v, A1);
+  limbs29_of(b.c0.v, B0);
+  limbs29_of(b.c1.v, B1);
+  uint64_t t0[17], t1[17];
+#pragma unroll
+  for (int k = 0; k < 17; ++k) t0[k] = t1[k] = 0;
+  wide_mac(t0, A0, B0);
+  wide_mac(t1, A1, B1);
+#pragma unroll
+  for (int k = 0; k < 17; ++k) {
+    const uint64_t x = t0[k], y = t1[k];
+    t0[k] = x - y;
+    t1[k] = 0 - x - y;
+  }
+  Fq2T<true, Fm> r;
+  redc29<Fq29>(t0, R);
+  pack29(R, r.c0.v);
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {  // limbs below 2^30: the columns' final values stay in range
+    A0[j] += A1[j];
+    B0[j] += B1[j];
+  }
+  wide_mac(t1, A0, B0);
+  redc29<Fq29, false>(t1, R);
+  pack29(R, r.c1.v);
+  return r;
+}
+
+// a b - c d, by default two products and a subtraction.
+template <class F> HD F mul_sub(const F& a, const F& b, const F& c, const F& d) {
+  return sub(mul(a, b), mul(c, d));
+}
+// In the MSM form over Fq2: both Karatsuba products wide in the same two accumulators, one
+// reduction per coordinate (two instead of four). c0 = a0 b0 - a1 b1 - c0 d0 + c1 d1 and
+// c1 = a0 b1 + a1 b0 - c0 d1 - c1 d0: each column at most 18 limb products in magnitude, and
+// |c0|, |c1| < 8 q^2.
+HD Fq2T<true, Fm> mul_sub(const Fq2T<true, Fm>& a, const Fq2T<true, Fm>& b,
+                          const Fq2T<true, Fm>& c, const Fq2T<true, Fm>& d) {
+  uint32_t X0[9], X1[9], Y0[9], Y1[9], R[9];
+  uint64_t t0[17], t1[17], w[17];
+  limbs29_of(a.c0.v, X0);
+  limbs29_of(a.c1.v, X1);
+  limbs29_of(b.c0.v, Y0);
+  limbs29_of(b.c1.v, Y1);
+#pragma unroll
+  for (int k = 0; k < 17; ++k) t0[k] = w[k] = 0;
+  wide_mac(t0, X0, Y0);
+  wide_mac(w, X1, Y1);
+#pragma unroll
+  for (int k = 0; k < 17; ++k) {
+    const uint64_t x = t0[k];
+    t0[k] = x - w[k];
+    t1[k] = 0 - x - w[k];
+  }
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    X0[j] += X1[j];
+    Y0[j] += Y1[j];
+  }
+  wide_mac(t1, X0, Y0);  // t1 = a0 b1 + a1 b0
+  limbs29_of(c.c0.v, X0);
+  limbs29_of(c.c1.v, X1);
+  limbs29_of(d.c0.v, Y0);
+  limbs29_of(d.c1.v, Y1);
+#pragma unroll
+  for (int k = 0; k < 17; ++k) w[k] = 0;
+  wide_mac(w, X0, Y0);
+#pragma unroll
+  for (int k = 0; k < 17; ++k) {
+    t0[k] -= w[k];
+    t1[k] += w[k];
+    w[k] = 0;
+  }
+  wide_mac(w, X1, Y1);
+#pragma unroll
+  for (int k = 0; k < 17; ++k) {
+    t0[k] += w[k];
+    t1[k] += w[k];
+  }
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {  // below 2^30: the signed products stay inside int64
+    X0[j] += X1[j];
+    Y0[j] += Y1[j];
+  }
+  wide_msub(t1, X0, Y0);  // t1 -= (c0 + c1)(d0 + d1): now a0 b1 + a1 b0 - c0 d1 - c1 d0
+  Fq2T<true, Fm> r;
+  redc29<Fq29>(t0, R);
+  pack29(R, r.c0.v);
+  redc29<Fq29>(t1, R);
+  pack29(R, r.c1.v);
+  return r;
+}
+
 // (a0 + a1 u)^2 = (a0 + a1)(a0 - a1) + 2 a0 a1 u: 2 base-field products.
 template <bool INL, class B> HD Fq2T<INL, B> sqr(const Fq2T<INL, B>& a) {
   B m = fq2_part_mul<INL>(add_into_mul(a.c0, a.c1), sub(a.c0, a.c1));
@@ -639,11 +795,9 @@ HD Xyzz<F> xyzz_add_aff(const Xyzz<F>& p, const Aff<F>& a) {
   SB();
   r.ZZZ = mul(p.ZZZ, PPP);
   SB();
-  F YP = mul(p.Y, PPP);
-  SB();
   r.X = sub(sub(sqr(R), PPP), dbl(Q));
   SB();
-  r.Y = sub(mul(R, sub(Q, r.X)), YP);
+  r.Y = mul_sub(R, sub(Q, r.X), p.Y, PPP);
   return r;
 }
 

@@ -206,16 +206,22 @@ struct Acc<Fm> {
     SB();
     mont29_limbs<Fq29>(p.ZZZ, PPP, r.ZZZ);
     SB();
-    limbs29_of(p.Y.v, L1);
-    const Fm YP = prod(L1, PPP);
-    SB();
     Fm PPPw;
     pack29(PPP, PPPw.v);
     limbs29_of(R.v, L0);
     r.X = sub(sub(prod(L0, L0), PPPw), dbl(Q));
     SB();
+    // Y3 = R (Q - X3) - Y1 PPP as one wide value and one reduction (|Y3 wide| < 4 q^2).
+    uint64_t t[17];
+#pragma unroll
+    for (int k = 0; k < 17; ++k) t[k] = 0;
     limbs29_of(sub(Q, r.X).v, L1);
-    r.Y = sub(prod(L0, L1), YP);
+    wide_mac(t, L0, L1);
+    limbs29_of(p.Y.v, L1);
+    wide_msub(t, L1, PPP);
+    uint32_t Y[9];
+    redc29<Fq29>(t, Y);
+    pack29(Y, r.Y.v);
     return r;
   }
 };

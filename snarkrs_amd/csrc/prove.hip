@@ -451,8 +451,9 @@ static void check_witness_len(const g16hip_circuit* C, size_t n) {
 // in (copying chunks itself while it waits), so the host copy overlaps the transfer. The
 // staging is free at the start of a proof: a slot is leased by one proof at a time, and a
 // proof returns only after its streams have finished.
+static constexpr size_t kStagingHelpers = 7;
 static void stage_witness(Slot* S, const uint8_t* w, size_t bytes) {
-  constexpr size_t CHUNK = 2u << 20;
+  constexpr size_t CHUNK = 1u << 20;
   uint8_t* dst = S->host_witness;
   char* dev = S->witness.as<char>();
   const size_t nch = (bytes + CHUNK - 1) / CHUNK;
@@ -472,13 +473,14 @@ static void stage_witness(Slot* S, const uint8_t* w, size_t bytes) {
     ready[c].store(1, std::memory_order_release);
     return true;
   };
-  std::vector<std::thread> helpers;
-  const size_t nh = std::min<size_t>(3, nch - 1);
+  // Pooled helpers (no thread is started per proof once the pool has them).
+  std::vector<std::unique_ptr<HostTask>> helpers;
+  const size_t nh = std::min<size_t>(kStagingHelpers, nch - 1);
   for (size_t h = 0; h < nh; ++h)
-    helpers.emplace_back([&] {
+    helpers.push_back(std::make_unique<HostTask>([&] {
       while (copy_one()) {
       }
-    });
+    }));
   std::exception_ptr err;
   try {
     for (size_t c = 0; c < nch; ++c) {
@@ -490,7 +492,7 @@ static void stage_witness(Slot* S, const uint8_t* w, size_t bytes) {
   } catch (...) {
     err = std::current_exception();
   }
-  for (auto& t : helpers) t.join();
+  for (auto& t : helpers) t->join();
   if (err) std::rethrow_exception(err);
 }
 
@@ -1012,10 +1014,11 @@ static void enqueue_wave(g16hip_circuit* C, Wave* W, const std::vector<WaveItem>
     auto copy = [&] {
       for (uint32_t k; (k = next.fetch_add(1)) < m;) std::memcpy(stage + wb * k, it[k].witness, wb);
     };
-    std::vector<std::thread> th;
-    for (uint32_t h = 1; h < std::min<uint32_t>(m, 4); ++h) th.emplace_back(copy);
+    std::vector<std::unique_ptr<HostTask>> th;
+    for (uint32_t h = 1; h < std::min<uint32_t>(m, 4); ++h)
+      th.push_back(std::make_unique<HostTask>(copy));
     copy();
-    for (auto& t : th) t.join();
+    for (auto& t : th) t->join();
   }
   if (replay_graphs()) {
     auto gi = W->graphs.find(2 * m + (uint32_t)b);
@@ -1083,10 +1086,11 @@ static void finish_wave(g16hip_circuit* C, Wave* W, const std::vector<WaveItem>&
   auto work = [&] {
     for (uint32_t k; (k = next.fetch_add(1)) < m;) one(k);
   };
-  std::vector<std::thread> th;
-  for (uint32_t h = 1; h < std::min<uint32_t>(m, 8); ++h) th.emplace_back(work);
+  std::vector<std::unique_ptr<HostTask>> th;
+  for (uint32_t h = 1; h < std::min<uint32_t>(m, 8); ++h)
+    th.push_back(std::make_unique<HostTask>(work));
   work();
-  for (auto& t : th) t.join();
+  for (auto& t : th) t->join();
 }
 
 // Proves the items of one circuit in waves of the circuit's cap, the host finishing wave i while

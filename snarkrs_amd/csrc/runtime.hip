@@ -3,6 +3,7 @@
 // prove.hip.
 #include <cstdio>
 #include <cstdlib>
+#include <deque>
 
 #include "runtime_impl.h"
 
@@ -21,6 +22,51 @@ void hip_check(hipError_t e, const char* what) {
 static thread_local std::string t_last_error;
 
 void set_last_error(const std::string& msg) { t_last_error = msg; }
+
+// The host thread pool. It lives as long as the process (never destroyed, so no thread waits
+// on a destroyed mutex at exit); each thread takes queued tasks until it has none, then waits.
+namespace {
+struct HostPool {
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<std::function<void()>> tasks;
+  size_t idle = 0;
+
+  void loop() {
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+      while (tasks.empty()) {
+        ++idle;
+        cv.wait(lk);
+        --idle;
+      }
+      std::function<void()> fn = std::move(tasks.front());
+      tasks.pop_front();
+      lk.unlock();
+      fn();
+      lk.lock();
+    }
+  }
+};
+HostPool& host_pool() {
+  static HostPool* pool = new HostPool;
+  return *pool;
+}
+}  // namespace
+
+void host_pool_run(std::function<void()> fn) {
+  HostPool& p = host_pool();
+  std::unique_lock<std::mutex> lk(p.mu);
+  p.tasks.push_back(std::move(fn));
+  // Idle threads take queued tasks in turn; when there are fewer of them than tasks queued,
+  // one more thread starts, so a task never waits behind a running one.
+  if (p.idle >= p.tasks.size()) {
+    p.cv.notify_one();
+    return;
+  }
+  lk.unlock();
+  std::thread([&p] { p.loop(); }).detach();
+}
 
 void build_ntt_tables(NttTables& t, int L, hipStream_t s) {
   size_t n = (size_t)1 << L;

@@ -3,8 +3,10 @@
 #pragma once
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
 #include <exception>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -337,24 +339,44 @@ struct Timer {
   double ms() const { return us() / 1000.0; }
 };
 
-// Runs fn() on a thread and keeps the first exception for the joiner.
+// Runs fn on a pooled host thread. Threads are kept and reused; a task never waits for a busy
+// one (the pool starts another thread when none is idle), so tasks may wait for each other.
+void host_pool_run(std::function<void()> fn);
+
+// Runs fn() on a pooled thread and keeps its exception for the joiner.
 struct HostTask {
-  std::exception_ptr err;  // declared first: the thread may set it as soon as it starts
-  std::thread th;
+  std::exception_ptr err;
   template <class Fn>
-  explicit HostTask(Fn fn) : th([this, fn] {
-    try {
-      fn();
-    } catch (...) {
-      err = std::current_exception();
-    }
-  }) {}
+  explicit HostTask(Fn fn) : st_(std::make_shared<State>()) {
+    std::shared_ptr<State> st = st_;
+    host_pool_run([st, fn]() mutable {
+      try {
+        fn();
+      } catch (...) {
+        st->err = std::current_exception();
+      }
+      std::lock_guard<std::mutex> lk(st->m);
+      st->done = true;
+      st->cv.notify_all();
+    });
+  }
   HostTask(const HostTask&) = delete;
   HostTask& operator=(const HostTask&) = delete;
   ~HostTask() { join(); }
   void join() {
-    if (th.joinable()) th.join();
+    std::unique_lock<std::mutex> lk(st_->m);
+    st_->cv.wait(lk, [&] { return st_->done; });
+    err = st_->err;
   }
+
+ private:
+  struct State {
+    std::mutex m;
+    std::condition_variable cv;
+    bool done = false;
+    std::exception_ptr err;
+  };
+  std::shared_ptr<State> st_;
 };
 
 // ---- devices ---------------------------------------------------------------------------
