@@ -44,10 +44,13 @@ void launch_fr_to_rp(void* data, size_t n, hipStream_t s);  // R form -> R' form
 // per-pass twiddle tables (launch_stage_tables).
 // dif_inverse: natural order in -> bit-reversed out, unscaled (w^-1 twiddles).
 // dit_forward: bit-reversed in -> natural out; if `prescale` is given, element p is first
-// multiplied by prescale[p] (the coset table in bit-reversed order, R' form).
+// multiplied by prescale[p] (the coset table in bit-reversed order, R' form). With
+// `natural_src` the input is read from there in natural order (the bit reversal folded into
+// the first pass's loads; natural_src may be data) and `tmp` (n elements per vector, distinct
+// from data and natural_src) holds the first pass's output.
 void ntt_dif_inverse(void* data, int nvec, int log_n, const void* tw_inv, hipStream_t s);
 void ntt_dit_forward(void* data, int nvec, int log_n, const void* tw_fwd, const void* prescale,
-                     hipStream_t s);
+                     hipStream_t s, const void* natural_src = nullptr, void* tmp = nullptr);
 // ntt_dif_inverse then ntt_dit_forward with `coset` as the prescale, the inverse's lowest pass
 // and the forward's first pass fused into one (the same tiles, in LDS). With h_out (nvec = 3:
 // A, B, C back to back), H = A B - C goes to h_out from the last pass and C is not stored.

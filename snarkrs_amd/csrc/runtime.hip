@@ -621,11 +621,10 @@ static void ntt_on_device(g16hip_device* d, void* data, int L, int kind) {
   hipStream_t s = d->stream;
   Fr* sc = t.scalars.as<Fr>();
   if (kind == 0) {
-    // natural -> natural: permute, then the bit-reversed-input DIT.
+    // natural -> natural: the bit-reversed-input DIT reading `data` at bit-reversed indices
+    // (its first pass out of place into scratch, the next back into `data`).
     d->scratch_c.alloc(32 * n);
-    launch_bitrev_permute(d->scratch_c.p, data, L, s);
-    G16_CHECK(hipMemcpyAsync(data, d->scratch_c.p, 32 * n, hipMemcpyDeviceToDevice, s));
-    ntt_dit_forward(data, 1, L, t.fwd.p, nullptr, s);
+    ntt_dit_forward(data, 1, L, t.fwd.p, nullptr, s, data, d->scratch_c.p);
   } else if (kind == 1) {
     ntt_dif_inverse(data, 1, L, t.inv.p, s);
     d->scratch_c.alloc(32 * n);
