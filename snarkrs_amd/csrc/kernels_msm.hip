@@ -107,7 +107,7 @@ static constexpr int FINE_THREADS = 256;
 static constexpr int FINE_ITEMS = 16;
 static constexpr uint32_t FINE_TILE = FINE_THREADS * FINE_ITEMS;  // entries per fine slice
 static constexpr uint32_t SCAN_BLOCK = 4096;   // elements per scan workgroup (256 x 16)
-static constexpr uint32_t MAX_CBINS = 4096;    // coarse bins per set (LDS counters)
+static constexpr uint32_t MAX_CBINS = 512;     // coarse bins per set (LDS counters)
 static constexpr int MAX_FINE_SHIFT = 12;      // fine bins per coarse group <= 4096
 
 // ctr[key] += 1 from every active lane, returning each lane's old value plus its rank among
@@ -129,6 +129,32 @@ __device__ __forceinline__ uint32_t lds_take(uint32_t* ctr, uint32_t key) {
   return atomicAdd(&ctr[key], 1u);
 }
 
+// Block-wide exclusive prefix of one value per thread (blockDim.x = T, a multiple of 64);
+// returns the thread's prefix and writes the block total to *sum.
+template <int T>
+__device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t* sh, uint32_t* sum) {
+  const uint32_t lane = __lane_id(), wave = threadIdx.x / 64;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off);
+    if (lane >= (uint32_t)off) x += y;
+  }
+  if (lane == 63) sh[wave] = x;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < T / 64; ++w) {
+    const uint32_t t = sh[w];
+    before += (uint32_t)w < wave ? t : 0u;
+    all += t;
+  }
+  __syncthreads();  // sh may be reused by the caller
+  *sum = all;
+  return before + x - v;
+}
+
+// sprefix[g] = slices before group g, sprefix[groups] = all slices (one workgroup).
 template <class D>
 __global__ void __launch_bounds__(256) k_digits(const uint4* scalars, uint32_t n, uint32_t total,
                                                 int c, int W, D* digits, int constant) {
@@ -174,40 +200,80 @@ __global__ void __launch_bounds__(SORT_THREADS) k_coarse_count(const D* digits, 
 }
 
 // Each entry of (chunk p, row) into its coarse bin's run of the staging array:
-// tmp[pos] = {fine bits of the bucket, value}.
+// tmp[pos] = {fine bits of the bucket, value}. The chunk goes through LDS in tiles of
+// SORT_THREADS x SORT_ITEMS digits: a tile's entries are counted and ordered by coarse bin in
+// LDS first, then written out with consecutive lanes on consecutive positions of each bin's
+// run (a wave's stores cover a few runs instead of up to 64 scattered words).
+// LDS: cur, lcnt, lstart (MAX_CBINS words each) and the tile's staged keys and values: 70 KB,
+// two workgroups per CU.
+static constexpr uint32_t SCATTER_TILE = SORT_THREADS * SORT_ITEMS;
+
 template <class D>
 __global__ void __launch_bounds__(SORT_THREADS) k_coarse_scatter(
     const D* digits, uint32_t n, uint32_t cs, int shift, uint32_t cbins, uint32_t rows_per_set,
     uint32_t chunks, int W, const uint32_t* cpos, uint2* tmp, int merged, int constant) {
-  extern __shared__ uint32_t cur[];  // cbins words
+  __shared__ uint32_t cur[MAX_CBINS];     // next global position of each bin's run
+  __shared__ uint32_t lcnt[MAX_CBINS];    // the tile's entries per bin
+  __shared__ uint32_t lstart[MAX_CBINS];  // their exclusive prefix
+  __shared__ uint32_t skey[SCATTER_TILE]; // staged (bin << 12 | fine bits), in bin order
+  __shared__ uint32_t sval[SCATTER_TILE];
+  __shared__ uint32_t scan_sh[SORT_THREADS / 64];
   const uint32_t p = blockIdx.x, row = blockIdx.y;
   const uint32_t set = row / rows_per_set, r = row - set * rows_per_set;
   for (uint32_t j = threadIdx.x; j < cbins; j += SORT_THREADS)
     cur[j] = cpos[(((size_t)set * cbins + j) * rows_per_set + r) * chunks + p];
-  __syncthreads();
   const uint32_t win = row % (uint32_t)W;  // window of this row (rows are vector-major)
   const uint32_t mask = (1u << shift) - 1u;
   const size_t lo = (size_t)p * cs;
   const size_t hi = lo + cs < n ? lo + cs : n;
   const D* d = digits + (size_t)row * n;
-  for (size_t base = lo; base < hi; base += SORT_THREADS * SORT_ITEMS) {
+  const uint32_t per = (cbins + SORT_THREADS - 1) / SORT_THREADS;  // bins per thread (0 or 1)
+  for (size_t base = lo; base < hi; base += SCATTER_TILE) {
+    for (uint32_t j = threadIdx.x; j < cbins; j += SORT_THREADS) lcnt[j] = 0;
+    __syncthreads();
     D e[SORT_ITEMS];
+    uint32_t rank[SORT_ITEMS];
 #pragma unroll
     for (int k = 0; k < SORT_ITEMS; ++k) {
       const size_t i = base + k * SORT_THREADS + threadIdx.x;
       e[k] = i < hi ? d[i] : Digit<D>::none();
     }
 #pragma unroll
+    for (int k = 0; k < SORT_ITEMS; ++k)
+      rank[k] = Digit<D>::zero(e[k]) ? 0u : lds_take(lcnt, Digit<D>::bucket(e[k]) >> shift);
+    __syncthreads();
+    // lstart = exclusive prefix of lcnt over the bins, `per` consecutive bins per thread.
+    const uint32_t j0 = threadIdx.x * per;
+    uint32_t loc = 0;
+    for (uint32_t j = j0; j < j0 + per && j < cbins; ++j) loc += lcnt[j];
+    uint32_t all;
+    uint32_t run = block_exclusive<SORT_THREADS>(loc, scan_sh, &all);
+    for (uint32_t j = j0; j < j0 + per && j < cbins; ++j) {
+      lstart[j] = run;
+      run += lcnt[j];
+    }
+    __syncthreads();
+#pragma unroll
     for (int k = 0; k < SORT_ITEMS; ++k) {
       if (Digit<D>::zero(e[k])) continue;
       const uint32_t i = (uint32_t)(base + k * SORT_THREADS + threadIdx.x);
-      const uint32_t b = Digit<D>::bucket(e[k]);
-      const uint32_t pos = lds_take(cur, b >> shift);
+      const uint32_t b = Digit<D>::bucket(e[k]), bin = b >> shift;
       const uint32_t idx = merged ? win * n + i : i;
       const uint32_t sign = Digit<D>::sign(e[k]);
-      tmp[pos] = make_uint2(b & mask, constant ? (idx << 2) | Digit<D>::discard(e[k]) | sign
-                                               : (idx << 1) | sign);
+      const uint32_t pos = lstart[bin] + rank[k];
+      skey[pos] = (bin << 12) | (b & mask);
+      sval[pos] = constant ? (idx << 2) | Digit<D>::discard(e[k]) | sign : (idx << 1) | sign;
     }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < all; i += SORT_THREADS) {
+      const uint32_t key = skey[i], bin = key >> 12;
+      tmp[cur[bin] + (i - lstart[bin])] = make_uint2(key & 0xFFFu, sval[i]);
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < cbins; j += SORT_THREADS) cur[j] += lcnt[j];
+    // the next tile's clearing of lcnt follows this thread's own reads of it; the barrier at
+    // the top of the loop orders everyone else's
+    __syncthreads();
   }
 }
 
@@ -218,32 +284,6 @@ __global__ void __launch_bounds__(SORT_THREADS) k_coarse_scatter(
 // fine-bin counts sit at fcnt[sprefix[g] fb + f ns + s] (ns = the group's slice count): laid
 // out [group][fine bin][slice], one exclusive scan per group turns them into positions.
 
-// Block-wide exclusive prefix of one value per thread (blockDim.x = T, a multiple of 64);
-// returns the thread's prefix and writes the block total to *sum.
-template <int T>
-__device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t* sh, uint32_t* sum) {
-  const uint32_t lane = __lane_id(), wave = threadIdx.x / 64;
-  uint32_t x = v;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t y = __shfl_up(x, off);
-    if (lane >= (uint32_t)off) x += y;
-  }
-  if (lane == 63) sh[wave] = x;
-  __syncthreads();
-  uint32_t before = 0, all = 0;
-#pragma unroll
-  for (int w = 0; w < T / 64; ++w) {
-    const uint32_t t = sh[w];
-    before += (uint32_t)w < wave ? t : 0u;
-    all += t;
-  }
-  __syncthreads();  // sh may be reused by the caller
-  *sum = all;
-  return before + x - v;
-}
-
-// sprefix[g] = slices before group g, sprefix[groups] = all slices (one workgroup).
 __device__ __forceinline__ uint32_t slices_of(uint32_t size) {
   return size ? (size + FINE_TILE - 1) / FINE_TILE : 1u;
 }
@@ -624,7 +664,7 @@ static void sort_coarse_count(MsmPlan& p, const void* scalars, hipStream_t s) {
 template <class D>
 static void sort_coarse_scatter(MsmPlan& p, hipStream_t s) {
   hipLaunchKernelGGL(k_coarse_scatter<D>, dim3(p.chunks, p.batch * p.windows), dim3(SORT_THREADS),
-                     4 * p.cbins, s, reinterpret_cast<const D*>(p.digits), p.n, p.chunk, p.shift,
+                     0, s, reinterpret_cast<const D*>(p.digits), p.n, p.chunk, p.shift,
                      p.cbins, p.rows_per_set, p.chunks, p.windows, p.cpos, p.tmp, (int)p.merged,
                      (int)p.constant);
   G16_LAUNCH_CHECK("msm coarse scatter");
