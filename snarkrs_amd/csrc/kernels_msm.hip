@@ -406,7 +406,9 @@ __global__ void __launch_bounds__(FINE_THREADS) k_fine_prefix(const uint32_t* cp
   }
 }
 
-// Each slice's entries ordered by bucket into ent, keys set nb + (bin << shift) + fine.
+// Each slice's entries ordered by bucket into ent, keys set nb + (bin << shift) + fine: the
+// slice is first ordered by fine bin in LDS, then written with consecutive lanes on
+// consecutive positions of each bin's run (as the coarse scatter does).
 __global__ void __launch_bounds__(FINE_THREADS) k_fine_scatter(const uint2* tmp,
                                                                const uint32_t* cpos,
                                                                uint32_t per_group,
@@ -415,15 +417,23 @@ __global__ void __launch_bounds__(FINE_THREADS) k_fine_scatter(const uint2* tmp,
                                                                uint32_t cbins, int shift,
                                                                uint32_t nb, const uint32_t* fcnt,
                                                                uint2* ent) {
-  __shared__ uint32_t cur[1u << MAX_FINE_SHIFT];
+  constexpr uint32_t FB = 1u << MAX_FINE_SHIFT;
+  __shared__ uint32_t cur[FB];   // the slice's first position in each fine bin's run
+  __shared__ uint32_t lcnt[FB];  // the slice's entries per fine bin, then their exclusive prefix
+  __shared__ uint32_t sf[FINE_TILE], sv[FINE_TILE];  // staged fine bits and values
+  __shared__ uint32_t scan_sh[FINE_THREADS / 64];
   const Slice sl = find_slice(blockIdx.x, sprefix, groups, cpos, per_group);
-  if (!sl.ns) return;
+  if (!sl.ns) return;  // the same for the whole workgroup
   const uint32_t fb = 1u << shift;
   const uint32_t* pos = fcnt + (size_t)sprefix[sl.g] * fb + sl.s;
-  for (uint32_t j = threadIdx.x; j < fb; j += FINE_THREADS) cur[j] = pos[(size_t)j * sl.ns];
+  for (uint32_t j = threadIdx.x; j < fb; j += FINE_THREADS) {
+    cur[j] = pos[(size_t)j * sl.ns];
+    lcnt[j] = 0;
+  }
   const uint32_t set = sl.g / cbins, bin = sl.g - set * cbins;
   const uint32_t key0 = set * nb + (bin << shift);
   uint2 t[FINE_ITEMS];
+  uint32_t rank[FINE_ITEMS];
 #pragma unroll
   for (int k = 0; k < FINE_ITEMS; ++k) {
     const uint32_t i = sl.start + k * FINE_THREADS + threadIdx.x;
@@ -431,8 +441,33 @@ __global__ void __launch_bounds__(FINE_THREADS) k_fine_scatter(const uint2* tmp,
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < FINE_ITEMS; ++k)
-    if (t[k].x != ~0u) ent[lds_take(cur, t[k].x)] = make_uint2(key0 + t[k].x, t[k].y);
+  for (int k = 0; k < FINE_ITEMS; ++k) rank[k] = t[k].x != ~0u ? lds_take(lcnt, t[k].x) : 0u;
+  __syncthreads();
+  const uint32_t per = (fb + FINE_THREADS - 1) / FINE_THREADS;
+  const uint32_t j0 = threadIdx.x * per;
+  uint32_t loc = 0;
+  for (uint32_t j = j0; j < j0 + per && j < fb; ++j) loc += lcnt[j];
+  uint32_t all;
+  uint32_t run = block_exclusive<FINE_THREADS>(loc, scan_sh, &all);
+  for (uint32_t j = j0; j < j0 + per && j < fb; ++j) {  // each thread rewrites only its bins
+    const uint32_t c = lcnt[j];
+    lcnt[j] = run;
+    run += c;
+  }
+  __syncthreads();
+  const uint32_t* lstart = lcnt;
+#pragma unroll
+  for (int k = 0; k < FINE_ITEMS; ++k) {
+    if (t[k].x == ~0u) continue;
+    const uint32_t q = lstart[t[k].x] + rank[k];
+    sf[q] = t[k].x;
+    sv[q] = t[k].y;
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < all; i += FINE_THREADS) {
+    const uint32_t f = sf[i];
+    ent[cur[f] + (i - lstart[f])] = make_uint2(key0 + f, sv[i]);
+  }
 }
 
 // Exclusive scan of count[0..K) into offset[0..K], offset[K] = total (K <= 1024 SCAN_BLOCK).
