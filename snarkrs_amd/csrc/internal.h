@@ -76,6 +76,9 @@ void launch_h(void* h, const void* a, const void* b, const void* c, size_t n, hi
 void launch_rows_abc(void* a, void* b, void* c, size_t rows, size_t n, hipStream_t s);
 
 // ---- MSM (kernels_msm.hip) ----
+// Window widths the kernels take: up to 20 (buckets of 2^19 per set; the 32-bit digit code has
+// 24 bits for |d| - 1). The standalone entry points keep to 16.
+constexpr int MSM_MAX_C = 20;
 struct MsmPlan {
   int c = 0;              // window width
   int windows = 0;        // ceil(255 / c)

@@ -52,23 +52,24 @@ __device__ __forceinline__ int signed_digit(const uint32_t s[8], int i, int c) {
 }
 
 // Digit codes, one per (window, scalar), window-major. Two widths:
-//   32-bit (c = 16, or constant work): (|d| - 1) | sign << 16, DIGIT_ZERO for d = 0; under
+//   32-bit (c >= 16, or constant work): (|d| - 1) | sign << 24, DIGIT_ZERO for d = 0; under
 //     constant work a zero digit is (i mod nb) | DIGIT_DISCARD instead, an entry whose addition
 //     is computed and dropped;
 //   16-bit (c <= 15, variable work): (|d| - 1) | sign << 15, 0x7FFF for d = 0 (|d| - 1 is at
 //     most 2^14 - 1 there), half the digit traffic.
-static constexpr uint32_t DIGIT_ZERO = 1u << 17;
-static constexpr uint32_t DIGIT_DISCARD = 1u << 18;
+static constexpr uint32_t DIGIT_SIGN = 1u << 24;
+static constexpr uint32_t DIGIT_ZERO = 1u << 25;
+static constexpr uint32_t DIGIT_DISCARD = 1u << 26;
 
 template <class D> struct Digit;
 template <> struct Digit<uint32_t> {
   static __device__ __forceinline__ uint32_t make(int d, uint32_t zero) {
-    return d == 0 ? zero : ((uint32_t)(d < 0 ? -d : d) - 1u) | (d < 0 ? 1u << 16 : 0u);
+    return d == 0 ? zero : ((uint32_t)(d < 0 ? -d : d) - 1u) | (d < 0 ? DIGIT_SIGN : 0u);
   }
   static __device__ __forceinline__ bool zero(uint32_t e) { return (e & DIGIT_ZERO) != 0; }
   static __device__ __forceinline__ uint32_t none() { return DIGIT_ZERO; }
-  static __device__ __forceinline__ uint32_t bucket(uint32_t e) { return e & 0xFFFFu; }
-  static __device__ __forceinline__ uint32_t sign(uint32_t e) { return (e >> 16) & 1u; }
+  static __device__ __forceinline__ uint32_t bucket(uint32_t e) { return e & (DIGIT_SIGN - 1u); }
+  static __device__ __forceinline__ uint32_t sign(uint32_t e) { return (e >> 24) & 1u; }
   static __device__ __forceinline__ uint32_t discard(uint32_t e) {
     return (e & DIGIT_DISCARD) ? 2u : 0u;
   }
@@ -640,7 +641,7 @@ size_t msm_sort_scratch_bytes(uint32_t n, int c, uint32_t batch) {
 
 void msm_plan_bind(MsmPlan& p, void* scratch, uint32_t n, int c, bool merged, bool constant,
                    uint32_t batch) {
-  if (c < 2 || c > 16) throw ArgError{"MSM window width must be in [2, 16]"};
+  if (c < 2 || c > MSM_MAX_C) throw ArgError{"MSM window width must be in [2, 20]"};
   if (batch == 0) throw ArgError{"MSM batch of zero vectors"};
   p.merged = merged;
   p.constant = constant;

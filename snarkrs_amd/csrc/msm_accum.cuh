@@ -589,9 +589,16 @@ inline int ilog2_u32(uint32_t x) {
 // 512 and 256 per slice, and no more than 256 workgroups of four waves (one wave per SIMD):
 // past that, waves share SIMDs and every tree level slows by as much as slicing saves. In
 // practice only a single bucket set (merged windows) is sliced.
+// Past 8192 groups (bucket sets of 2^15 and more: widths above 16) a row is cut into slices of
+// about 2048 groups, up to 256 per row, so that the row sums run across the device instead of
+// in a few long chains.
 inline uint32_t reduce_slices(uint32_t G, uint32_t sets) {
   const uint32_t rows = 1 + (uint32_t)ilog2_u32(G);
   uint32_t S = 1;
+  if (G > 8192) {
+    while (S < 256 && S * 2048 < G) S *= 2;
+    return S;
+  }
   while (S < 64 && 2 * S * 512 <= G && (uint64_t)2 * S * rows * sets <= 256) S *= 2;
   return S;
 }

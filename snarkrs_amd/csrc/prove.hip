@@ -200,11 +200,13 @@ static KeyPlan plan_key(const g16hip_key* k, bool merged) {
   KeyPlan P;
   P.merged = merged;
   // The witness digits feed A, B1, L (G1) and B2 (G2); H feeds one G1 MSM.
-  // G16HIP_C_WITNESS / G16HIP_C_H override the widths (2..16).
+  // G16HIP_C_WITNESS / G16HIP_C_H override the widths (2..16, and H up to 20 over window
+  // tables).
   P.cw = merged ? msm_choose_c_merged((uint32_t)nv, 3, 1) : msm_choose_c((uint32_t)nv, 3, 1);
   P.ch = merged ? msm_choose_c_merged((uint32_t)nh, 1, 0) : msm_default_c((uint32_t)nh);
   if (const char* e = std::getenv("G16HIP_C_WITNESS")) P.cw = std::max(2, std::min(16, std::atoi(e)));
-  if (const char* e = std::getenv("G16HIP_C_H")) P.ch = std::max(2, std::min(16, std::atoi(e)));
+  if (const char* e = std::getenv("G16HIP_C_H"))
+    P.ch = std::max(2, std::min(merged ? MSM_MAX_C : 16, std::atoi(e)));
   const size_t tw = merged ? (255 + P.cw - 1) / P.cw : 1, th = merged ? (255 + P.ch - 1) / P.ch : 1;
   size_t b = 0;
   if (k->row_ptr[0]) {

@@ -7,7 +7,9 @@
 * a prove from a snarkjs .zkey / .wtns pair written to disk and read back by the codec
   (the reference's warm bench input path, bin/snarkrs/src/bench.rs:461-467);
 * the device-memory budget: a key over it is refused as DeviceMemory with nothing allocated,
-  a key at it prepares and proves (crates/cuda/src/backend.rs:722-771).
+  a key at it prepares and proves (crates/cuda/src/backend.rs:722-771);
+* H MSM widths above 16 over window tables (G16HIP_C_H = 17 and 20: buckets of 2^16 and 2^19,
+  the 24-bit digit code, rows cut into many slices), which the width model does not pick.
 """
 import numpy as np
 import pytest
@@ -50,6 +52,15 @@ def test_prove_without_window_tables(dev, orc, monkeypatch, nc, bools):
     monkeypatch.setenv("G16HIP_WINDOW_TABLES", "0")
     syn = Synth(orc, nc, n_public=2, n_private_in=max(8, nc // 11), bool_permille=bools,
                 input_bool_permille=bools, seed=80 + nc, setup_seed=81 + nc)
+    key, wit = syn.key()
+    _check_against_oracle(dev, orc, key, wit)
+
+
+@pytest.mark.parametrize("ch", [17, 20])
+def test_prove_with_wide_h_windows(dev, orc, monkeypatch, ch):
+    monkeypatch.setenv("G16HIP_C_H", str(ch))
+    syn = Synth(orc, 30_000, n_public=2, n_private_in=2000, bool_permille=400,
+                input_bool_permille=400, seed=60 + ch, setup_seed=61 + ch)
     key, wit = syn.key()
     _check_against_oracle(dev, orc, key, wit)
 
