@@ -926,4 +926,124 @@ HD G2Aff g2_generator() {
   return {{from_std(x0), from_std(x1)}, {from_std(y0), from_std(y1)}};
 }
 
+// ---------------------------------------------------------------------------------------
+// Complete formulas for constant-work MSMs (Renes, Costello, Batina, "Complete addition
+// formulas for prime order elliptic curves", 2016, Algorithms 7 and 8 for a = 0). A point is
+// (X : Y : Z) with x = X / Z, y = Y / Z, and the identity (0 : 1 : 0); the formulas have no
+// exceptional case (doubling, opposite points and the identity come out of the same
+// products), so an addition's instruction stream does not depend on its operands. b3 = 3 b:
+// 9 on G1 (y^2 = x^3 + 3), 9 / (9 + u) = (81 - 9 u) / 82 on the G2 twist
+// (y^2 = x^3 + 3 / (9 + u)); the MSM form's constant below is that value times 2^261 mod q.
+// ---------------------------------------------------------------------------------------
+template <class F>
+struct Proj {
+  F X, Y, Z;
+};
+
+template <class C>
+HD Fp<C> f_sel(bool c, const Fp<C>& a, const Fp<C>& b) {
+  Fp<C> r;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) r.v[i] = c ? a.v[i] : b.v[i];
+  return r;
+}
+template <bool INL, class B>
+HD Fq2T<INL, B> f_sel(bool c, const Fq2T<INL, B>& a, const Fq2T<INL, B>& b) {
+  return {f_sel(c, a.c0, b.c0), f_sel(c, a.c1, b.c1)};
+}
+template <class F>
+HD Proj<F> proj_sel(bool c, const Proj<F>& a, const Proj<F>& b) {
+  return {f_sel(c, a.X, b.X), f_sel(c, a.Y, b.Y), f_sel(c, a.Z, b.Z)};
+}
+template <class F>
+HD Proj<F> proj_inf() {
+  return {f_zero<F>(), f_one<F>(), f_zero<F>()};
+}
+
+template <class F> struct CurveB3;
+template <> struct CurveB3<Fm> {
+  static HD Fm times(const Fm& x) {  // 9 x
+    const Fm x8 = dbl(dbl(dbl(x)));
+    return add(x8, x);
+  }
+};
+template <> struct CurveB3<Fm2i> {
+  static HD Fm2i times(const Fm2i& x) {
+    constexpr uint32_t C0[8] = {0x295cc5c1u, 0x582d62c9u, 0x8acc80b0u, 0x9042cb3du,
+                                0x9e1cb150u, 0xf6155910u, 0x1a987d08u, 0x1b2ff417u};
+    constexpr uint32_t C1[8] = {0x01e5cc12u, 0xdeb7b4a1u, 0x9c61623au, 0x0b4079f4u,
+                                0xfd7d8f2bu, 0x4b0dc77fu, 0x24d1bcddu, 0x17dd1076u};
+    Fm2i k;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      k.c0.v[i] = C0[i];
+      k.c1.v[i] = C1[i];
+    }
+    return mul(x, k);
+  }
+};
+
+// Algorithm 7: P + Q for any two points (12 products and 2 by b3).
+template <class F>
+HD Proj<F> proj_add(const Proj<F>& p, const Proj<F>& q) {
+  F t0 = mul(p.X, q.X);
+  F t1 = mul(p.Y, q.Y);
+  F t2 = mul(p.Z, q.Z);
+  SB();
+  F t3 = sub(mul(add(p.X, p.Y), add(q.X, q.Y)), add(t0, t1));  // X1 Y2 + X2 Y1
+  SB();
+  F t4 = sub(mul(add(p.Y, p.Z), add(q.Y, q.Z)), add(t1, t2));  // Y1 Z2 + Y2 Z1
+  SB();
+  F t5 = sub(mul(add(p.X, p.Z), add(q.X, q.Z)), add(t0, t2));  // X1 Z2 + X2 Z1
+  SB();
+  t0 = add(dbl(t0), t0);                // 3 X1 X2
+  t2 = CurveB3<F>::times(t2);           // b3 Z1 Z2
+  const F s = add(t1, t2), d = sub(t1, t2);
+  t5 = CurveB3<F>::times(t5);           // b3 (X1 Z2 + X2 Z1)
+  Proj<F> r;
+  r.X = mul_sub(t3, d, t4, t5);
+  SB();
+  r.Y = add(mul(s, d), mul(t5, t0));
+  SB();
+  r.Z = add(mul(t4, s), mul(t0, t3));
+  return r;
+}
+
+// Algorithm 8: P + A for any point P and an affine A (not the identity: the affine encoding
+// has none; a caller selects around it). 11 products and 2 by b3.
+template <class F>
+HD Proj<F> proj_add_aff(const Proj<F>& p, const Aff<F>& a) {
+  F t0 = mul(p.X, a.x);
+  F t1 = mul(p.Y, a.y);
+  SB();
+  F t3 = sub(mul(add(a.x, a.y), add(p.X, p.Y)), add(t0, t1));  // X1 Y2 + X2 Y1
+  SB();
+  F t4 = add(mul(a.y, p.Z), p.Y);  // Y1 + Y2 Z1
+  SB();
+  F t5 = add(mul(a.x, p.Z), p.X);  // X1 + X2 Z1
+  SB();
+  t0 = add(dbl(t0), t0);           // 3 X1 X2
+  const F t2 = CurveB3<F>::times(p.Z);
+  const F s = add(t1, t2), d = sub(t1, t2);
+  t5 = CurveB3<F>::times(t5);
+  Proj<F> r;
+  r.X = mul_sub(t3, d, t4, t5);
+  SB();
+  r.Y = add(mul(s, d), mul(t5, t0));
+  SB();
+  r.Z = add(mul(t4, s), mul(t0, t3));
+  return r;
+}
+
+// (X : Y : Z) as XYZZ (X Z, Y Z^2, Z^2, Z^3): the identity (Z = 0) becomes XYZZ's (ZZ = 0).
+template <class F>
+HD Xyzz<F> proj_to_xyzz(const Proj<F>& p) {
+  Xyzz<F> r;
+  r.ZZ = sqr(p.Z);
+  r.ZZZ = mul(r.ZZ, p.Z);
+  r.X = mul(p.X, p.Z);
+  r.Y = mul(p.Y, r.ZZ);
+  return r;
+}
+
 }  // namespace g16

@@ -106,6 +106,33 @@ __device__ __forceinline__ void st_xyzz(uint4* base, size_t i, const Xyzz<F>& r)
   st_f(p + 3 * W, r.ZZZ);
 }
 
+// How the merge and reduction kernels see a point: XYZZ with the exceptional-case shortcuts
+// (XOps), or, for constant-work MSMs, (X : Y : Z) with the complete formulas (POps), stored in
+// an XYZZ-sized slot as (X, Y, Z, 0). `out` is the form the host reads (XYZZ).
+template <class F>
+struct XOps {
+  using P = Xyzz<F>;
+  static __device__ __forceinline__ P inf() { return xyzz_inf<F>(); }
+  static __device__ __forceinline__ P add(const P& a, const P& b) { return xyzz_add(a, b); }
+  static __device__ __forceinline__ P load(const uint4* base, size_t i) { return ld_xyzz<F>(base, i); }
+  static __device__ __forceinline__ void store(uint4* base, size_t i, const P& p) { st_xyzz(base, i, p); }
+  static __device__ __forceinline__ Xyzz<F> out(const P& p) { return p; }
+};
+template <class F>
+struct POps {
+  using P = Proj<F>;
+  static __device__ __forceinline__ P inf() { return proj_inf<F>(); }
+  static __device__ __forceinline__ P add(const P& a, const P& b) { return proj_add(a, b); }
+  static __device__ __forceinline__ P load(const uint4* base, size_t i) {
+    const Xyzz<F> x = ld_xyzz<F>(base, i);
+    return {x.X, x.Y, x.ZZ};
+  }
+  static __device__ __forceinline__ void store(uint4* base, size_t i, const P& p) {
+    st_xyzz(base, i, Xyzz<F>{p.X, p.Y, p.Z, f_zero<F>()});
+  }
+  static __device__ __forceinline__ Xyzz<F> out(const P& p) { return proj_to_xyzz(p); }
+};
+
 // Window tables for merged MSMs (msm_window_tables): entry w n + i = 2^(c w) P_i, affine, from
 // the previous window's point by c doublings and one inversion; infinity stays (0, 0).
 template <class F>
@@ -228,9 +255,7 @@ struct Acc<Fm> {
 
 // Segment t of the sorted entries: its runs added into XYZZ accumulators, a run whole inside
 // the segment stored as its bucket, the end pieces to pfirst / plast.
-// CONST: constant-work entries (index << 2 | discard << 1 | sign): the addition of a
-// discarded entry is computed and its result dropped by a select.
-template <class F, bool CONST>
+template <class F>
 __device__ __forceinline__ void accumulate_segment(size_t t, const uint2* __restrict__ ent,
                                                    const uint32_t* total, size_t nseg,
                                                    uint32_t seg,
@@ -247,12 +272,9 @@ __device__ __forceinline__ void accumulate_segment(size_t t, const uint2* __rest
   using A = Acc<F>;
   typename A::T acc = A::inf();
   // The next entry's point is loaded before this one is added, so the load latency of a
-  // random base hides behind a mixed addition. G2 holds it in accumulation registers (3 %
-  // faster); its constant-work loop would spill to scratch, so it loads in turn.
-  constexpr bool PREFETCH = sizeof(F) == 32 || !CONST;
-  constexpr int SH = CONST ? 2 : 1;
+  // random base hides behind a mixed addition.
   uint32_t v = ent[b].y;
-  Aff<F> a = ld_aff<F>(bases, v >> SH);  // merged: v >> SH = w n + i, a window-table entry
+  Aff<F> a = ld_aff<F>(bases, v >> 1);  // merged: v >> 1 = w n + i, a window-table entry
   for (size_t k = b; k < e; ++k) {
     uint32_t key = ent[k].x;
     if (key != cur) {
@@ -264,26 +286,15 @@ __device__ __forceinline__ void accumulate_segment(size_t t, const uint2* __rest
       starts = true;
       rs = k;
     }
-    if (!PREFETCH && k != b) {
-      v = ent[k].y;
-      a = ld_aff<F>(bases, v >> SH);
-    }
     uint32_t v_next = 0;
     Aff<F> a_next;
-    if (PREFETCH && k + 1 < e) {
+    if (k + 1 < e) {
       v_next = ent[k + 1].y;
-      a_next = ld_aff<F>(bases, v_next >> SH);
+      a_next = ld_aff<F>(bases, v_next >> 1);
     }
-    if (CONST) {
-      typename A::T sum = A::madd(acc, (v & 1) ? aff_neg(a) : a);
-      acc = (v & 2) ? acc : sum;
-    } else {
-      acc = A::madd(acc, (v & 1) ? aff_neg(a) : a);
-    }
-    if (PREFETCH) {
-      v = v_next;
-      a = a_next;
-    }
+    acc = A::madd(acc, (v & 1) ? aff_neg(a) : a);
+    v = v_next;
+    a = a_next;
   }
   bool ends = (e == T) || ent[e].x != cur;
   const Xyzz<F> sum = A::out(acc);
@@ -295,15 +306,128 @@ __device__ __forceinline__ void accumulate_segment(size_t t, const uint2* __rest
   }
 }
 
-template <class F, bool CONST>
+template <class F>
 __global__ void __launch_bounds__(128) k_accumulate(const uint2* __restrict__ ent,
                                                     const uint32_t* total, size_t nseg,
                                                     uint32_t seg,
                                                     const uint4* __restrict__ bases,
                                                     uint4* buckets, uint4* pfirst,
                                                     uint4* plast) {
-  accumulate_segment<F, CONST>((size_t)blockIdx.x * blockDim.x + threadIdx.x, ent, total, nseg,
-                               seg, bases, buckets, pfirst, plast);
+  accumulate_segment<F>((size_t)blockIdx.x * blockDim.x + threadIdx.x, ent, total, nseg, seg,
+                        bases, buckets, pfirst, plast);
+}
+
+// ---------------------------------------------------------------------------------------
+// Constant work (opt in; crates/cuda/src/msm.rs:143-164): every (scalar, window) has an entry
+// (a zero digit's is flagged discard), and from the accumulation to the reduction no decision
+// is a branch on the data: points are (X : Y : Z) added by the complete formulas (bn254.cuh),
+// loops run a count the plan fixes, and what the data decide is picked by selects after the
+// work is done. What the data still decide is where stores land (a bucket's address), as
+// with the counting sort before it. Not constant time.
+//
+// Accumulation: thread t runs exactly `seg` iterations over segment t (a short last segment
+// repeats its last entry, added and dropped). Every iteration stores the running sum once:
+// to its bucket when a run that began inside the segment ends, to slot 2t (head) when the
+// segment's first run ends, else to slot 2t + 1 (tail), which the end overwrites with the
+// last run. Slot rows hold the bucket key, or FOLD_NONE.
+// ---------------------------------------------------------------------------------------
+static constexpr uint32_t FOLD_NONE = 0xFFFFFFFFu;
+static constexpr uint32_t FOLD_LEN = 32;  // slots a fold thread walks
+
+template <class F>
+__global__ void __launch_bounds__(128) k_accumulate_const(const uint2* __restrict__ ent,
+                                                          const uint32_t* total, size_t nseg,
+                                                          uint32_t seg,
+                                                          const uint4* __restrict__ bases,
+                                                          uint4* buckets, uint4* slots,
+                                                          uint32_t* rows) {
+  using O = POps<F>;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nseg) return;
+  const size_t b = t * seg, T = total[0];  // the plan's entry count: b < T for t < nseg
+  if (b >= T) {  // not reached on a full entry list; keeps every read inside it regardless
+    rows[2 * t] = FOLD_NONE;
+    rows[2 * t + 1] = FOLD_NONE;
+    return;
+  }
+  const uint32_t len = (uint32_t)(b + seg < T ? seg : T - b);
+  uint32_t cur = ent[b].x;
+  bool first = true;
+  const Proj<F> inf = O::inf();
+  Proj<F> acc = inf;
+  for (uint32_t i = 0; i < seg; ++i) {
+    const bool live = i < len;
+    const uint2 e = ent[b + (live ? i : len - 1)];
+    const uint32_t key = live ? e.x : cur;
+    const bool boundary = key != cur;
+    const bool to_bucket = boundary && !first;
+    const size_t slot = 2 * t + ((boundary && first) ? 0 : 1);
+    O::store(to_bucket ? buckets : slots, to_bucket ? cur : slot, acc);
+    rows[slot] = cur;
+    first = first && !boundary;
+    acc = proj_sel(boundary, inf, acc);
+    cur = key;
+    Aff<F> a = ld_aff<F>(bases, e.y >> 2);
+    const bool keep = !live || (e.y & 2u) || aff_is_inf(a);
+    a.y = f_sel((e.y & 1u) != 0, neg(a.y), a.y);
+    const Proj<F> sum = proj_add_aff(acc, a);
+    acc = proj_sel(keep, acc, sum);
+  }
+  const size_t slot = 2 * t + (first ? 0 : 1);
+  O::store(slots, slot, acc);
+  rows[slot] = cur;
+  rows[2 * t + 1] = first ? FOLD_NONE : cur;
+}
+
+// One level of the fold: thread g walks slots [g FOLD_LEN, (g + 1) FOLD_LEN) of the level's m
+// (exactly FOLD_LEN iterations; past m it repeats the last slot as empty), adding runs of equal
+// rows. A run that ends inside the group, other than its first, is complete (all of a bucket's
+// pieces are consecutive) and goes to its bucket; the group's first and last runs go to slots
+// 2g and 2g + 1 of the next level, or, on the last level (one group), to their buckets too.
+template <class F>
+__global__ void __launch_bounds__(128) k_fold_const(const uint4* in, const uint32_t* in_rows,
+                                                    uint32_t m, uint4* out, uint32_t* out_rows,
+                                                    uint4* buckets, int last) {
+  using O = POps<F>;
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t groups = (m + FOLD_LEN - 1) / FOLD_LEN;
+  if (g >= groups) return;
+  const uint32_t lo = g * FOLD_LEN, hi = lo + FOLD_LEN < m ? lo + FOLD_LEN : m;
+  uint32_t cur = FOLD_NONE;
+  bool first = true;
+  const Proj<F> inf = O::inf();
+  Proj<F> acc = inf;
+  for (uint32_t i = lo; i < lo + FOLD_LEN; ++i) {
+    const bool live = i < hi;
+    const uint32_t j = live ? i : hi - 1;
+    const uint32_t r = live ? in_rows[j] : FOLD_NONE;
+    const bool empty = r == FOLD_NONE;
+    const bool boundary = !empty && r != cur;
+    const bool finished = boundary && cur != FOLD_NONE;
+    const bool to_head = finished && first && !last;
+    const bool to_bucket = finished && !to_head;
+    const size_t slot = 2 * (size_t)g + (to_head ? 0 : 1);
+    O::store(to_bucket ? buckets : out, to_bucket ? cur : slot, acc);
+    out_rows[slot] = cur;
+    first = first && !finished;
+    acc = proj_sel(boundary, inf, acc);
+    cur = boundary ? r : cur;
+    const Proj<F> q = proj_sel(empty, inf, O::load(in, j));
+    acc = O::add(acc, q);
+  }
+  const bool has_run = cur != FOLD_NONE;
+  const size_t slot = 2 * (size_t)g + (first ? 0 : 1);
+  const bool to_bucket = last && has_run;
+  O::store(to_bucket ? buckets : out, to_bucket ? cur : slot, acc);
+  out_rows[slot] = has_run ? cur : FOLD_NONE;
+  out_rows[2 * (size_t)g + 1] = (has_run && !first) ? cur : FOLD_NONE;
+}
+
+// Every bucket starts as the identity (a bucket no entry names is never stored).
+template <class F>
+__global__ void k_fill_proj_inf(uint4* buckets, size_t K) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < K) POps<F>::store(buckets, i, POps<F>::inf());
 }
 
 // Several MSMs' bucket accumulations as one persistent launch over a device-side work queue:
@@ -341,8 +465,8 @@ __global__ void __launch_bounds__(QUEUE_TILE) k_accumulate_queue(AccQueue q, uin
     int d = 0;
     while (d + 1 < q.n && t >= q.start[d + 1]) ++d;
     const AccQueueJob& j = q.job[d];
-    accumulate_segment<F, false>((size_t)(t - q.start[d]) * QUEUE_TILE + threadIdx.x, j.ent,
-                                 j.total, j.nseg, j.seg, j.bases, j.buckets, j.pfirst, j.plast);
+    accumulate_segment<F>((size_t)(t - q.start[d]) * QUEUE_TILE + threadIdx.x, j.ent, j.total,
+                          j.nseg, j.seg, j.bases, j.buckets, j.pfirst, j.plast);
   }
 }
 
@@ -451,11 +575,11 @@ __global__ void __launch_bounds__(256) k_merge_wide(const uint32_t* offset, uint
 }
 
 // Sum of BLOCK per-thread values in LDS, left in sh[0].
-template <class F, int BLOCK>
-__device__ __forceinline__ void block_tree(Xyzz<F>* sh) {
+template <class Ops, int BLOCK>
+__device__ __forceinline__ void block_tree(typename Ops::P* sh) {
   for (int off = BLOCK / 2; off > 0; off >>= 1) {
     __syncthreads();
-    if ((int)threadIdx.x < off) sh[threadIdx.x] = xyzz_add(sh[threadIdx.x], sh[threadIdx.x + off]);
+    if ((int)threadIdx.x < off) sh[threadIdx.x] = Ops::add(sh[threadIdx.x], sh[threadIdx.x + off]);
   }
   __syncthreads();
 }
@@ -471,7 +595,7 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_tasks(const uint32_t* cnt, cons
     uint32_t lo = tasks[2 * t], hi = tasks[2 * t + 1];
     uint32_t seg = lo + threadIdx.x;
     sh[threadIdx.x] = seg < hi ? ld_xyzz<F>(pfirst, seg) : xyzz_inf<F>();
-    block_tree<F, BLOCK>(sh);
+    block_tree<XOps<F>, BLOCK>(sh);
     if (threadIdx.x == 0) st_xyzz(task_sum, t, sh[0]);
     __syncthreads();
   }
@@ -498,7 +622,7 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_finish(const uint32_t* offset,
       acc = xyzz_add(acc, ld_xyzz<F>(src, i));
     }
     sh[threadIdx.x] = acc;
-    block_tree<F, BLOCK>(sh);
+    block_tree<XOps<F>, BLOCK>(sh);
     if (threadIdx.x == 0) st_xyzz(buckets, k, sh[0]);
     __syncthreads();
   }
@@ -519,7 +643,7 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_finish(const uint32_t* offset,
 // buckets: 8192 groups) costs two short trees rather than one long strided chain. Every
 // chain is short; the doublings of the bit weights happen once, in the host's Horner pass
 // over all windows.
-template <class F>
+template <class F, class Ops = XOps<F>>
 __global__ void __launch_bounds__(128) k_reduce_groups(const uint4* buckets, uint32_t nb,
                                                        int W, uint32_t B, uint4* T,
                                                        uint4* U) {
@@ -527,53 +651,55 @@ __global__ void __launch_bounds__(128) k_reduce_groups(const uint4* buckets, uin
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (uint32_t)W * G) return;
   size_t base = (size_t)t * B;  // = w * nb + g * B
-  Xyzz<F> run = xyzz_inf<F>(), acc = xyzz_inf<F>();
+  typename Ops::P run = Ops::inf(), acc = Ops::inf();
   for (int i = (int)B - 1; i >= 0; --i) {
-    run = xyzz_add(run, ld_xyzz<F>(buckets, base + i));
-    acc = xyzz_add(acc, run);
+    run = Ops::add(run, Ops::load(buckets, base + i));
+    acc = Ops::add(acc, run);
   }
-  st_xyzz(T, t, run);
-  st_xyzz(U, t, acc);
+  Ops::store(T, t, run);
+  Ops::store(U, t, acc);
 }
 
-// Workgroup x = ((w rows + r) S + slice); writes out[x].
-template <class F, int BLOCK>
+// Workgroup x = ((w rows + r) S + slice); writes out[x] (the host's XYZZ when S = 1, else a
+// slice sum in the policy's form for k_reduce_slices).
+template <class F, int BLOCK, class Ops = XOps<F>>
 __global__ void __launch_bounds__(BLOCK) k_reduce_rows(const uint4* T, const uint4* U,
                                                        uint32_t G, uint32_t rows, uint32_t S,
                                                        uint4* out) {
-  __shared__ Xyzz<F> sh[BLOCK];
+  __shared__ typename Ops::P sh[BLOCK];
   const uint32_t sl = blockIdx.x % S, wr = blockIdx.x / S;
   const uint32_t w = wr / rows, r = wr % rows;
   size_t base = (size_t)w * G;
-  Xyzz<F> acc = xyzz_inf<F>();
+  typename Ops::P acc = Ops::inf();
   const uint4* src = r == 0 ? U : T;
   const uint32_t k = r == 0 ? 0 : r - 1, low = (1u << k) - 1u, count = r == 0 ? G : G / 2;
   const uint32_t per = count / S, m0 = sl * per;
   for (uint32_t m = m0 + threadIdx.x; m < m0 + per; m += BLOCK) {
     // row 0: every g; row k + 1: m with a 1 inserted at bit k
     const uint32_t g = r == 0 ? m : ((m & ~low) << 1) | (1u << k) | (m & low);
-    acc = xyzz_add(acc, ld_xyzz<F>(src, base + g));
+    acc = Ops::add(acc, Ops::load(src, base + g));
   }
   sh[threadIdx.x] = acc;
-  block_tree<F, BLOCK>(sh);
+  block_tree<Ops, BLOCK>(sh);
   if (threadIdx.x == 0) {
-    st_xyzz(out, blockIdx.x, sh[0]);
+    if (S == 1) st_xyzz(out, blockIdx.x, Ops::out(sh[0]));
+    else Ops::store(out, blockIdx.x, sh[0]);
     __threadfence_system();  // `out` may be pinned host memory the host reads next
   }
 }
 
-// Row x = w rows + r: out[x] = sum of parts[x S .. x S + S).
-template <class F, int BLOCK>
+// Row x = w rows + r: out[x] = sum of parts[x S .. x S + S), in the host's XYZZ.
+template <class F, int BLOCK, class Ops = XOps<F>>
 __global__ void __launch_bounds__(BLOCK) k_reduce_slices(const uint4* parts, uint32_t S,
                                                          uint4* out) {
-  __shared__ Xyzz<F> sh[BLOCK];
-  Xyzz<F> acc = xyzz_inf<F>();
+  __shared__ typename Ops::P sh[BLOCK];
+  typename Ops::P acc = Ops::inf();
   for (uint32_t i = threadIdx.x; i < S; i += BLOCK)
-    acc = xyzz_add(acc, ld_xyzz<F>(parts, (size_t)blockIdx.x * S + i));
+    acc = Ops::add(acc, Ops::load(parts, (size_t)blockIdx.x * S + i));
   sh[threadIdx.x] = acc;
-  block_tree<F, BLOCK>(sh);
+  block_tree<Ops, BLOCK>(sh);
   if (threadIdx.x == 0) {
-    st_xyzz(out, blockIdx.x, sh[0]);
+    st_xyzz(out, blockIdx.x, Ops::out(sh[0]));
     __threadfence_system();  // `out` may be pinned host memory the host reads next
   }
 }
@@ -606,8 +732,14 @@ inline uint32_t reduce_slices(uint32_t G, uint32_t sets) {
 // Scratch of one accumulation (one base vector over one sorted entry list): buckets, the
 // segment end partials, heavy-bucket bookkeeping, task sums, reduction groups, counters.
 struct AccumLayout {
-  size_t buckets, pfirst, plast, heavy, tasks, task_sum, gT, gU, parts, cnt, total;
+  size_t buckets, pfirst, plast, heavy, tasks, task_sum, gT, gU, parts, cnt;
+  size_t fold0, frows0, fold1, frows1;  // constant work: the fold's two levels of slots
+  size_t total;
 };
+
+// Slots the constant-work fold needs: 2 per segment, then 2 per group of FOLD_LEN.
+inline size_t fold_slots0(size_t nseg) { return 2 * nseg; }
+inline size_t fold_slots1(size_t nseg) { return 2 * ((2 * nseg + FOLD_LEN - 1) / FOLD_LEN) + 2; }
 
 inline AccumLayout accum_layout(const MsmPlan& p, size_t pt) {
   size_t K = (size_t)msm_bucket_sets(p) * p.nb;
@@ -635,6 +767,10 @@ inline AccumLayout accum_layout(const MsmPlan& p, size_t pt) {
   const uint32_t S = reduce_slices((uint32_t)G, (uint32_t)msm_bucket_sets(p));
   L.parts = take(S > 1 ? pt * S * (1 + ilog2_u32((uint32_t)G)) * msm_bucket_sets(p) : 0);
   L.cnt = take(16);
+  L.fold0 = take(p.constant ? pt * fold_slots0(nseg) : 0);
+  L.frows0 = take(p.constant ? 4 * fold_slots0(nseg) : 0);
+  L.fold1 = take(p.constant ? pt * fold_slots1(nseg) : 0);
+  L.frows1 = take(p.constant ? 4 * fold_slots1(nseg) : 0);
   L.total = off;
   return L;
 }
@@ -647,7 +783,10 @@ inline size_t msm_accum_scratch_bytes_impl(const MsmPlan& p, int group) {
 struct AccumPtrs {
   uint4 *buckets, *pfirst, *plast, *task_sum, *gT, *gU, *parts;
   uint32_t *heavy, *tasks, *cnt;
+  uint4 *fold0, *fold1;
+  uint32_t *frows0, *frows1;
   uint32_t seg;  // entries per segment
+  size_t nseg;
 };
 
 // Launchers of the merge and reduction kernels over field F. The G1 ones are defined here;
@@ -662,7 +801,34 @@ struct Tail {
 };
 
 template <class F>
+inline void tail_fold_const(const AccumPtrs& a, hipStream_t s) {
+  // Levels until one group holds every slot: 2 nseg slots, then 2 per group of FOLD_LEN.
+  size_t m = fold_slots0(a.nseg);
+  const uint4* in = a.fold0;
+  const uint32_t* in_rows = a.frows0;
+  bool to1 = true;
+  while (m) {
+    const size_t groups = (m + FOLD_LEN - 1) / FOLD_LEN;
+    const int last = groups == 1;
+    uint4* out = to1 ? a.fold1 : a.fold0;
+    uint32_t* out_rows = to1 ? a.frows1 : a.frows0;
+    hipLaunchKernelGGL(k_fold_const<F>, dim3((unsigned)((groups + 127) / 128)), dim3(128), 0, s,
+                       in, in_rows, (uint32_t)m, out, out_rows, a.buckets, last);
+    G16_LAUNCH_CHECK("msm fold (constant work)");
+    if (last) break;
+    in = out;
+    in_rows = out_rows;
+    m = 2 * groups;
+    to1 = !to1;
+  }
+}
+
+template <class F>
 inline void tail_merge_impl(const MsmPlan& p, const AccumPtrs& a, hipStream_t s) {
+  if (p.constant) {
+    tail_fold_const<F>(a, s);
+    return;
+  }
   size_t K = (size_t)msm_bucket_sets(p) * p.nb;
   hipLaunchKernelGGL(k_clear_cnt<0>, dim3(1), dim3(64), 0, s, a.cnt);
   G16_LAUNCH_CHECK("msm clear counters");
@@ -685,26 +851,34 @@ inline void tail_merge_impl(const MsmPlan& p, const AccumPtrs& a, hipStream_t s)
   G16_LAUNCH_CHECK("msm heavy finish");
 }
 
-template <class F>
-inline void tail_reduce_impl(const MsmPlan& p, const AccumPtrs& a, uint32_t B, void* window_out,
-                             hipStream_t s) {
+template <class F, class Ops>
+inline void tail_reduce_ops(const MsmPlan& p, const AccumPtrs& a, uint32_t B, void* window_out,
+                            hipStream_t s) {
   uint32_t G = p.nb / B;
   uint32_t rows = 1 + (uint32_t)ilog2_u32(G);
   const int sets = msm_bucket_sets(p);
   uint32_t nthreads = (uint32_t)sets * G;
-  hipLaunchKernelGGL(k_reduce_groups<F>, dim3((nthreads + 127) / 128), dim3(128), 0, s,
+  hipLaunchKernelGGL((k_reduce_groups<F, Ops>), dim3((nthreads + 127) / 128), dim3(128), 0, s,
                      a.buckets, p.nb, sets, B, a.gT, a.gU);
   G16_LAUNCH_CHECK("msm reduce groups");
   const uint32_t S = reduce_slices(G, (uint32_t)sets);
   uint4* wout = static_cast<uint4*>(window_out);
-  hipLaunchKernelGGL((k_reduce_rows<F, 256>), dim3((unsigned)sets * rows * S), dim3(256), 0, s,
-                     a.gT, a.gU, G, rows, S, S > 1 ? a.parts : wout);
+  hipLaunchKernelGGL((k_reduce_rows<F, 256, Ops>), dim3((unsigned)sets * rows * S), dim3(256), 0,
+                     s, a.gT, a.gU, G, rows, S, S > 1 ? a.parts : wout);
   G16_LAUNCH_CHECK("msm reduce rows");
   if (S > 1) {
-    hipLaunchKernelGGL((k_reduce_slices<F, 64>), dim3((unsigned)sets * rows), dim3(64), 0, s,
+    hipLaunchKernelGGL((k_reduce_slices<F, 64, Ops>), dim3((unsigned)sets * rows), dim3(64), 0, s,
                        a.parts, S, wout);
     G16_LAUNCH_CHECK("msm reduce slices");
   }
+}
+
+// Constant-work plans reduce their projective buckets with the complete formulas.
+template <class F>
+inline void tail_reduce_impl(const MsmPlan& p, const AccumPtrs& a, uint32_t B, void* window_out,
+                             hipStream_t s) {
+  if (p.constant) tail_reduce_ops<F, POps<F>>(p, a, B, window_out, s);
+  else tail_reduce_ops<F, XOps<F>>(p, a, B, window_out, s);
 }
 
 // F: the coordinate field in the MSM's Montgomery form R' = 2^261 (Fm, or Fm2i for G2); the
@@ -725,7 +899,12 @@ inline AccumPtrs accum_ptrs(const MsmPlan& p, void* scratch) {
   a.gU = reinterpret_cast<uint4*>(q + L.gU);
   a.parts = reinterpret_cast<uint4*>(q + L.parts);
   a.cnt = reinterpret_cast<uint32_t*>(q + L.cnt);
+  a.fold0 = reinterpret_cast<uint4*>(q + L.fold0);
+  a.frows0 = reinterpret_cast<uint32_t*>(q + L.frows0);
+  a.fold1 = reinterpret_cast<uint4*>(q + L.fold1);
+  a.frows1 = reinterpret_cast<uint32_t*>(q + L.frows1);
   a.seg = accum_seg(p, PT);
+  a.nseg = (p.max_entries + a.seg - 1) / a.seg;
   return a;
 }
 
@@ -737,6 +916,8 @@ inline void accumulate_queue_impl(const MsmPlan* const* plans, const void* const
                                   void* const* scratch, void* const* window_out, int n,
                                   uint32_t* counter, int grid, hipStream_t s) {
   if (n < 1 || n > QUEUE_JOBS) throw ArgError{"accumulation queue of " + std::to_string(n) + " jobs"};
+  for (int d = 0; d < n; ++d)
+    if (plans[d]->constant) throw ArgError{"constant-work MSMs do not go through the queue"};
   AccQueue q{};
   q.n = n;
   q.start[0] = 0;
@@ -766,34 +947,24 @@ inline void accumulate_queue_impl(const MsmPlan* const* plans, const void* const
 template <class F>
 inline void accumulate_impl(const MsmPlan& p, const void* bases, void* scratch, void* window_out,
                             hipStream_t s, hipEvent_t* ev) {
-  constexpr size_t PT = sizeof(Xyzz<F>);
-  const uint32_t seg = accum_seg(p, PT);
-  size_t nseg = (p.max_entries + seg - 1) / seg;
-  uint32_t B = reduce_group_size(p);
-  AccumLayout L = accum_layout(p, PT);
-  char* q = static_cast<char*>(scratch);
-  AccumPtrs a;
-  a.buckets = reinterpret_cast<uint4*>(q + L.buckets);
-  a.pfirst = reinterpret_cast<uint4*>(q + L.pfirst);
-  a.plast = reinterpret_cast<uint4*>(q + L.plast);
-  a.heavy = reinterpret_cast<uint32_t*>(q + L.heavy);
-  a.tasks = reinterpret_cast<uint32_t*>(q + L.tasks);
-  a.task_sum = reinterpret_cast<uint4*>(q + L.task_sum);
-  a.gT = reinterpret_cast<uint4*>(q + L.gT);
-  a.gU = reinterpret_cast<uint4*>(q + L.gU);
-  a.parts = reinterpret_cast<uint4*>(q + L.parts);
-  a.cnt = reinterpret_cast<uint32_t*>(q + L.cnt);
-  a.seg = seg;
+  const AccumPtrs a = accum_ptrs<F>(p, scratch);
+  const uint32_t B = reduce_group_size(p);
   auto bp = static_cast<const uint4*>(bases);
-
+  const unsigned grid = (unsigned)((a.nseg + 127) / 128);
   if (ev) G16_CHECK(hipEventRecord(ev[0], s));
-  if (nseg) {
-    if (p.constant)
-      hipLaunchKernelGGL((k_accumulate<F, true>), dim3((unsigned)((nseg + 127) / 128)), dim3(128),
-                         0, s, p.ent, p.total, nseg, seg, bp, a.buckets, a.pfirst, a.plast);
-    else
-      hipLaunchKernelGGL((k_accumulate<F, false>), dim3((unsigned)((nseg + 127) / 128)), dim3(128),
-                         0, s, p.ent, p.total, nseg, seg, bp, a.buckets, a.pfirst, a.plast);
+  if (p.constant) {
+    const size_t K = (size_t)msm_bucket_sets(p) * p.nb;
+    hipLaunchKernelGGL(k_fill_proj_inf<F>, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, s,
+                       a.buckets, K);
+    G16_LAUNCH_CHECK("msm buckets (constant work)");
+    if (a.nseg) {
+      hipLaunchKernelGGL(k_accumulate_const<F>, dim3(grid), dim3(128), 0, s, p.ent, p.total,
+                         a.nseg, a.seg, bp, a.buckets, a.fold0, a.frows0);
+      G16_LAUNCH_CHECK("msm accumulate (constant work)");
+    }
+  } else if (a.nseg) {
+    hipLaunchKernelGGL(k_accumulate<F>, dim3(grid), dim3(128), 0, s, p.ent, p.total, a.nseg,
+                       a.seg, bp, a.buckets, a.pfirst, a.plast);
     G16_LAUNCH_CHECK("msm accumulate");
   }
   if (ev) G16_CHECK(hipEventRecord(ev[1], s));

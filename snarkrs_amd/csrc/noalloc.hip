@@ -49,7 +49,7 @@ extern "C" g16hip_status g16hip_msm_device_noalloc(g16hip_device* d, int group, 
     set_device(d);
     char* base = static_cast<char*>(device_scratch(d));
     const int w = msm_width((uint32_t)n, c);
-    const MsmScratchSizes z = msm_scratch_sizes(group, (uint32_t)n, w);
+    const MsmScratchSizes z = msm_scratch_sizes(group, (uint32_t)n, w, d->constant_work);
     if (z.total() > G16HIP_NOALLOC_SCRATCH_BYTES)
       throw StatusError{G16HIP_DEVICE_MEMORY,
                         "an MSM of " + std::to_string(n) + " points at window width " +

@@ -193,9 +193,10 @@ struct KeyPlan {
   size_t total() const { return key_bytes + slot_bytes; }
 };
 
-static size_t slot_bytes_for(uint32_t nv, uint32_t n, uint32_t nh, int cw, int ch, bool merged);
+static size_t slot_bytes_for(uint32_t nv, uint32_t n, uint32_t nh, int cw, int ch, bool merged,
+                             bool constant);
 
-static KeyPlan plan_key(const g16hip_key* k, bool merged) {
+static KeyPlan plan_key(const g16hip_key* k, bool merged, bool constant) {
   const size_t n = k->domain_size, nv = k->n_vars, nh = k->h_basis == 1 ? n - 1 : n;
   KeyPlan P;
   P.merged = merged;
@@ -222,7 +223,8 @@ static KeyPlan plan_key(const g16hip_key* k, bool merged) {
   b += 2 * dev_pages(32 * (n > 1 ? n - 1 : 1)) + dev_pages(32 * n) + dev_pages(256) +
        (k->h_basis == 1 ? dev_pages(32 * n) : 0) + dev_pages(32 * (n / 2 ? n / 2 : 1));
   P.key_bytes = b;
-  P.slot_bytes = slot_bytes_for((uint32_t)nv, (uint32_t)n, (uint32_t)nh, P.cw, P.ch, merged);
+  P.slot_bytes = slot_bytes_for((uint32_t)nv, (uint32_t)n, (uint32_t)nh, P.cw, P.ch, merged,
+                                constant);
   return P;
 }
 
@@ -233,10 +235,10 @@ static KeyPlan choose_plan(const g16hip_device* d, const g16hip_key* k) {
   const size_t budget = device_budget(d);
   const char* wt = std::getenv("G16HIP_WINDOW_TABLES");
   if (!(wt && wt[0] == '0')) {
-    KeyPlan P = plan_key(k, true);
+    KeyPlan P = plan_key(k, true, d->constant_work);
     if (P.total() + (8ull << 30) <= budget) return P;
   }
-  KeyPlan P = plan_key(k, false);
+  KeyPlan P = plan_key(k, false, d->constant_work);
   if (P.total() > budget)
     throw too_big(P.total(), budget, "this key's resident data and one proof's stage scratch");
   return P;
@@ -378,11 +380,12 @@ static std::unique_ptr<Slot> make_slot(const g16hip_circuit* C) {
 }
 
 // Device bytes of one slot (what size_slot allocates), rounded up to pages.
-static size_t slot_bytes_for(uint32_t nv, uint32_t n, uint32_t nh, int cw, int ch, bool merged) {
+static size_t slot_bytes_for(uint32_t nv, uint32_t n, uint32_t nh, int cw, int ch, bool merged,
+                             bool constant) {
   MsmPlan pw, ph;
   static char anchor;  // the plans are bound only to size their pieces
-  msm_plan_bind(pw, &anchor, nv, cw, merged);
-  msm_plan_bind(ph, &anchor, nh, ch, merged);
+  msm_plan_bind(pw, &anchor, nv, cw, merged, constant);
+  msm_plan_bind(ph, &anchor, nh, ch, merged, constant);
   return dev_pages(32 * (size_t)nv) + dev_pages(32 * 3 * (size_t)n) + dev_pages(32 * (size_t)n) +
          dev_pages(msm_sort_scratch_bytes(nv, cw)) + dev_pages(msm_sort_scratch_bytes(nh, ch)) +
          dev_pages(msm_accum_scratch_bytes(pw, 1)) + dev_pages(msm_accum_scratch_bytes(pw, 2)) +
@@ -390,7 +393,7 @@ static size_t slot_bytes_for(uint32_t nv, uint32_t n, uint32_t nh, int cw, int c
 }
 
 static size_t slot_bytes_estimate(const g16hip_circuit* C) {
-  return slot_bytes_for(C->n_vars, C->domain, C->n_h, C->cw, C->ch, C->merged);
+  return slot_bytes_for(C->n_vars, C->domain, C->n_h, C->cw, C->ch, C->merged, C->constant);
 }
 
 static bool try_add_slot(g16hip_circuit* C, size_t reserve);
@@ -883,7 +886,11 @@ static bool replay_graphs() {
 // batched proving: waves
 // ---------------------------------------------------------------------------------------
 // Waves need the gather path and snarkjs' H basis; other keys prove item by item on a slot.
-static bool can_wave(const g16hip_circuit* C) { return C->has_matrices && C->h_basis == 0; }
+// Waves take gather-path keys with snarkjs' H basis and variable-work MSMs (a constant-work
+// circuit proves on slots: its accumulation is not in the persistent queue).
+static bool can_wave(const g16hip_circuit* C) {
+  return C->has_matrices && C->h_basis == 0 && !C->constant;
+}
 
 struct WavePlans {
   MsmPlan w, h;      // bound to the wave's scratch for m proofs
@@ -1298,7 +1305,7 @@ g16hip_status g16hip_key_resident_bytes(const g16hip_key* key, int window_tables
   return guarded([&] {
     if (!bytes) throw ArgError{"null argument"};
     validate_key(key);
-    *bytes = plan_key(key, window_tables != 0).total();
+    *bytes = plan_key(key, window_tables != 0, false).total();
     return G16HIP_OK;
   });
 }

@@ -120,10 +120,10 @@ NttTables& ntt_tables_for(g16hip_device* d, int L) {
 
 static size_t round256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-MsmScratchSizes msm_scratch_sizes(int group, uint32_t n, int c) {
+MsmScratchSizes msm_scratch_sizes(int group, uint32_t n, int c, bool constant) {
   MsmPlan plan;
   static char anchor;  // the plan is bound only to size its pieces
-  msm_plan_bind(plan, &anchor, n, c);
+  msm_plan_bind(plan, &anchor, n, c, false, constant);
   const size_t pt = group == 1 ? 128 : 256;  // XYZZ point
   MsmScratchSizes z;
   z.sort = round256(msm_sort_scratch_bytes(n, c));
@@ -177,7 +177,7 @@ template <class F>
 static Xyzz<F> run_msm(g16hip_device* d, int group, const void* bases, const void* scalars,
                        uint32_t n, int c) {
   c = msm_width(n, c);
-  const MsmScratchSizes z = msm_scratch_sizes(group, n, c);
+  const MsmScratchSizes z = msm_scratch_sizes(group, n, c, d->constant_work);
   d->scratch_d.alloc(z.sort);
   d->scratch_e.alloc(z.accum);
   d->msm_bases.alloc(z.bases);
@@ -195,12 +195,13 @@ void msm_device_on_scratch(g16hip_device* d, int group, const void* bases, const
 
 
 // Device bytes one standalone MSM of n points at width c takes: the uploaded bases and
-// scalars, the bases in the MSM form, the sort scratch and the accumulation scratch.
-static size_t msm_run_bytes(int group, uint32_t n, int c) {
+// scalars, the bases in the MSM form, the sort scratch and the accumulation scratch (with
+// the fold's slots under constant work).
+static size_t msm_run_bytes(int group, uint32_t n, int c, bool constant) {
   size_t pw = group == 1 ? 64 : 128;
   MsmPlan plan;
   static char anchor;  // the plan is bound only to size its pieces
-  msm_plan_bind(plan, &anchor, n, c);
+  msm_plan_bind(plan, &anchor, n, c, false, constant);
   return 2 * pw * (size_t)n + 32 * (size_t)n + msm_sort_scratch_bytes(n, c) +
          msm_accum_scratch_bytes(plan, group) + 256 * msm_out_points(plan);
 }
@@ -268,7 +269,7 @@ static Xyzz<F> ceremony_run(g16hip_ceremony* cer, int group, const uint8_t* base
                             const uint8_t* scalars, size_t n) {
   g16hip_device* d = cer->dev;
   if (cer->vram_limit) {
-    size_t need = msm_run_bytes(group, (uint32_t)n, msm_default_c((uint32_t)n));
+    size_t need = msm_run_bytes(group, (uint32_t)n, msm_default_c((uint32_t)n), d->constant_work);
     if (need > cer->vram_limit)
       throw StatusError{G16HIP_DEVICE_MEMORY,
                         "a run of " + std::to_string(n) + " points needs " + std::to_string(need) +
@@ -374,7 +375,7 @@ g16hip_status g16hip_msm_scratch_bytes(int group, size_t n, size_t* bytes) {
   return guarded([&] {
     if (!bytes || (group != 1 && group != 2)) throw ArgError{"bad argument"};
     if (n >= (1u << 31)) throw ArgError{"MSM larger than 2^31 points"};
-    *bytes = n ? msm_run_bytes(group, (uint32_t)n, msm_default_c((uint32_t)n)) : 0;
+    *bytes = n ? msm_run_bytes(group, (uint32_t)n, msm_default_c((uint32_t)n), false) : 0;
     return G16HIP_OK;
   });
 }

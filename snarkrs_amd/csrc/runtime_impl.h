@@ -457,7 +457,7 @@ struct MsmScratchSizes {
   size_t sort = 0, accum = 0, bases = 0;
   size_t total() const { return sort + accum + bases; }
 };
-MsmScratchSizes msm_scratch_sizes(int group, uint32_t n, int c);
+MsmScratchSizes msm_scratch_sizes(int group, uint32_t n, int c, bool constant);
 // c, or the library's width for n when c <= 0; throws outside [2, 16].
 int msm_width(uint32_t n, int c);
 // One device MSM (group 1 or 2) on caller-provided scratch, result affine in the wire

@@ -2,11 +2,14 @@
 constant_work_device_accounting cases, crates/cuda/src/msm.rs:6000-6033): every scalar emits in
 every window, so the entry count is n * windows whatever the scalars (all zero, all one,
 alternating, sparse -1, random), and every MSM and proof is bit-identical to the default
-path and to the oracle."""
+path and to the oracle. The cases cover the complete formulas' exceptional inputs (every
+scalar 1: one bucket spanning every segment, folded over several levels; alternating: the
+identity and repeated points) and a batch of constant-work proofs (proved on slots)."""
 import numpy as np
 import pytest
 
 from oracle.oracle import Synth
+from snarkrs_amd import native
 from snarkrs_amd.wire import R_MOD, SplitMix64, fr_array
 
 pytestmark = pytest.mark.gpu
@@ -66,3 +69,15 @@ def test_constant_work_proofs_match_the_oracle(dev, orc, constant):
     msm_o, proof_o = orc.prove(key, wit, 0x1234567, 0x7654321)
     assert (circ.h_and_msms(wit) == msm_o).all()
     assert (circ.prove(wit, r, s) == proof_o).all()
+
+
+def test_constant_work_batch_matches_single_proofs(dev, orc, constant):
+    syn = Synth(orc, 2000, n_public=2, n_private_in=20, bool_permille=400,
+                input_bool_permille=400, seed=83, setup_seed=84)
+    key, wit = syn.key()
+    circ = constant.prepare(key)
+    rs = [(fr_array([0x1000 + k]), fr_array([0x2000 + k])) for k in range(3)]
+    proofs = native.prove_batch([(circ, wit, r, s) for r, s in rs])
+    for (r, s), p in zip(rs, proofs):
+        assert (p == circ.prove_with_blinders(wit, r, s)).all()
+
