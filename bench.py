@@ -40,10 +40,10 @@ PEAK_MUL32_PER_S = 256 * 4 * 16 * 2.4e9
 # Algorithmic work per bucket addition (mixed add, madd-2008-s): 10 Fq products (G1),
 # 28 (G2: 3 Fq products per Fq2 product); one Fq product = 136 32x32 partial products (CIOS).
 MUL32_PER_MADD = {1: 10 * 136, 2: 28 * 136}
-# HBM bytes per k_accumulate<Fm, false> launch in the 1.1M-constraint prove (FETCH_SIZE + WRITE_SIZE,
+# HBM bytes per k_accumulate<Fm> launch in the 1.1M-constraint prove (FETCH_SIZE + WRITE_SIZE,
 # scaled by the counters' reading of a 256 MiB device copy: fetch x2, write x1), from the
 # rocprofv3 --pmc runs in profiles/k_accumulate_traffic_1_1M.txt.
-ACCUMULATE_TRAFFIC_BYTES_1_1M = 3.50e9
+ACCUMULATE_TRAFFIC_BYTES_1_1M = 3.61e9
 # Reference, CUDA back end on a Tesla T4: single prove, synthetic key, domain 2^21, warm:
 # 1263 ms (bench/results/cuda-size-sweep-2026-11-10.md:26).
 REFERENCE_T4_PROVE_MS_2_21 = 1263.0
@@ -489,7 +489,7 @@ def main() -> int:
     launches, entries, acc_ms = ks["launches"][0], ks["entries"][0], ks["accumulate_ms"][0]
     achieved = entries * MUL32_PER_MADD[1] / (acc_ms / 1e3) if acc_ms else 0.0
     roofline = {
-        "kernel": "k_accumulate<Fm, false> (G1 bucket accumulation)",
+        "kernel": "k_accumulate<Fm> (G1 bucket accumulation)",
         "bound": "valu",
         "achieved": achieved / 1e12,
         "peak": PEAK_MUL32_PER_S / 1e12,
