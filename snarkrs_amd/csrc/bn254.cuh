@@ -866,6 +866,171 @@ HD Aff<F> xyzz_to_aff(const Xyzz<F>& p) {
   return {mul(p.X, iz2), mul(p.Y, izzz)};
 }
 
+// ---------------------------------------------------------------------------------------
+// G1 bucket sums in limbs
+// ---------------------------------------------------------------------------------------
+// The G1 bucket loop keeps its running XYZZ sum as nine 29-bit limbs per coordinate and does
+// the mixed addition without leaving them: no word/limb conversion per product, no carry
+// chains. A difference a - b is taken limb by limb as a + k q - b, with k q written in raised
+// limbs (every limb below the top one increased by b' 2^29 - b', the top one lowered by b',
+// the value unchanged) so that no limb goes negative, then carried once, all limbs in
+// parallel (norm29). Values are bounded rather than reduced:
+//   products (mont29_limbs, redc29): exact limbs, below 2q;
+//   X < 9q, limbs below 2^29 + 8 (X3 = R^2 + 7q - PPP - 2Q); Y, ZZ, ZZZ < 2q exact.
+// Each product stays inside mont29_limbs' bound (operands' product below 2^261 q): the largest
+// is P^2 with P = U2 + 10q - X1 < 12q, 144 q^2 < 2^261 q. Every limb sum stays in [0, 2^32).
+// Leaving the loop, X is reduced below 2q by a product with one (g1l_out).
+struct G1L {
+  uint32_t X[9], Y[9], ZZ[9], ZZZ[9];
+};
+
+// k q in raised limbs (the comments give k and the raise b').
+struct Kq3 {  // 3q, b' = 2: limbs >= 2^30 - 2
+  static HD uint32_t k(int j) {
+    constexpr uint32_t K[9] = {0x4976f7d5u, 0x430d2222u, 0x5557e9ebu, 0x487f6870u, 0x40918c66u,
+                               0x4891c240u, 0x41f4a3c1u, 0x4b14729au, 0x00912ce9u};
+    return K[j];
+  }
+};
+struct Kq7 {  // 7q, b' = 4: limbs >= 2^31 - 4, above a sum of three exact limbs
+  static HD uint32_t k(int j) {
+    constexpr uint32_t K[9] = {0x8b6aecf1u, 0x871ea4fbu, 0x87227725u, 0x93d3f3b2u, 0x96a8f244u,
+                               0x93fec540u, 0x849028c3u, 0x84850b68u, 0x0152be21u};
+    return K[j];
+  }
+};
+struct Kq10 {  // 10q, b' = 2
+  static HD uint32_t k(int j) {
+    constexpr uint32_t K[9] = {0x54e1e4c6u, 0x4a2bc721u, 0x5c7a6114u, 0x5c535c26u, 0x573a7eaeu,
+                               0x5c908784u, 0x4684cc88u, 0x4f997e06u, 0x01e3eb0eu};
+    return K[j];
+  }
+};
+// 2^261 mod q (the MSM form's one) in limbs.
+HD uint32_t one29(int j) {
+  constexpr uint32_t O[9] = {0x157ccc21u, 0x141c2758u, 0x185230d3u, 0x014c0419u, 0x0aa36fb9u,
+                             0x1d4240ceu, 0x11d54c07u, 0x052ac7a8u, 0x000dc836u};
+  return O[j];
+}
+
+// Carries every limb's bits above 29 into the next limb, all at once: the value is unchanged
+// (below 2^261, so the top limb has nothing to carry out) and limbs below 2^32 end below
+// 2^29 + 8.
+HD void norm29(uint32_t x[9]) {
+  constexpr uint32_t MASK = (1u << 29) - 1u;
+  uint32_t c[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) c[j] = x[j] >> 29;
+  x[0] &= MASK;
+#pragma unroll
+  for (int j = 1; j < 8; ++j) x[j] = (x[j] & MASK) + c[j - 1];
+  x[8] += c[7];
+}
+
+// r = a + K - b limb by limb, then carried.
+template <class K>
+HD void sub29(const uint32_t a[9], const uint32_t b[9], uint32_t r[9]) {
+#pragma unroll
+  for (int j = 0; j < 9; ++j) r[j] = a[j] + K::k(j) - b[j];
+  norm29(r);
+}
+
+// Exact limbs of a value below 2q that is 0 mod q: 0 or q.
+HD bool zero29(const uint32_t a[9]) {
+  uint32_t z = 0, q = 0;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    z |= a[j];
+    q |= a[j] ^ Fq29::p29(j);
+  }
+  return z == 0 || q == 0;
+}
+
+HD G1L g1l_from(const Xyzz<Fm>& p) {
+  G1L r;
+  limbs29_of(p.X.v, r.X);
+  limbs29_of(p.Y.v, r.Y);
+  limbs29_of(p.ZZ.v, r.ZZ);
+  limbs29_of(p.ZZZ.v, r.ZZZ);
+  return r;
+}
+
+HD G1L g1l_inf() {
+  G1L r;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) r.X[j] = r.Y[j] = r.ZZ[j] = r.ZZZ[j] = 0;
+  return r;
+}
+
+HD Xyzz<Fm> g1l_out(const G1L& p) {
+  uint32_t one[9], X[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) one[j] = one29(j);
+  mont29_limbs<Fq29>(p.X, one, X);  // X mod q, below 2q
+  Xyzz<Fm> r;
+  pack29(X, r.X.v);
+  pack29(p.Y, r.Y.v);
+  pack29(p.ZZ, r.ZZ.v);
+  pack29(p.ZZZ, r.ZZZ.v);
+  return r;
+}
+
+// madd-2008-s, p + (x, negate ? -y : y), with the doubling and cancellation cases resolved
+// (the same result as xyzz_add_aff up to the representation). P = 0 mod q is read off
+// PP = P^2 (below 2q, so 0 or q), R = 0 off R^2.
+HD G1L g1l_madd(const G1L& p, const Aff<Fm>& a, bool negate) {
+  if (aff_is_inf(a)) return p;
+  if (zero29(p.ZZ)) {
+    G1L r = g1l_from({a.x, negate ? neg(a.y) : a.y, f_one<Fm>(), f_one<Fm>()});
+    return r;
+  }
+  uint32_t x[9], y[9], P[9], R[9];
+  limbs29_of(a.x.v, x);
+  limbs29_of(a.y.v, y);
+#pragma unroll
+  for (int j = 0; j < 9; ++j) y[j] = negate ? Kq3::k(j) - y[j] : y[j];  // 3q - y, below 3q
+  mont29_limbs<Fq29>(x, p.ZZ, P);  // U2
+  sub29<Kq10>(P, p.X, P);          // P = U2 - X1 + 10q < 12q
+  SB();
+  mont29_limbs<Fq29>(y, p.ZZZ, R);  // S2
+  sub29<Kq3>(R, p.Y, R);            // R = S2 - Y1 + 3q < 5q
+  SB();
+  uint32_t PP[9];
+  mont29_limbs<Fq29>(P, P, PP);
+  if (zero29(PP)) {
+    uint32_t R2[9];
+    mont29_limbs<Fq29>(R, R, R2);
+    if (zero29(R2)) return g1l_from(xyzz_dbl_aff(negate ? aff_neg(a) : a));
+    return g1l_inf();
+  }
+  SB();
+  uint32_t PPP[9];
+  mont29_limbs<Fq29>(P, PP, PPP);
+  SB();
+  G1L r;
+  mont29_limbs<Fq29>(p.ZZ, PP, r.ZZ);
+  SB();
+  uint32_t Q[9];
+  mont29_limbs<Fq29>(p.X, PP, Q);
+  SB();
+  mont29_limbs<Fq29>(p.ZZZ, PPP, r.ZZZ);
+  SB();
+  mont29_limbs<Fq29>(R, R, r.X);
+#pragma unroll
+  for (int j = 0; j < 9; ++j) r.X[j] = r.X[j] + Kq7::k(j) - PPP[j] - Q[j] - Q[j];
+  norm29(r.X);  // X3 = R^2 - PPP - 2Q + 7q < 9q
+  SB();
+  sub29<Kq10>(Q, r.X, Q);  // Q - X3 + 10q < 12q
+  // Y3 = R (Q - X3) - Y1 PPP as one wide value (|.| < 60 q^2) and one reduction.
+  uint64_t t[17];
+#pragma unroll
+  for (int k = 0; k < 17; ++k) t[k] = 0;
+  wide_mac(t, R, Q);
+  wide_msub(t, p.Y, PPP);
+  redc29<Fq29>(t, r.Y);
+  return r;
+}
+
 // k * a for a plain (standard-form) 256-bit scalar, double-and-add from the top bit.
 template <class F>
 HD Xyzz<F> xyzz_mul_plain(const Aff<F>& a, const uint32_t k[8]) {

@@ -153,104 +153,27 @@ __global__ void __launch_bounds__(128) k_window_tables(uint4* table, size_t n, i
 // ---------------------------------------------------------------------------------------
 // bucket accumulation
 // ---------------------------------------------------------------------------------------
-// The bucket loop's running sum. Generic: an XYZZ point. G1 (Fm): X and Y as words, ZZ and
-// ZZZ as nine 29-bit limbs. ZZ and ZZZ only ever feed 29-bit products, so keeping them (and
-// PP, PPP inside the addition) in limbs skips the word/limb round trip each product would
-// make; the value and every output are the same.
+// The bucket loop's running sum. Generic: an XYZZ point. G1 (Fm): the limb form G1L, whose
+// mixed addition never leaves 29-bit limbs (bn254.cuh); the value and every output are the
+// same.
 template <class F>
 struct Acc {
   using T = Xyzz<F>;
   static __device__ __forceinline__ T inf() { return xyzz_inf<F>(); }
-  static __device__ __forceinline__ T madd(const T& p, const Aff<F>& a) { return xyzz_add_aff(p, a); }
+  static __device__ __forceinline__ T madd(const T& p, const Aff<F>& a, bool negate) {
+    return xyzz_add_aff(p, negate ? aff_neg(a) : a);
+  }
   static __device__ __forceinline__ Xyzz<F> out(const T& p) { return p; }
-};
-
-struct AccFm {
-  Fm X, Y;
-  uint32_t ZZ[9], ZZZ[9];
 };
 
 template <>
 struct Acc<Fm> {
-  using T = AccFm;
-  static __device__ __forceinline__ T from(const Xyzz<Fm>& p) {
-    T r;
-    r.X = p.X;
-    r.Y = p.Y;
-    limbs29_of(p.ZZ.v, r.ZZ);
-    limbs29_of(p.ZZZ.v, r.ZZZ);
-    return r;
+  using T = G1L;
+  static __device__ __forceinline__ T inf() { return g1l_inf(); }
+  static __device__ __forceinline__ T madd(const T& p, const Aff<Fm>& a, bool negate) {
+    return g1l_madd(p, a, negate);
   }
-  static __device__ __forceinline__ T inf() { return from(xyzz_inf<Fm>()); }
-  static __device__ __forceinline__ Xyzz<Fm> out(const T& p) {
-    Xyzz<Fm> r;
-    r.X = p.X;
-    r.Y = p.Y;
-    pack29(p.ZZ, r.ZZ.v);
-    pack29(p.ZZZ, r.ZZZ.v);
-    return r;
-  }
-  // The MSM form's zero is 0 or q.
-  static __device__ __forceinline__ bool zero29(const uint32_t a[9]) {
-    uint32_t z = 0, q = 0;
-#pragma unroll
-    for (int j = 0; j < 9; ++j) {
-      z |= a[j];
-      q |= a[j] ^ Fq29::p29(j);
-    }
-    return z == 0 || q == 0;
-  }
-  static __device__ __forceinline__ Fm prod(const uint32_t A[9], const uint32_t B[9]) {
-    uint32_t R[9];
-    mont29_limbs<Fq29>(A, B, R);
-    Fm r;
-    pack29(R, r.v);
-    return r;
-  }
-  // madd-2008-s as xyzz_add_aff, on the mixed form.
-  static __device__ __forceinline__ T madd(const T& p, const Aff<Fm>& a) {
-    if (aff_is_inf(a)) return p;
-    if (zero29(p.ZZ)) return from({a.x, a.y, fp_one<FqmCfg>(), fp_one<FqmCfg>()});
-    uint32_t L0[9], L1[9];
-    limbs29_of(a.x.v, L0);
-    const Fm P = sub(prod(L0, p.ZZ), p.X);
-    SB();
-    limbs29_of(a.y.v, L1);
-    const Fm R = sub(prod(L1, p.ZZZ), p.Y);
-    SB();
-    if (is_zero(P)) return from(is_zero(R) ? xyzz_dbl_aff(a) : xyzz_inf<Fm>());
-    uint32_t P29[9], PP[9], PPP[9];
-    limbs29_of(P.v, P29);
-    mont29_limbs<Fq29>(P29, P29, PP);
-    SB();
-    mont29_limbs<Fq29>(P29, PP, PPP);
-    SB();
-    T r;
-    mont29_limbs<Fq29>(p.ZZ, PP, r.ZZ);
-    SB();
-    limbs29_of(p.X.v, L0);
-    const Fm Q = prod(L0, PP);
-    SB();
-    mont29_limbs<Fq29>(p.ZZZ, PPP, r.ZZZ);
-    SB();
-    Fm PPPw;
-    pack29(PPP, PPPw.v);
-    limbs29_of(R.v, L0);
-    r.X = sub(sub(prod(L0, L0), PPPw), dbl(Q));
-    SB();
-    // Y3 = R (Q - X3) - Y1 PPP as one wide value and one reduction (|Y3 wide| < 4 q^2).
-    uint64_t t[17];
-#pragma unroll
-    for (int k = 0; k < 17; ++k) t[k] = 0;
-    limbs29_of(sub(Q, r.X).v, L1);
-    wide_mac(t, L0, L1);
-    limbs29_of(p.Y.v, L1);
-    wide_msub(t, L1, PPP);
-    uint32_t Y[9];
-    redc29<Fq29>(t, Y);
-    pack29(Y, r.Y.v);
-    return r;
-  }
+  static __device__ __forceinline__ Xyzz<Fm> out(const T& p) { return g1l_out(p); }
 };
 
 // Segment t of the sorted entries: its runs added into XYZZ accumulators, a run whole inside
@@ -292,7 +215,7 @@ __device__ __forceinline__ void accumulate_segment(size_t t, const uint2* __rest
       v_next = ent[k + 1].y;
       a_next = ld_aff<F>(bases, v_next >> 1);
     }
-    acc = A::madd(acc, (v & 1) ? aff_neg(a) : a);
+    acc = A::madd(acc, a, v & 1);
     v = v_next;
     a = a_next;
   }
