@@ -981,7 +981,11 @@ HD Xyzz<Fm> g1l_out(const G1L& p) {
 HD G1L g1l_madd(const G1L& p, const Aff<Fm>& a, bool negate) {
   if (aff_is_inf(a)) return p;
   if (zero29(p.ZZ)) {
-    G1L r = g1l_from({a.x, negate ? neg(a.y) : a.y, f_one<Fm>(), f_one<Fm>()});
+    G1L r;
+    limbs29_of(a.x.v, r.X);
+    limbs29_of((negate ? neg(a.y) : a.y).v, r.Y);
+#pragma unroll
+    for (int j = 0; j < 9; ++j) r.ZZ[j] = r.ZZZ[j] = one29(j);
     return r;
   }
   uint32_t x[9], y[9], P[9], R[9];
@@ -1028,6 +1032,248 @@ HD G1L g1l_madd(const G1L& p, const Aff<Fm>& a, bool negate) {
   wide_mac(t, R, Q);
   wide_msub(t, p.Y, PPP);
   redc29<Fq29>(t, r.Y);
+  return r;
+}
+
+// ---------------------------------------------------------------------------------------
+// G2 bucket sums in limbs
+// ---------------------------------------------------------------------------------------
+// The same for G2 over Fq2 = Fq[u]/(u^2 + 1), each coordinate two limb vectors. An Fq2 product
+// is the schoolbook one, c0 = a0 b0 - a1 b1 and c1 = a0 b1 + a1 b0, accumulated wide with
+// signed multiply-adds and reduced once per component: the fourth product costs less than the
+// 64-bit column subtractions (and their carry hazards) Karatsuba would need. A reduction takes
+// a wide value below 2^261 q in magnitude: with components below A q and B q, |c0| < A B q^2
+// and c1 < 2 A B q^2, so 2 A B must stay below 2^261 / q = 169.3. Bounds:
+//   X < 5q, exact limbs (X3 is formed below 9q and 4q subtracted when it is at least 4q);
+//   Y, ZZ, ZZZ < 2q exact; P = U2 + 6q - X1 < 8q; R = +-S2 + 5q - Y1 < 7q;
+//   T = Q + 6q - X3 < 8q; the largest product is P^2 (c1 = 2 p0 p1 < 128 q^2), and
+//   Y3 = R T - Y1 PPP has |c1| < 112 q^2.
+// Every column holds at most 18 limb products on each side of its sign, as redc29 requires.
+struct F2L {
+  uint32_t c0[9], c1[9];
+};
+struct G2L {
+  F2L X, Y, ZZ, ZZZ;
+};
+
+struct Kq5 {  // 5q, b' = 4: limbs >= 2^31 - 4, above a sum of two exact limbs
+  static HD uint32_t k(int j) {
+    constexpr uint32_t K[9] = {0x9a70f263u, 0x8515e38du, 0x8e3d3087u, 0x8e29ae10u, 0x8b9d3f54u,
+                               0x8e4843bfu, 0x83426641u, 0x87ccbf00u, 0x00f1f584u};
+    return K[j];
+  }
+};
+struct Kq6 {  // 6q, b' = 2
+  static HD uint32_t k(int j) {
+    constexpr uint32_t K[9] = {0x52edefaau, 0x461a4446u, 0x4aafd3d8u, 0x50fed0e3u, 0x412318ceu,
+                               0x51238482u, 0x43e94784u, 0x5628e536u, 0x012259d4u};
+    return K[j];
+  }
+};
+HD uint32_t q4_29(int j) {  // 4q, exact limbs
+  constexpr uint32_t K[9] = {0x01f3f51cu, 0x041182dbu, 0x11ca8d3cu, 0x0b548b43u, 0x161765e0u,
+                             0x0b6d0302u, 0x029b8504u, 0x197098d0u, 0x00c19139u};
+  return K[j];
+}
+
+// A value below 9q in limbs below 2^32, to exact limbs and below 5q.
+HD void below5q(uint32_t x[9]) {
+  constexpr uint32_t MASK = (1u << 29) - 1u;
+  uint32_t c = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const uint32_t v = x[j] + c;
+    x[j] = v & MASK;
+    c = v >> 29;
+  }
+  x[8] += c;
+  uint32_t d[9];
+  int32_t b = 0;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    const int32_t v = (int32_t)x[j] - (int32_t)q4_29(j) + b;
+    d[j] = (uint32_t)v & MASK;
+    b = v >> 29;  // 0 or -1
+  }
+#pragma unroll
+  for (int j = 0; j < 9; ++j) x[j] = b < 0 ? x[j] : d[j];
+}
+
+HD void wide_zero(uint64_t t[17]) {
+#pragma unroll
+  for (int k = 0; k < 17; ++k) t[k] = 0;
+}
+
+// r = a b (r must not alias a or b).
+HD void f2l_mul(const F2L& a, const F2L& b, F2L& r) {
+  uint64_t t[17];
+  wide_zero(t);
+  wide_mac(t, a.c0, b.c0);
+  wide_msub(t, a.c1, b.c1);
+  redc29<Fq29>(t, r.c0);
+  wide_zero(t);
+  wide_mac(t, a.c0, b.c1);
+  wide_mac(t, a.c1, b.c0);
+  redc29<Fq29, false>(t, r.c1);
+}
+
+// r = a^2 = (a0 + a1)(a0 - a1) + 2 a0 a1 u (r must not alias a).
+HD void f2l_sqr(const F2L& a, F2L& r) {
+  uint32_t s[9];
+  int32_t d[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    s[j] = a.c0[j] + a.c1[j];
+    d[j] = (int32_t)a.c0[j] - (int32_t)a.c1[j];
+  }
+  uint64_t t[17];
+  wide_zero(t);
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+#pragma unroll
+    for (int j = 0; j < 9; ++j) t[i + j] += (uint64_t)((int64_t)(int32_t)s[i] * (int64_t)d[j]);
+  redc29<Fq29>(t, r.c0);
+#pragma unroll
+  for (int j = 0; j < 9; ++j) s[j] = a.c0[j] + a.c0[j];
+  wide_zero(t);
+  wide_mac(t, s, a.c1);
+  redc29<Fq29, false>(t, r.c1);
+}
+
+HD bool f2l_zero(const F2L& a) { return zero29(a.c0) && zero29(a.c1); }
+
+HD void f2l_of(const Fm2i& a, F2L& r) {
+  limbs29_of(a.c0.v, r.c0);
+  limbs29_of(a.c1.v, r.c1);
+}
+
+HD G2L g2l_from(const Xyzz<Fm2i>& p) {
+  G2L r;
+  f2l_of(p.X, r.X);
+  f2l_of(p.Y, r.Y);
+  f2l_of(p.ZZ, r.ZZ);
+  f2l_of(p.ZZZ, r.ZZZ);
+  return r;
+}
+
+// (x, negate ? -y : y) with ZZ = ZZZ = 1.
+HD G2L g2l_from_aff(const Aff<Fm2i>& a, bool negate) {
+  G2L r;
+  f2l_of(a.x, r.X);
+  f2l_of(negate ? neg(a.y) : a.y, r.Y);
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    r.ZZ.c0[j] = r.ZZZ.c0[j] = one29(j);
+    r.ZZ.c1[j] = r.ZZZ.c1[j] = 0;
+  }
+  return r;
+}
+
+HD G2L g2l_inf() {
+  G2L r;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    r.X.c0[j] = r.X.c1[j] = r.Y.c0[j] = r.Y.c1[j] = 0;
+    r.ZZ.c0[j] = r.ZZ.c1[j] = r.ZZZ.c0[j] = r.ZZZ.c1[j] = 0;
+  }
+  return r;
+}
+
+HD Xyzz<Fm2i> g2l_out(const G2L& p) {
+  uint32_t one[9], X[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) one[j] = one29(j);
+  Xyzz<Fm2i> r;
+  mont29_limbs<Fq29>(p.X.c0, one, X);  // below 2q
+  pack29(X, r.X.c0.v);
+  mont29_limbs<Fq29>(p.X.c1, one, X);
+  pack29(X, r.X.c1.v);
+  pack29(p.Y.c0, r.Y.c0.v);
+  pack29(p.Y.c1, r.Y.c1.v);
+  pack29(p.ZZ.c0, r.ZZ.c0.v);
+  pack29(p.ZZ.c1, r.ZZ.c1.v);
+  pack29(p.ZZZ.c0, r.ZZZ.c0.v);
+  pack29(p.ZZZ.c1, r.ZZZ.c1.v);
+  return r;
+}
+
+// madd-2008-s over Fq2 for p not infinity: r = p + (x, negate ? -y : y) and 0, or 1 when the
+// sum is a doubling (p = that point, r unset) or 2 when it is infinity. `a` is dead after the
+// first two products, so a caller that can reload it keeps it out of the registers of the rest.
+HD int g2l_madd_core(const G2L& p, const Aff<Fm2i>& a, bool negate, G2L& r) {
+  F2L v, P, R;
+  f2l_of(a.x, v);
+  f2l_mul(v, p.ZZ, P);  // U2
+  sub29<Kq6>(P.c0, p.X.c0, P.c0);
+  sub29<Kq6>(P.c1, p.X.c1, P.c1);  // P = U2 - X1 + 6q < 8q
+  SB();
+  f2l_of(a.y, v);
+  f2l_mul(v, p.ZZZ, R);  // S2
+  const uint32_t m = 0u - (uint32_t)negate;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {  // R = +-S2 - Y1 + 5q < 7q
+    R.c0[j] = ((R.c0[j] ^ m) - m) + Kq5::k(j) - p.Y.c0[j];
+    R.c1[j] = ((R.c1[j] ^ m) - m) + Kq5::k(j) - p.Y.c1[j];
+  }
+  norm29(R.c0);
+  norm29(R.c1);
+  SB();
+  F2L PP;
+  f2l_sqr(P, PP);
+  if (f2l_zero(PP)) {
+    F2L R2;
+    f2l_sqr(R, R2);
+    return f2l_zero(R2) ? 1 : 2;
+  }
+  SB();
+  F2L PPP;
+  f2l_mul(P, PP, PPP);
+  SB();
+  f2l_mul(p.ZZ, PP, r.ZZ);
+  SB();
+  F2L Q;
+  f2l_mul(p.X, PP, Q);
+  SB();
+  f2l_mul(p.ZZZ, PPP, r.ZZZ);
+  SB();
+  f2l_sqr(R, r.X);
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {  // X3 = R^2 - PPP - 2Q + 7q < 9q
+    r.X.c0[j] = r.X.c0[j] + Kq7::k(j) - PPP.c0[j] - Q.c0[j] - Q.c0[j];
+    r.X.c1[j] = r.X.c1[j] + Kq7::k(j) - PPP.c1[j] - Q.c1[j] - Q.c1[j];
+  }
+  below5q(r.X.c0);
+  below5q(r.X.c1);
+  SB();
+  sub29<Kq6>(Q.c0, r.X.c0, Q.c0);
+  sub29<Kq6>(Q.c1, r.X.c1, Q.c1);  // T = Q - X3 + 6q < 8q
+  // Y3 = R T - Y1 PPP, one wide value and one reduction per component.
+  uint64_t t[17];
+  wide_zero(t);
+  wide_mac(t, R.c0, Q.c0);
+  wide_msub(t, R.c1, Q.c1);
+  wide_msub(t, p.Y.c0, PPP.c0);
+  wide_mac(t, p.Y.c1, PPP.c1);
+  redc29<Fq29>(t, r.Y.c0);
+  SB();
+  wide_zero(t);
+  wide_mac(t, R.c0, Q.c1);
+  wide_mac(t, R.c1, Q.c0);
+  wide_msub(t, p.Y.c0, PPP.c1);
+  wide_msub(t, p.Y.c1, PPP.c0);
+  redc29<Fq29>(t, r.Y.c1);
+  return 0;
+}
+
+// madd-2008-s over Fq2, p + (x, negate ? -y : y), with the doubling and cancellation cases
+// resolved (as g1l_madd).
+HD G2L g2l_madd(const G2L& p, const Aff<Fm2i>& a, bool negate) {
+  if (aff_is_inf(a)) return p;
+  if (f2l_zero(p.ZZ)) return g2l_from_aff(a, negate);
+  G2L r;
+  const int k = g2l_madd_core(p, a, negate, r);
+  if (k == 1) return g2l_from(xyzz_dbl_aff(negate ? aff_neg(a) : a));
+  if (k == 2) return g2l_inf();
   return r;
 }
 

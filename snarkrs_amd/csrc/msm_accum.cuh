@@ -153,14 +153,15 @@ __global__ void __launch_bounds__(128) k_window_tables(uint4* table, size_t n, i
 // ---------------------------------------------------------------------------------------
 // bucket accumulation
 // ---------------------------------------------------------------------------------------
-// The bucket loop's running sum. Generic: an XYZZ point. G1 (Fm): the limb form G1L, whose
-// mixed addition never leaves 29-bit limbs (bn254.cuh); the value and every output are the
-// same.
+// The bucket loop's running sum. Generic: an XYZZ point. G1 (Fm) and G2 (Fm2i): the limb
+// forms G1L and G2L, whose mixed additions never leave 29-bit limbs (bn254.cuh); the value and
+// every output are the same.
 template <class F>
 struct Acc {
   using T = Xyzz<F>;
   static __device__ __forceinline__ T inf() { return xyzz_inf<F>(); }
-  static __device__ __forceinline__ T madd(const T& p, const Aff<F>& a, bool negate) {
+  static __device__ __forceinline__ T madd(const T& p, const Aff<F>& a, bool negate,
+                                           const uint4*, uint32_t) {
     return xyzz_add_aff(p, negate ? aff_neg(a) : a);
   }
   static __device__ __forceinline__ Xyzz<F> out(const T& p) { return p; }
@@ -170,10 +171,32 @@ template <>
 struct Acc<Fm> {
   using T = G1L;
   static __device__ __forceinline__ T inf() { return g1l_inf(); }
-  static __device__ __forceinline__ T madd(const T& p, const Aff<Fm>& a, bool negate) {
+  static __device__ __forceinline__ T madd(const T& p, const Aff<Fm>& a, bool negate,
+                                           const uint4*, uint32_t) {
     return g1l_madd(p, a, negate);
   }
   static __device__ __forceinline__ Xyzz<Fm> out(const T& p) { return g1l_out(p); }
+};
+
+template <>
+struct Acc<Fm2i> {
+  using T = G2L;
+  static __device__ __forceinline__ T inf() { return g2l_inf(); }
+  // The doubling case reloads the base (entry i of `bases`) instead of keeping it live.
+  static __device__ __forceinline__ T madd(const T& p, const Aff<Fm2i>& a, bool negate,
+                                           const uint4* bases, uint32_t i) {
+    if (aff_is_inf(a)) return p;
+    if (f2l_zero(p.ZZ)) return g2l_from_aff(a, negate);
+    G2L r;
+    const int k = g2l_madd_core(p, a, negate, r);
+    if (k == 1) {
+      const Aff<Fm2i> b = ld_aff<Fm2i>(bases, i);
+      return g2l_from(xyzz_dbl_aff(negate ? aff_neg(b) : b));
+    }
+    if (k == 2) return g2l_inf();
+    return r;
+  }
+  static __device__ __forceinline__ Xyzz<Fm2i> out(const T& p) { return g2l_out(p); }
 };
 
 // Segment t of the sorted entries: its runs added into XYZZ accumulators, a run whole inside
@@ -215,7 +238,7 @@ __device__ __forceinline__ void accumulate_segment(size_t t, const uint2* __rest
       v_next = ent[k + 1].y;
       a_next = ld_aff<F>(bases, v_next >> 1);
     }
-    acc = A::madd(acc, a, v & 1);
+    acc = A::madd(acc, a, v & 1, bases, v >> 1);
     v = v_next;
     a = a_next;
   }
