@@ -1035,6 +1035,57 @@ HD G1L g1l_madd(const G1L& p, const Aff<Fm>& a, bool negate) {
   return r;
 }
 
+// add-2008-s on the limb form, p + q (both in the bounds above); the doubling case goes
+// through the word form.
+HD G1L g1l_add(const G1L& p, const G1L& q) {
+  if (zero29(p.ZZ)) return q;
+  if (zero29(q.ZZ)) return p;
+  uint32_t U1[9], P[9], S1[9], R[9];
+  mont29_limbs<Fq29>(p.X, q.ZZ, U1);
+  mont29_limbs<Fq29>(q.X, p.ZZ, P);  // U2
+  sub29<Kq3>(P, U1, P);              // P = U2 - U1 + 3q < 5q
+  SB();
+  mont29_limbs<Fq29>(p.Y, q.ZZZ, S1);
+  mont29_limbs<Fq29>(q.Y, p.ZZZ, R);  // S2
+  sub29<Kq3>(R, S1, R);               // R = S2 - S1 + 3q < 5q
+  SB();
+  uint32_t PP[9];
+  mont29_limbs<Fq29>(P, P, PP);
+  if (zero29(PP)) {
+    uint32_t R2[9];
+    mont29_limbs<Fq29>(R, R, R2);
+    if (zero29(R2)) return g1l_from(xyzz_dbl(g1l_out(p)));
+    return g1l_inf();
+  }
+  SB();
+  uint32_t PPP[9], Z[9];
+  mont29_limbs<Fq29>(P, PP, PPP);
+  SB();
+  G1L r;
+  mont29_limbs<Fq29>(p.ZZ, q.ZZ, Z);
+  mont29_limbs<Fq29>(Z, PP, r.ZZ);
+  SB();
+  mont29_limbs<Fq29>(p.ZZZ, q.ZZZ, Z);
+  mont29_limbs<Fq29>(Z, PPP, r.ZZZ);
+  SB();
+  uint32_t Q[9];
+  mont29_limbs<Fq29>(U1, PP, Q);
+  SB();
+  mont29_limbs<Fq29>(R, R, r.X);
+#pragma unroll
+  for (int j = 0; j < 9; ++j) r.X[j] = r.X[j] + Kq7::k(j) - PPP[j] - Q[j] - Q[j];
+  norm29(r.X);  // X3 = R^2 - PPP - 2Q + 7q < 9q
+  SB();
+  sub29<Kq10>(Q, r.X, Q);  // Q - X3 + 10q < 12q
+  uint64_t t[17];
+#pragma unroll
+  for (int k = 0; k < 17; ++k) t[k] = 0;
+  wide_mac(t, R, Q);
+  wide_msub(t, S1, PPP);
+  redc29<Fq29>(t, r.Y);  // Y3 = R (Q - X3) - S1 PPP, |.| < 60 q^2
+  return r;
+}
+
 // ---------------------------------------------------------------------------------------
 // G2 bucket sums in limbs
 // ---------------------------------------------------------------------------------------
@@ -1274,6 +1325,72 @@ HD G2L g2l_madd(const G2L& p, const Aff<Fm2i>& a, bool negate) {
   const int k = g2l_madd_core(p, a, negate, r);
   if (k == 1) return g2l_from(xyzz_dbl_aff(negate ? aff_neg(a) : a));
   if (k == 2) return g2l_inf();
+  return r;
+}
+
+// add-2008-s over Fq2 on the limb form, p + q; the doubling case goes through the word form.
+// Bounds as g2l_madd's with U1 < 2q: P, R < 5q, T < 8q.
+HD G2L g2l_add(const G2L& p, const G2L& q) {
+  if (f2l_zero(p.ZZ)) return q;
+  if (f2l_zero(q.ZZ)) return p;
+  F2L U1, P, S1, R;
+  f2l_mul(p.X, q.ZZ, U1);
+  f2l_mul(q.X, p.ZZ, P);  // U2
+  sub29<Kq3>(P.c0, U1.c0, P.c0);
+  sub29<Kq3>(P.c1, U1.c1, P.c1);  // P = U2 - U1 + 3q < 5q
+  SB();
+  f2l_mul(p.Y, q.ZZZ, S1);
+  f2l_mul(q.Y, p.ZZZ, R);  // S2
+  sub29<Kq3>(R.c0, S1.c0, R.c0);
+  sub29<Kq3>(R.c1, S1.c1, R.c1);  // R = S2 - S1 + 3q < 5q
+  SB();
+  F2L PP;
+  f2l_sqr(P, PP);
+  if (f2l_zero(PP)) {
+    F2L R2;
+    f2l_sqr(R, R2);
+    if (f2l_zero(R2)) return g2l_from(xyzz_dbl(g2l_out(p)));
+    return g2l_inf();
+  }
+  SB();
+  F2L PPP, Z;
+  f2l_mul(P, PP, PPP);
+  SB();
+  G2L r;
+  f2l_mul(p.ZZ, q.ZZ, Z);
+  f2l_mul(Z, PP, r.ZZ);
+  SB();
+  f2l_mul(p.ZZZ, q.ZZZ, Z);
+  f2l_mul(Z, PPP, r.ZZZ);
+  SB();
+  F2L Q;
+  f2l_mul(U1, PP, Q);
+  SB();
+  f2l_sqr(R, r.X);
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {  // X3 = R^2 - PPP - 2Q + 7q < 9q
+    r.X.c0[j] = r.X.c0[j] + Kq7::k(j) - PPP.c0[j] - Q.c0[j] - Q.c0[j];
+    r.X.c1[j] = r.X.c1[j] + Kq7::k(j) - PPP.c1[j] - Q.c1[j] - Q.c1[j];
+  }
+  below5q(r.X.c0);
+  below5q(r.X.c1);
+  SB();
+  sub29<Kq6>(Q.c0, r.X.c0, Q.c0);
+  sub29<Kq6>(Q.c1, r.X.c1, Q.c1);  // T = Q - X3 + 6q < 8q
+  uint64_t t[17];
+  wide_zero(t);
+  wide_mac(t, R.c0, Q.c0);
+  wide_msub(t, R.c1, Q.c1);
+  wide_msub(t, S1.c0, PPP.c0);
+  wide_mac(t, S1.c1, PPP.c1);
+  redc29<Fq29>(t, r.Y.c0);
+  SB();
+  wide_zero(t);
+  wide_mac(t, R.c0, Q.c1);
+  wide_mac(t, R.c1, Q.c0);
+  wide_msub(t, S1.c0, PPP.c1);
+  wide_msub(t, S1.c1, PPP.c0);
+  redc29<Fq29>(t, r.Y.c1);
   return r;
 }
 

@@ -133,6 +133,38 @@ struct POps {
   static __device__ __forceinline__ Xyzz<F> out(const P& p) { return proj_to_xyzz(p); }
 };
 
+// The tails' additions (merges, heavy buckets, reduction): XOps, or for G1 and G2 the limb
+// forms (LOps): points are loaded into limbs, added there, and packed back to the XYZZ words
+// (X reduced below 2q) when stored.
+template <class F>
+struct LOps : XOps<F> {};
+template <>
+struct LOps<Fm> {
+  using P = G1L;
+  static __device__ __forceinline__ P inf() { return g1l_inf(); }
+  static __device__ __forceinline__ P add(const P& a, const P& b) { return g1l_add(a, b); }
+  static __device__ __forceinline__ P load(const uint4* base, size_t i) {
+    return g1l_from(ld_xyzz<Fm>(base, i));
+  }
+  static __device__ __forceinline__ void store(uint4* base, size_t i, const P& p) {
+    st_xyzz(base, i, g1l_out(p));
+  }
+  static __device__ __forceinline__ Xyzz<Fm> out(const P& p) { return g1l_out(p); }
+};
+template <>
+struct LOps<Fm2i> {
+  using P = G2L;
+  static __device__ __forceinline__ P inf() { return g2l_inf(); }
+  static __device__ __forceinline__ P add(const P& a, const P& b) { return g2l_add(a, b); }
+  static __device__ __forceinline__ P load(const uint4* base, size_t i) {
+    return g2l_from(ld_xyzz<Fm2i>(base, i));
+  }
+  static __device__ __forceinline__ void store(uint4* base, size_t i, const P& p) {
+    st_xyzz(base, i, g2l_out(p));
+  }
+  static __device__ __forceinline__ Xyzz<Fm2i> out(const P& p) { return g2l_out(p); }
+};
+
 // Window tables for merged MSMs (msm_window_tables): entry w n + i = 2^(c w) P_i, affine, from
 // the previous window's point by c doublings and one inversion; infinity stays (0, 0).
 template <class F>
@@ -453,9 +485,10 @@ __global__ void __launch_bounds__(256) k_merge(const uint32_t* offset, uint32_t 
     return;
   }
   // One addition site: a G2 addition is over 100 KB of code.
-  Xyzz<F> acc = ld_xyzz<F>(plast, sa);
-  for (uint32_t t = sa + 1; t <= sb; ++t) acc = xyzz_add(acc, ld_xyzz<F>(pfirst, t));
-  st_xyzz(buckets, k, acc);
+  using O = LOps<F>;
+  typename O::P acc = O::load(plast, sa);
+  for (uint32_t t = sa + 1; t <= sb; ++t) acc = O::add(acc, O::load(pfirst, t));
+  O::store(buckets, k, acc);
 }
 
 // Merged windows: every bucket holds many entries and spans many segments (about 40 at the
@@ -475,10 +508,11 @@ __global__ void __launch_bounds__(256) k_merge_wide(const uint32_t* offset, uint
                                                     const uint4* plast, uint32_t* cnt,
                                                     uint32_t* heavy, uint32_t* tasks,
                                                     uint32_t seg) {
-  __shared__ Xyzz<F> sh[256];
+  using O = LOps<F>;
+  __shared__ typename O::P sh[256];
   const uint32_t lane = threadIdx.x % MERGE_LANES;
   const uint32_t k = blockIdx.x * (256 / MERGE_LANES) + threadIdx.x / MERGE_LANES;
-  Xyzz<F> acc = xyzz_inf<F>();
+  typename O::P acc = O::inf();
   bool store = false;  // the bucket's sum is merged here
   uint32_t s = 0, e = 0;
   if (k < K) {
@@ -504,19 +538,19 @@ __global__ void __launch_bounds__(256) k_merge_wide(const uint32_t* offset, uint
       } else if (sa != sb) {
         store = true;
         for (uint32_t t = sa + lane; t <= sb; t += MERGE_LANES)
-          acc = xyzz_add(acc, t == sa ? ld_xyzz<F>(plast, t) : ld_xyzz<F>(pfirst, t));
+          acc = O::add(acc, O::load(t == sa ? plast : pfirst, t));
       }
     }
   }
   sh[threadIdx.x] = acc;
   for (int off = MERGE_LANES / 2; off > 0; off >>= 1) {
     __syncthreads();
-    if (lane < (uint32_t)off) sh[threadIdx.x] = xyzz_add(sh[threadIdx.x], sh[threadIdx.x + off]);
+    if (lane < (uint32_t)off) sh[threadIdx.x] = O::add(sh[threadIdx.x], sh[threadIdx.x + off]);
   }
   __syncthreads();
   if (k < K && lane == 0) {
     if (s == e) st_xyzz(buckets, k, xyzz_inf<F>());
-    else if (store) st_xyzz(buckets, k, sh[threadIdx.x]);
+    else if (store) O::store(buckets, k, sh[threadIdx.x]);
   }
 }
 
@@ -535,14 +569,15 @@ template <class F, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_heavy_tasks(const uint32_t* cnt, const uint32_t* tasks,
                                                        const uint4* pfirst, uint4* task_sum) {
   static_assert(BLOCK == HEAVY_TASK, "one segment partial per thread");
-  __shared__ Xyzz<F> sh[BLOCK];
+  using O = LOps<F>;
+  __shared__ typename O::P sh[BLOCK];
   uint32_t nt = cnt[1];
   for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
     uint32_t lo = tasks[2 * t], hi = tasks[2 * t + 1];
     uint32_t seg = lo + threadIdx.x;
-    sh[threadIdx.x] = seg < hi ? ld_xyzz<F>(pfirst, seg) : xyzz_inf<F>();
-    block_tree<XOps<F>, BLOCK>(sh);
-    if (threadIdx.x == 0) st_xyzz(task_sum, t, sh[0]);
+    sh[threadIdx.x] = seg < hi ? O::load(pfirst, seg) : O::inf();
+    block_tree<O, BLOCK>(sh);
+    if (threadIdx.x == 0) O::store(task_sum, t, sh[0]);
     __syncthreads();
   }
 }
@@ -556,20 +591,21 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_finish(const uint32_t* offset,
                                                         const uint4* task_sum, uint4* buckets,
                                                         const uint4* pfirst, const uint4* plast,
                                                         uint32_t seg) {
-  __shared__ Xyzz<F> sh[BLOCK];
+  using O = LOps<F>;
+  __shared__ typename O::P sh[BLOCK];
   uint32_t nh = cnt[0];
   for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
     uint32_t k = heavy[3 * h], tb = heavy[3 * h + 1], nt = heavy[3 * h + 2];
     uint32_t sa = offset[k] / seg, sb = (offset[k + 1] - 1) / seg;
-    Xyzz<F> acc = xyzz_inf<F>();
+    typename O::P acc = O::inf();
     for (uint32_t t = threadIdx.x; t < nt + 2; t += BLOCK) {
       const uint4* src = t < nt ? task_sum : (t == nt ? plast : pfirst);
       const size_t i = t < nt ? (size_t)tb + t : (t == nt ? sa : sb);
-      acc = xyzz_add(acc, ld_xyzz<F>(src, i));
+      acc = O::add(acc, O::load(src, i));
     }
     sh[threadIdx.x] = acc;
-    block_tree<XOps<F>, BLOCK>(sh);
-    if (threadIdx.x == 0) st_xyzz(buckets, k, sh[0]);
+    block_tree<O, BLOCK>(sh);
+    if (threadIdx.x == 0) O::store(buckets, k, sh[0]);
     __syncthreads();
   }
 }
@@ -589,7 +625,7 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_finish(const uint32_t* offset,
 // buckets: 8192 groups) costs two short trees rather than one long strided chain. Every
 // chain is short; the doublings of the bit weights happen once, in the host's Horner pass
 // over all windows.
-template <class F, class Ops = XOps<F>>
+template <class F, class Ops = LOps<F>>
 __global__ void __launch_bounds__(128) k_reduce_groups(const uint4* buckets, uint32_t nb,
                                                        int W, uint32_t B, uint4* T,
                                                        uint4* U) {
@@ -608,7 +644,7 @@ __global__ void __launch_bounds__(128) k_reduce_groups(const uint4* buckets, uin
 
 // Workgroup x = ((w rows + r) S + slice); writes out[x] (the host's XYZZ when S = 1, else a
 // slice sum in the policy's form for k_reduce_slices).
-template <class F, int BLOCK, class Ops = XOps<F>>
+template <class F, int BLOCK, class Ops = LOps<F>>
 __global__ void __launch_bounds__(BLOCK) k_reduce_rows(const uint4* T, const uint4* U,
                                                        uint32_t G, uint32_t rows, uint32_t S,
                                                        uint4* out) {
@@ -635,7 +671,7 @@ __global__ void __launch_bounds__(BLOCK) k_reduce_rows(const uint4* T, const uin
 }
 
 // Row x = w rows + r: out[x] = sum of parts[x S .. x S + S), in the host's XYZZ.
-template <class F, int BLOCK, class Ops = XOps<F>>
+template <class F, int BLOCK, class Ops = LOps<F>>
 __global__ void __launch_bounds__(BLOCK) k_reduce_slices(const uint4* parts, uint32_t S,
                                                          uint4* out) {
   __shared__ typename Ops::P sh[BLOCK];
@@ -824,7 +860,7 @@ template <class F>
 inline void tail_reduce_impl(const MsmPlan& p, const AccumPtrs& a, uint32_t B, void* window_out,
                              hipStream_t s) {
   if (p.constant) tail_reduce_ops<F, POps<F>>(p, a, B, window_out, s);
-  else tail_reduce_ops<F, XOps<F>>(p, a, B, window_out, s);
+  else tail_reduce_ops<F, LOps<F>>(p, a, B, window_out, s);
 }
 
 // F: the coordinate field in the MSM's Montgomery form R' = 2^261 (Fm, or Fm2i for G2); the
