@@ -43,7 +43,7 @@ MUL32_PER_MADD = {1: 10 * 136, 2: 28 * 136}
 # HBM bytes per k_accumulate<Fm> launch in the 1.1M-constraint prove (FETCH_SIZE + WRITE_SIZE,
 # scaled by the counters' reading of a 256 MiB device copy: fetch x2, write x1), from the
 # rocprofv3 --pmc runs in profiles/k_accumulate_traffic_1_1M.txt.
-ACCUMULATE_TRAFFIC_BYTES_1_1M = 3.61e9
+ACCUMULATE_TRAFFIC_BYTES_1_1M = 3.39e9
 # Reference, CUDA back end on a Tesla T4: single prove, synthetic key, domain 2^21, warm:
 # 1263 ms (bench/results/cuda-size-sweep-2026-11-10.md:26).
 REFERENCE_T4_PROVE_MS_2_21 = 1263.0
