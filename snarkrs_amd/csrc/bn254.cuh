@@ -1155,17 +1155,18 @@ HD void wide_zero(uint64_t t[17]) {
   for (int k = 0; k < 17; ++k) t[k] = 0;
 }
 
-// r = a b (r must not alias a or b).
+// r = a b (r must not alias a or b). Both components' sums are formed before either is
+// reduced, so the two reductions' serial chains can interleave.
 HD void f2l_mul(const F2L& a, const F2L& b, F2L& r) {
-  uint64_t t[17];
+  uint64_t t[17], u[17];
   wide_zero(t);
+  wide_zero(u);
   wide_mac(t, a.c0, b.c0);
+  wide_mac(u, a.c0, b.c1);
   wide_msub(t, a.c1, b.c1);
+  wide_mac(u, a.c1, b.c0);
   redc29<Fq29>(t, r.c0);
-  wide_zero(t);
-  wide_mac(t, a.c0, b.c1);
-  wide_mac(t, a.c1, b.c0);
-  redc29<Fq29, false>(t, r.c1);
+  redc29<Fq29, false>(u, r.c1);
 }
 
 // r = a^2 = (a0 + a1)(a0 - a1) + 2 a0 a1 u (r must not alias a).
