@@ -451,36 +451,38 @@ static void check_witness_len(const g16hip_circuit* C, size_t n) {
                                                  std::to_string(C->n_vars)};
 }
 
-// A host witness goes up through the slot's page-locked staging in chunks: helper threads
-// copy chunks in, and the calling thread enqueues each chunk's DMA in order as soon as it is
-// in (copying chunks itself while it waits), so the host copy overlaps the transfer. The
+// A host witness goes up through the slot's page-locked staging: the calling thread and
+// helper threads copy it in, in order, in pieces of 512 KiB, and the calling thread enqueues
+// each DMA chunk (1 MiB first, then 8 MiB) as soon as all its pieces are in (copying pieces
+// itself while it waits), so the host copy overlaps the transfer. Measured on the MI355X box:
+// one 38 MB DMA from page-locked memory runs at 57 GB/s, 8 MiB chunks at 54, 1 MiB chunks at
+// 38 (each DMA costs ~9 us); eight threads copy into page-locked memory at ~69 GB/s. The
 // staging is free at the start of a proof: a slot is leased by one proof at a time, and a
 // proof returns only after its streams have finished.
 static constexpr size_t kStagingHelpers = 7;
 static void stage_witness(Slot* S, const uint8_t* w, size_t bytes) {
-  constexpr size_t CHUNK = 1u << 20;
+  constexpr size_t PIECE = 512u << 10, FIRST = 1u << 20, CHUNK = 8u << 20;
   uint8_t* dst = S->host_witness;
   char* dev = S->witness.as<char>();
-  const size_t nch = (bytes + CHUNK - 1) / CHUNK;
-  auto len = [&](size_t c) { return std::min(CHUNK, bytes - c * CHUNK); };
-  if (nch <= 2) {
+  const size_t np = (bytes + PIECE - 1) / PIECE;
+  if (np <= 2) {
     std::memcpy(dst, w, bytes);
     G16_CHECK(hipMemcpyAsync(dev, dst, bytes, hipMemcpyHostToDevice, S->stream));
     return;
   }
   std::atomic<size_t> next{0};
-  std::unique_ptr<std::atomic<uint8_t>[]> ready(new std::atomic<uint8_t>[nch]);
-  for (size_t c = 0; c < nch; ++c) ready[c].store(0, std::memory_order_relaxed);
+  std::unique_ptr<std::atomic<uint8_t>[]> ready(new std::atomic<uint8_t>[np]);
+  for (size_t c = 0; c < np; ++c) ready[c].store(0, std::memory_order_relaxed);
   auto copy_one = [&]() -> bool {
     const size_t c = next.fetch_add(1, std::memory_order_relaxed);
-    if (c >= nch) return false;
-    std::memcpy(dst + c * CHUNK, w + c * CHUNK, len(c));
+    if (c >= np) return false;
+    std::memcpy(dst + c * PIECE, w + c * PIECE, std::min(PIECE, bytes - c * PIECE));
     ready[c].store(1, std::memory_order_release);
     return true;
   };
   // Pooled helpers (no thread is started per proof once the pool has them).
   std::vector<std::unique_ptr<HostTask>> helpers;
-  const size_t nh = std::min<size_t>(kStagingHelpers, nch - 1);
+  const size_t nh = std::min<size_t>(kStagingHelpers, np - 1);
   for (size_t h = 0; h < nh; ++h)
     helpers.push_back(std::make_unique<HostTask>([&] {
       while (copy_one()) {
@@ -488,11 +490,14 @@ static void stage_witness(Slot* S, const uint8_t* w, size_t bytes) {
     }));
   std::exception_ptr err;
   try {
-    for (size_t c = 0; c < nch; ++c) {
-      while (!ready[c].load(std::memory_order_acquire))
-        if (!copy_one()) std::this_thread::yield();
-      G16_CHECK(hipMemcpyAsync(dev + c * CHUNK, dst + c * CHUNK, len(c), hipMemcpyHostToDevice,
-                               S->stream));
+    size_t off = 0, p = 0;  // next byte to send, next piece to wait for
+    while (off < bytes) {
+      const size_t len = std::min(off == 0 ? FIRST : CHUNK, bytes - off);
+      for (; p * PIECE < off + len; ++p)
+        while (!ready[p].load(std::memory_order_acquire))
+          if (!copy_one()) std::this_thread::yield();
+      G16_CHECK(hipMemcpyAsync(dev + off, dst + off, len, hipMemcpyHostToDevice, S->stream));
+      off += len;
     }
   } catch (...) {
     err = std::current_exception();
