@@ -11,6 +11,7 @@
 // captured once as a hipGraph.
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
@@ -43,6 +44,7 @@ struct Slot {
   // staging, crates/cuda/src/stages.rs:946-985).
   uint8_t* host_witness = nullptr;
   size_t host_witness_bytes = 0;
+  double witness_host_ms = -1.0;  // the last host witness's staging time (timeline only)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   size_t bytes = 0;
@@ -506,10 +508,24 @@ static void stage_witness(Slot* S, const uint8_t* w, size_t bytes) {
   if (err) std::rethrow_exception(err);
 }
 
+static bool timeline_on() {
+  static const bool on = std::getenv("G16HIP_TIMELINE") != nullptr;
+  return on;
+}
+
 static void load_witness(const g16hip_circuit* C, Slot* S, const void* w, size_t n,
                          bool on_device) {
   check_witness_len(C, n);
-  if (on_device)
+  if (timeline_on()) {
+    G16_CHECK(hipEventRecord(S->ev[6], S->stream));
+    S->witness_host_ms = -1.0;
+  }
+  if (!on_device && timeline_on()) {
+    const auto t0 = std::chrono::steady_clock::now();
+    stage_witness(S, static_cast<const uint8_t*>(w), 32 * n);
+    S->witness_host_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  } else if (on_device)
     G16_CHECK(hipMemcpyAsync(S->witness.p, w, 32 * n, hipMemcpyDeviceToDevice, S->stream));
   else
     stage_witness(S, static_cast<const uint8_t*>(w), 32 * n);
@@ -731,12 +747,17 @@ static void run_timed(g16hip_circuit* C, Slot* S, bool with_h, void* h, g16hip_m
   for (size_t k = 0; k < kMsmCount; ++k)
     note_accumulate(C->dev, k == 3 ? 2 : 1, k == 4 ? totals[1] : totals[0],
                     ev_ms(S->mev[k][0], S->mev[k][1]));
-  static const bool timeline = std::getenv("G16HIP_TIMELINE") != nullptr;
-  if (timeline) {
+  if (timeline_on()) {
     // Milliseconds from the witness arriving on the device, per stream (the
     // G16_CUDA_MEMORY_TRACE kind of switch, crates/cuda/src/backend.rs:89-93).
     static const char* names[5] = {"A", "B1", "L", "B2", "H"};
     std::fprintf(stderr, "[g16hip timeline]");
+    if (S->witness_host_ms >= 0)
+      std::fprintf(stderr, " witness: staged on the host in %.2f, on the device %.2f after the "
+                   "first copy |", S->witness_host_ms, ev_ms(S->ev[6], S->ev[4]));
+    else
+      std::fprintf(stderr, " witness: on the device %.2f after the copy |",
+                   ev_ms(S->ev[6], S->ev[4]));
     if (with_h)
       std::fprintf(stderr, " ntt %.2f-%.2f h_done %.2f |", ev_ms(S->ev[4], S->ev[1]),
                    ev_ms(S->ev[4], S->ev[2]), ev_ms(S->ev[4], S->ev[3]));
