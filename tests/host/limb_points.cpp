@@ -1,0 +1,167 @@
+// Host check of the limb-form point arithmetic (G1L, G2L in snarkrs_amd/csrc/bn254.cuh)
+// against the word-form XYZZ formulas of the same header: chains of mixed additions with
+// random signs, repeated points (doublings) and opposite points (cancellations), general
+// additions of such sums (including p + p and p + (-p)), and every value's bound after every
+// step (G1 X < 9q, G2 X < 5q, the rest < 2q, limbs as the header states). Points are random
+// field elements, not curve points: the formulas are compared as algebra, not the group law.
+// Prints "ok <steps>" and exits 0, or the first mismatches and exits 1.
+#include <cstdio>
+#include <random>
+#include <vector>
+
+#include "../../snarkrs_amd/csrc/bn254.cuh"
+
+using namespace g16;
+
+static std::mt19937_64 rng(2024);
+
+static void rnd_below_2q(Fm& v) {
+  for (;;) {
+    for (int i = 0; i < 8; ++i) v.v[i] = (uint32_t)rng();
+    v.v[7] &= 0x7fffffffu;
+    for (int i = 7; i >= 0; --i)
+      if (v.v[i] != fqm_2q(i)) {
+        if (v.v[i] < fqm_2q(i)) return;
+        break;
+      }
+  }
+}
+
+// value(x) < k q for limbs below 2^32 (any carries)
+static bool below_kq(const uint32_t x[9], int k) {
+  int64_t kq[10], xs[10];
+  uint64_t c = 0;
+  for (int j = 0; j < 9; ++j) {
+    const uint64_t v = (uint64_t)Fq29::p29(j) * (uint64_t)k + c;
+    kq[j] = (int64_t)(v & 0x1fffffffu);
+    c = v >> 29;
+  }
+  kq[9] = (int64_t)c;
+  int64_t cc = 0;
+  for (int j = 0; j < 9; ++j) {
+    const int64_t v = (int64_t)x[j] + cc;
+    xs[j] = v & 0x1fffffff;
+    cc = v >> 29;
+  }
+  xs[9] = cc;
+  for (int j = 9; j >= 0; --j)
+    if (xs[j] != kq[j]) return xs[j] < kq[j];
+  return false;
+}
+
+static bool limbs_below(const uint32_t x[9], uint32_t lim) {
+  for (int j = 0; j < 9; ++j)
+    if (x[j] >= lim) return false;
+  return true;
+}
+
+template <class F>
+static bool same(const Xyzz<F>& p, const Xyzz<F>& q) {
+  const bool pi = is_zero(p.ZZ), qi = is_zero(q.ZZ);
+  if (pi || qi) return pi == qi;
+  return is_zero(sub(mul(p.X, q.ZZ), mul(q.X, p.ZZ))) &&
+         is_zero(sub(mul(p.Y, q.ZZZ), mul(q.Y, p.ZZZ)));
+}
+
+static long bad = 0, steps = 0;
+
+static void report(const char* what, int chain, int step) {
+  if (++bad <= 8) std::printf("%s: chain %d step %d\n", what, chain, step);
+}
+
+static bool g1_bounds(const G1L& a) {
+  return below_kq(a.X, 9) && below_kq(a.Y, 2) && below_kq(a.ZZ, 2) && below_kq(a.ZZZ, 2) &&
+         limbs_below(a.X, (1u << 29) + 8) && limbs_below(a.Y, 1u << 29) &&
+         limbs_below(a.ZZ, 1u << 29) && limbs_below(a.ZZZ, 1u << 29);
+}
+
+static bool g2_bounds(const G2L& a) {
+  const F2L* c[4] = {&a.X, &a.Y, &a.ZZ, &a.ZZZ};
+  const int k[4] = {5, 2, 2, 2};
+  for (int i = 0; i < 4; ++i)
+    for (const uint32_t* v : {c[i]->c0, c[i]->c1})
+      if (!below_kq(v, k[i]) || !limbs_below(v, 1u << 29)) return false;
+  return true;
+}
+
+// One chain of mixed additions in both forms; returns the limb sum and the word-form sum.
+template <class F, class L, class Madd, class Out, class Bounds, class Inf>
+static void chains(const char* name, const std::vector<Aff<F>>& B, int n_chains, Madd madd,
+                   Out out, Bounds bounds, Inf inf, std::vector<std::pair<L, Xyzz<F>>>* keep) {
+  for (int ch = 0; ch < n_chains; ++ch) {
+    Xyzz<F> ref = xyzz_inf<F>();
+    L acc = inf();
+    const int len = 1 + (int)(rng() % 40);
+    int prev = -1;
+    bool prev_neg = false;
+    for (int s = 0; s < len; ++s) {
+      int i = (int)(rng() % B.size());
+      bool ng = rng() & 1;
+      const int mode = (int)(rng() % 10);
+      if (mode == 0 && prev >= 0) i = prev, ng = prev_neg;                // doubling when acc = it
+      if (mode == 1 && prev >= 0) i = prev, ng = !prev_neg;               // cancellation
+      if (mode == 2) acc = inf(), ref = xyzz_inf<F>();                     // a fresh run
+      ref = xyzz_add_aff(ref, ng ? aff_neg(B[i]) : B[i]);
+      acc = madd(acc, B[i], ng);
+      ++steps;
+      if (!same(out(acc), ref)) report(name, ch, s);
+      if (!bounds(acc)) report("bounds", ch, s);
+      prev = i;
+      prev_neg = ng;
+    }
+    if (keep) keep->push_back({acc, ref});
+  }
+}
+
+template <class F, class L, class Add, class Out, class Bounds>
+static void sums(const char* name, const std::vector<std::pair<L, Xyzz<F>>>& pts, Add add,
+                 Out out, Bounds bounds) {
+  for (size_t i = 0; i + 1 < pts.size(); ++i) {
+    const auto& p = pts[i];
+    const auto& q = pts[i + 1];
+    const L r = add(p.first, q.first);
+    ++steps;
+    if (!same(out(r), xyzz_add(p.second, q.second))) report(name, (int)i, 0);
+    if (!bounds(r)) report("bounds (add)", (int)i, 0);
+    const L d = add(p.first, p.first);  // the doubling case
+    if (!same(out(d), xyzz_add(p.second, p.second))) report(name, (int)i, 1);
+    if (!bounds(d)) report("bounds (add)", (int)i, 1);
+  }
+}
+
+int main() {
+  std::vector<Aff<Fm>> B1(24);
+  for (auto& b : B1) rnd_below_2q(b.x), rnd_below_2q(b.y);
+  std::vector<std::pair<G1L, Xyzz<Fm>>> k1;
+  chains<Fm, G1L>("G1 madd", B1, 600, g1l_madd, g1l_out, g1_bounds, g1l_inf, &k1);
+  // p + (-p) through the general addition: negate Y (word form) and convert back.
+  for (size_t i = 0; i < 40 && i < k1.size(); ++i) {
+    Xyzz<Fm> w = g1l_out(k1[i].first);
+    w.Y = neg(w.Y);
+    const G1L r = g1l_add(k1[i].first, g1l_from(w));
+    ++steps;
+    if (!is_zero(g1l_out(r).ZZ) && !is_zero(k1[i].second.ZZ)) report("G1 p + (-p)", (int)i, 0);
+  }
+  sums<Fm, G1L>("G1 add", k1, g1l_add, g1l_out, g1_bounds);
+
+  std::vector<Aff<Fm2i>> B2(24);
+  for (auto& b : B2) rnd_below_2q(b.x.c0), rnd_below_2q(b.x.c1), rnd_below_2q(b.y.c0),
+      rnd_below_2q(b.y.c1);
+  std::vector<std::pair<G2L, Xyzz<Fm2i>>> k2;
+  chains<Fm2i, G2L>("G2 madd", B2, 300, g2l_madd, g2l_out, g2_bounds, g2l_inf, &k2);
+  for (size_t i = 0; i < 20 && i < k2.size(); ++i) {
+    Xyzz<Fm2i> w = g2l_out(k2[i].first);
+    w.Y = neg(w.Y);
+    const G2L r = g2l_add(k2[i].first, g2l_from(w));
+    ++steps;
+    if (!is_zero(g2l_out(r).ZZ) && !is_zero(k2[i].second.ZZ)) report("G2 p + (-p)", (int)i, 0);
+  }
+  sums<Fm2i, G2L>("G2 add", k2, g2l_add, g2l_out, g2_bounds);
+
+  if (bad) {
+    std::printf("%ld mismatches in %ld steps\n", bad, steps);
+    return 1;
+  }
+  std::printf("ok %ld\n", steps);
+  return 0;
+}
