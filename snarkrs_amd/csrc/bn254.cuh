@@ -232,12 +232,6 @@ HD Fp<C> mul(const Fp<C>& a, const Fp<C>& b) {
 // gfx950 this runs 1.42x the 32-bit form's products per second. Inputs and output are the
 // usual 8 x 32-bit limbs, the output fully reduced. M29 gives the modulus in 29-bit limbs and
 // -modulus^-1 mod 2^29.
-HD uint32_t limb29(const uint32_t v[8], int k) {
-  const int o = 29 * k, w = o >> 5, sh = o & 31;
-  const uint64_t pair = (uint64_t)v[w] | ((uint64_t)(w + 1 < 8 ? v[w + 1] : 0u) << 32);
-  return (uint32_t)(pair >> sh) & ((1u << 29) - 1u);
-}
-
 struct Fq29 {
   using Cfg = FqCfg;
   static HD uint32_t p29(int j) {
@@ -259,25 +253,42 @@ struct Fr29 {
 };
 
 // The 29-bit limbs of an 8 x 32-bit value, and back (a value below 2^256 in limbs < 2^29).
+// Every index is a template constant, so no word array is ever indexed at run time (where a
+// large kernel left the loop rolled, a run-time index put the words in scratch memory).
+template <int K>
+HD uint32_t limb29_at(const uint32_t v[8]) {  // bits [29K, 29K + 29)
+  constexpr int o = 29 * K, w = o >> 5, sh = o & 31;
+  constexpr uint32_t MASK = (1u << 29) - 1u;
+  if constexpr (sh == 0) return v[w] & MASK;
+  else if constexpr (w + 1 < 8) return ((v[w] >> sh) | (v[w + 1] << (32 - sh))) & MASK;
+  else return (v[w] >> sh) & MASK;
+}
+template <int I>
+HD uint32_t word29_at(const uint32_t R[9]) {  // bits [32I, 32I + 32): two limbs (32I mod 29 < 27)
+  constexpr int j = (32 * I) / 29, s = 32 * I - 29 * j;
+  static_assert(s < 27, "a word spans at most two limbs");
+  return (R[j] >> s) | (R[j + 1] << (29 - s));
+}
 HD void limbs29_of(const uint32_t v[8], uint32_t A[9]) {
-#pragma unroll
-  for (int k = 0; k < 9; ++k) A[k] = limb29(v, k);
+  A[0] = limb29_at<0>(v);
+  A[1] = limb29_at<1>(v);
+  A[2] = limb29_at<2>(v);
+  A[3] = limb29_at<3>(v);
+  A[4] = limb29_at<4>(v);
+  A[5] = limb29_at<5>(v);
+  A[6] = limb29_at<6>(v);
+  A[7] = limb29_at<7>(v);
+  A[8] = limb29_at<8>(v);
 }
 HD void pack29(const uint32_t R[9], uint32_t out[8]) {
-  uint64_t acc = 0;
-  int bits = 0, w = 0;
-#pragma unroll
-  for (int j = 0; j < 9; ++j) {
-    acc |= (uint64_t)R[j] << bits;
-    bits += 29;
-    if (bits >= 32) {
-      if (w < 8) out[w] = (uint32_t)acc;
-      ++w;
-      acc >>= 32;
-      bits -= 32;
-    }
-  }
-  if (w < 8) out[w] = (uint32_t)acc;
+  out[0] = word29_at<0>(R);
+  out[1] = word29_at<1>(R);
+  out[2] = word29_at<2>(R);
+  out[3] = word29_at<3>(R);
+  out[4] = word29_at<4>(R);
+  out[5] = word29_at<5>(R);
+  out[6] = word29_at<6>(R);
+  out[7] = word29_at<7>(R);
 }
 
 // The product on limbs: A B 2^-261 mod the modulus, in [0, 2 modulus) (any inputs below
