@@ -23,7 +23,7 @@ template <int = 0>
 __global__ void k_clear_cnt(uint32_t* cnt) {
   if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
 }
-static constexpr int HEAVY_SPAN = 32;   // segments a merge thread folds itself
+static constexpr int HEAVY_SPAN = 8;    // segments a merge thread folds itself
 
 __device__ __forceinline__ Fr ld_fr(const uint4* p) {
   uint4 a = p[0], b = p[1];
