@@ -495,12 +495,13 @@ __global__ void __launch_bounds__(256) k_merge(const uint32_t* offset, uint32_t 
 // 1.1M-constraint size), so a bucket gets MERGE_LANES lanes: each sums every MERGE_LANES-th of
 // the bucket's segment pieces (plast of its first segment, pfirst of the rest), then the lanes
 // of a bucket add up through LDS. 256-lane workgroups take 256 / MERGE_LANES buckets. A bucket
-// past WIDE_SPAN segments (the ones of a boolean witness pile into one) is handed to the heavy
-// path instead, as k_merge does. Four lanes measured best inside the prove (the GPU is full
+// past WIDE_SPAN segments (the ones of a boolean witness pile into one; a skewed witness's
+// small values would make more) is handed to the heavy path instead, as k_merge does, so no
+// lane folds more than WIDE_SPAN / MERGE_LANES pieces in a row. Four lanes measured best inside the prove (the GPU is full
 // there, so the tree's extra additions cost more than the shorter chains save): 16 lanes 73,
 // 8 lanes 76, 4 lanes 78, 2 lanes 77 proofs/s at the 1.1M-constraint size.
 static constexpr int MERGE_LANES = 4;
-static constexpr uint32_t WIDE_SPAN = 512;
+static constexpr uint32_t WIDE_SPAN = 64;
 
 template <class F>
 __global__ void __launch_bounds__(256) k_merge_wide(const uint32_t* offset, uint32_t K,
