@@ -310,7 +310,11 @@ __global__ void __launch_bounds__(128) k_accumulate(const uint2* __restrict__ en
 // last run. Slot rows hold the bucket key, or FOLD_NONE.
 // ---------------------------------------------------------------------------------------
 static constexpr uint32_t FOLD_NONE = 0xFFFFFFFFu;
-static constexpr uint32_t FOLD_LEN = 32;  // slots a fold thread walks
+// Slots a fold thread walks. A group emits at most two slots (its first and last run), so a
+// level shrinks the slot count only for FOLD_LEN > 2; 8 measured best (G1 MSM 2^20 4.04 ms,
+// G2 2^18 5.42 ms, against 4.74 / 7.58 at 32 and 4.03 / 5.62 at 4).
+static constexpr uint32_t FOLD_LEN = 8;
+static_assert(FOLD_LEN > 2, "each fold level must shrink the slot count");
 
 template <class F>
 __global__ void __launch_bounds__(128) k_accumulate_const(const uint2* __restrict__ ent,
@@ -491,15 +495,16 @@ __global__ void __launch_bounds__(256) k_merge(const uint32_t* offset, uint32_t 
   O::store(buckets, k, acc);
 }
 
-// Merged windows: every bucket holds many entries and spans many segments (about 40 at the
+// Merged windows: every bucket holds many entries and spans many segments (11-32 at the
 // 1.1M-constraint size), so a bucket gets MERGE_LANES lanes: each sums every MERGE_LANES-th of
 // the bucket's segment pieces (plast of its first segment, pfirst of the rest), then the lanes
 // of a bucket add up through LDS. 256-lane workgroups take 256 / MERGE_LANES buckets. A bucket
 // past WIDE_SPAN segments (the ones of a boolean witness pile into one; a skewed witness's
 // small values would make more) is handed to the heavy path instead, as k_merge does, so no
-// lane folds more than WIDE_SPAN / MERGE_LANES pieces in a row. Four lanes measured best inside the prove (the GPU is full
-// there, so the tree's extra additions cost more than the shorter chains save): 16 lanes 73,
-// 8 lanes 76, 4 lanes 78, 2 lanes 77 proofs/s at the 1.1M-constraint size.
+// lane folds more than WIDE_SPAN / MERGE_LANES pieces in a row. Four lanes measured best
+// inside the prove (the GPU is full there, so the tree's extra additions cost more than the
+// shorter chains save): 16 lanes 73, 8 lanes 76, 4 lanes 78, 2 lanes 77 proofs/s at the
+// 1.1M-constraint size.
 static constexpr int MERGE_LANES = 4;
 static constexpr uint32_t WIDE_SPAN = 64;
 
