@@ -1016,6 +1016,13 @@ static void wave_device(g16hip_circuit* C, Wave* W, uint32_t m, int b) {
   hipStream_t s = W->stream;
   G16_CHECK(hipMemcpyAsync(W->witness.p, W->staging[b], wb * m, hipMemcpyHostToDevice, s));
   WavePlans P = wave_plans(C, W->sort_w.p, W->sort_h.p, m);
+  // The buffers are sized for the wave's cap; a plan for m <= cap proofs must fit them.
+  if (msm_accum_scratch_bytes(P.w, 1) > W->accum_g1[0].bytes ||
+      msm_accum_scratch_bytes(P.h, 1) > W->accum_g1[3].bytes ||
+      msm_accum_scratch_bytes(P.w, 2) > W->accum_w2.bytes)
+    throw StatusError{G16HIP_BACKEND, "a wave of " + std::to_string(m) +
+                                          " proofs needs more accumulation scratch than its "
+                                          "cap's buffers hold"};
   const uint32_t* rp[2] = {C->row_ptr[0].as<uint32_t>(), C->row_ptr[1].as<uint32_t>()};
   const uint32_t* sg[2] = {C->signal[0].as<uint32_t>(), C->signal[1].as<uint32_t>()};
   const uint32_t* vi[2] = {C->value_id[0].as<uint32_t>(), C->value_id[1].as<uint32_t>()};
