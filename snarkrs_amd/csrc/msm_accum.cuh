@@ -822,7 +822,11 @@ inline void tail_merge_impl(const MsmPlan& p, const AccumPtrs& a, hipStream_t s)
   size_t K = (size_t)msm_bucket_sets(p) * p.nb;
   hipLaunchKernelGGL(k_clear_cnt<0>, dim3(1), dim3(64), 0, s, a.cnt);
   G16_LAUNCH_CHECK("msm clear counters");
-  if (p.merged) {
+  // Merged windows give a bucket many segments in a large MSM (11-32 at the 1.1M-constraint
+  // size: MERGE_LANES lanes each); a wave of small proofs gives most buckets one segment or
+  // none, where lanes that mostly exit at once only cost launches: one thread per bucket then.
+  const size_t avg_segs = p.max_entries / ((size_t)a.seg * (K ? K : 1));
+  if (p.merged && avg_segs >= 4) {
     constexpr unsigned per_block = 256 / MERGE_LANES;
     hipLaunchKernelGGL(k_merge_wide<F>, dim3((unsigned)((K + per_block - 1) / per_block)),
                        dim3(256), 0, s, p.offset, (uint32_t)K, a.buckets, a.pfirst, a.plast,
