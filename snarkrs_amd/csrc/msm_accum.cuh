@@ -25,7 +25,10 @@ template <int = 0>
 __global__ void k_clear_cnt(uint32_t* cnt) {
   if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
 }
-static constexpr int HEAVY_SPAN = 8;    // segments a merge thread folds itself
+// Segments a merge thread folds itself, as a chain of dependent additions; a bucket past it
+// goes to the heavy path's parallel trees. At 32 a few dense buckets (the narrow top window of
+// the standalone G2 MSM) kept the merge running ~450 us with most SIMDs idle.
+static constexpr int HEAVY_SPAN = 8;
 
 __device__ __forceinline__ Fr ld_fr(const uint4* p) {
   uint4 a = p[0], b = p[1];
