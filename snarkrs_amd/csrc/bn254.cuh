@@ -1575,6 +1575,158 @@ HD Proj<F> proj_add_aff(const Proj<F>& p, const Aff<F>& a) {
   return r;
 }
 
+// G1's complete formulas on 29-bit limbs (constant work): (X : Y : Z) with every coordinate
+// below 2q in exact limbs, as the products leave them; the same algorithms, values mod q and
+// outputs as proj_add / proj_add_aff. b3 = 9 is a product with 9 2^261 mod q, which leaves b3
+// times a value below 2q. Bounds, for inputs below 2q: t0, t1, t2 < 2q (3 t0 < 6q); the cross
+// terms (cross29) t3 and, in Algorithm 7, t4, t5 lie in (q, 7q); Algorithm 8's t4, t5 (a
+// product plus a coordinate) < 4q; s = t1 + b3 t2 < 4q, d = t1 - b3 t2 + 3q < 5q. The output
+// sums stay below 70 q^2 (a reduction takes 169 q^2 = 2^261 q) and every factor is carried
+// to limbs below 2^29 + 8 first, so a column holds at most 18 such products.
+struct P1L {
+  uint32_t X[9], Y[9], Z[9];
+};
+
+HD uint32_t nine29(int j) {  // 9 2^261 mod q
+  constexpr uint32_t K[9] = {0x1069329bu, 0x12f4a0b0u, 0x01fe70d2u, 0x0601df46u, 0x14b33a91u,
+                             0x019dc5bfu, 0x1f31e9c5u, 0x11c8b884u, 0x001b414au};
+  return K[j];
+}
+
+HD void mul_b3_29(const uint32_t a[9], uint32_t r[9]) {  // 9 a, below 2q for a < 169q
+  uint32_t k[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) k[j] = nine29(j);
+  mont29_limbs<Fq29>(a, k, r);
+}
+
+// a0 b1 + a1 b0 as (a0 + a1)(b0 + b1) - p0 - p1 + 5q, for a0, a1, b0, b1 below 2q and
+// p0 = a0 b0, p1 = a1 b1 in exact limbs (below 2q): the value lies in (q, 7q).
+HD void cross29(const uint32_t a0[9], const uint32_t a1[9], const uint32_t b0[9],
+                const uint32_t b1[9], const uint32_t p0[9], const uint32_t p1[9],
+                uint32_t r[9]) {
+  uint32_t u[9], v[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    u[j] = a0[j] + a1[j];
+    v[j] = b0[j] + b1[j];
+  }
+  norm29(u);
+  norm29(v);
+  mont29_limbs<Fq29>(u, v, r);  // 16 q^2 < 2^261 q: below 2q
+#pragma unroll
+  for (int j = 0; j < 9; ++j) r[j] = r[j] + Kq5::k(j) - p0[j] - p1[j];
+  norm29(r);
+}
+
+HD P1L p1l_inf() {
+  P1L r;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    r.X[j] = r.Z[j] = 0;
+    r.Y[j] = one29(j);
+  }
+  return r;
+}
+
+HD P1L p1l_sel(bool c, const P1L& a, const P1L& b) {
+  P1L r;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    r.X[j] = c ? a.X[j] : b.X[j];
+    r.Y[j] = c ? a.Y[j] : b.Y[j];
+    r.Z[j] = c ? a.Z[j] : b.Z[j];
+  }
+  return r;
+}
+
+HD P1L p1l_from(const Proj<Fm>& p) {
+  P1L r;
+  limbs29_of(p.X.v, r.X);
+  limbs29_of(p.Y.v, r.Y);
+  limbs29_of(p.Z.v, r.Z);
+  return r;
+}
+
+HD Proj<Fm> p1l_out(const P1L& p) {
+  Proj<Fm> r;
+  pack29(p.X, r.X.v);
+  pack29(p.Y, r.Y.v);
+  pack29(p.Z, r.Z.v);
+  return r;
+}
+
+// Both algorithms' common end: from t0 = X1 X2, t1 = Y1 Y2, t2 = Z1 Z2 (Algorithm 8: Z1) and
+// the carried t3, t4, t5 it forms 3 t0, b3 t2, b3 t5 and the three output sums.
+HD P1L p1l_finish(uint32_t t0[9], const uint32_t t1[9], const uint32_t t2[9],
+                  const uint32_t t3[9], const uint32_t t4[9], const uint32_t t5[9]) {
+  uint32_t b[9], s[9], d[9], b5[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) t0[j] *= 3u;
+  norm29(t0);         // 3 t0 < 6q
+  mul_b3_29(t2, b);   // b3 t2 < 2q
+#pragma unroll
+  for (int j = 0; j < 9; ++j) s[j] = t1[j] + b[j];
+  norm29(s);             // s = t1 + b3 t2 < 4q
+  sub29<Kq3>(t1, b, d);  // d = t1 - b3 t2 + 3q < 5q
+  mul_b3_29(t5, b5);     // b3 t5 < 2q
+  SB();
+  P1L r;
+  uint64_t w[17];
+  wide_zero(w);
+  wide_mac(w, t3, d);
+  wide_msub(w, t4, b5);
+  redc29<Fq29>(w, r.X);  // X3 = t3 d - t4 b3 t5, |.| < 49 q^2
+  SB();
+  wide_zero(w);
+  wide_mac(w, s, d);
+  wide_mac(w, b5, t0);
+  redc29<Fq29, false>(w, r.Y);  // Y3 = s d + b3 t5 3 t0 < 32 q^2
+  SB();
+  wide_zero(w);
+  wide_mac(w, t4, s);
+  wide_mac(w, t0, t3);
+  redc29<Fq29, false>(w, r.Z);  // Z3 = t4 s + 3 t0 t3 < 70 q^2
+  return r;
+}
+
+// Algorithm 8: p + (x2, y2), an affine point (never the identity) with limbs below 2q.
+HD P1L p1l_add_aff(const P1L& p, const uint32_t x2[9], const uint32_t y2[9]) {
+  uint32_t t0[9], t1[9], t3[9], t4[9], t5[9];
+  mont29_limbs<Fq29>(p.X, x2, t0);
+  mont29_limbs<Fq29>(p.Y, y2, t1);
+  SB();
+  cross29(p.X, p.Y, x2, y2, t0, t1, t3);  // X1 y2 + Y1 x2
+  SB();
+  mont29_limbs<Fq29>(y2, p.Z, t4);
+  mont29_limbs<Fq29>(x2, p.Z, t5);
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    t4[j] += p.Y[j];  // Y1 + y2 Z1 < 4q
+    t5[j] += p.X[j];  // X1 + x2 Z1 < 4q
+  }
+  norm29(t4);
+  norm29(t5);
+  SB();
+  return p1l_finish(t0, t1, p.Z, t3, t4, t5);
+}
+
+// Algorithm 7: p + q for any two points.
+HD P1L p1l_add(const P1L& p, const P1L& q) {
+  uint32_t t0[9], t1[9], t2[9], t3[9], t4[9], t5[9];
+  mont29_limbs<Fq29>(p.X, q.X, t0);
+  mont29_limbs<Fq29>(p.Y, q.Y, t1);
+  mont29_limbs<Fq29>(p.Z, q.Z, t2);
+  SB();
+  cross29(p.X, p.Y, q.X, q.Y, t0, t1, t3);  // X1 Y2 + Y1 X2
+  SB();
+  cross29(p.Y, p.Z, q.Y, q.Z, t1, t2, t4);  // Y1 Z2 + Z1 Y2
+  SB();
+  cross29(p.X, p.Z, q.X, q.Z, t0, t2, t5);  // X1 Z2 + Z1 X2
+  SB();
+  return p1l_finish(t0, t1, t2, t3, t4, t5);
+}
+
 // (X : Y : Z) as XYZZ (X Z, Y Z^2, Z^2, Z^3): the identity (Z = 0) becomes XYZZ's (ZZ = 0).
 template <class F>
 HD Xyzz<F> proj_to_xyzz(const Proj<F>& p) {

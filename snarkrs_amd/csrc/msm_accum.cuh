@@ -136,6 +136,33 @@ struct POps {
     st_xyzz(base, i, Xyzz<F>{p.X, p.Y, p.Z, f_zero<F>()});
   }
   static __device__ __forceinline__ Xyzz<F> out(const P& p) { return proj_to_xyzz(p); }
+  static __device__ __forceinline__ P sel(bool c, const P& a, const P& b) { return proj_sel(c, a, b); }
+  static __device__ __forceinline__ P add_aff(const P& p, const Aff<F>& a) { return proj_add_aff(p, a); }
+};
+
+// The constant-work kernels' arithmetic: POps, or for G1 the complete formulas on 29-bit limbs
+// (P1L), loaded from and stored to the same (X, Y, Z, 0) slots (words below 2q).
+template <class F>
+struct COps : POps<F> {};
+template <>
+struct COps<Fm> {
+  using P = P1L;
+  static __device__ __forceinline__ P inf() { return p1l_inf(); }
+  static __device__ __forceinline__ P add(const P& a, const P& b) { return p1l_add(a, b); }
+  static __device__ __forceinline__ P load(const uint4* base, size_t i) {
+    return p1l_from(POps<Fm>::load(base, i));
+  }
+  static __device__ __forceinline__ void store(uint4* base, size_t i, const P& p) {
+    POps<Fm>::store(base, i, p1l_out(p));
+  }
+  static __device__ __forceinline__ Xyzz<Fm> out(const P& p) { return proj_to_xyzz(p1l_out(p)); }
+  static __device__ __forceinline__ P sel(bool c, const P& a, const P& b) { return p1l_sel(c, a, b); }
+  static __device__ __forceinline__ P add_aff(const P& p, const Aff<Fm>& a) {
+    uint32_t x[9], y[9];
+    limbs29_of(a.x.v, x);
+    limbs29_of(a.y.v, y);
+    return p1l_add_aff(p, x, y);
+  }
 };
 
 // The tails' additions (merges, heavy buckets, reduction): XOps, or for G1 and G2 the limb
@@ -328,7 +355,7 @@ __global__ void __launch_bounds__(128) k_accumulate_const(const uint2* __restric
                                                           const uint4* __restrict__ bases,
                                                           uint4* buckets, uint4* slots,
                                                           uint32_t* rows) {
-  using O = POps<F>;
+  using O = COps<F>;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nseg) return;
   const size_t b = t * seg, T = total[0];  // the plan's entry count: b < T for t < nseg
@@ -340,8 +367,8 @@ __global__ void __launch_bounds__(128) k_accumulate_const(const uint2* __restric
   const uint32_t len = (uint32_t)(b + seg < T ? seg : T - b);
   uint32_t cur = ent[b].x;
   bool first = true;
-  const Proj<F> inf = O::inf();
-  Proj<F> acc = inf;
+  const typename O::P inf = O::inf();
+  typename O::P acc = inf;
   for (uint32_t i = 0; i < seg; ++i) {
     const bool live = i < len;
     const uint2 e = ent[b + (live ? i : len - 1)];
@@ -352,13 +379,13 @@ __global__ void __launch_bounds__(128) k_accumulate_const(const uint2* __restric
     O::store(to_bucket ? buckets : slots, to_bucket ? cur : slot, acc);
     rows[slot] = cur;
     first = first && !boundary;
-    acc = proj_sel(boundary, inf, acc);
+    acc = O::sel(boundary, inf, acc);
     cur = key;
     Aff<F> a = ld_aff<F>(bases, e.y >> 2);
     const bool keep = !live || (e.y & 2u) || aff_is_inf(a);
     a.y = f_sel((e.y & 1u) != 0, neg(a.y), a.y);
-    const Proj<F> sum = proj_add_aff(acc, a);
-    acc = proj_sel(keep, acc, sum);
+    const typename O::P sum = O::add_aff(acc, a);
+    acc = O::sel(keep, acc, sum);
   }
   const size_t slot = 2 * t + (first ? 0 : 1);
   O::store(slots, slot, acc);
@@ -375,15 +402,15 @@ template <class F>
 __global__ void __launch_bounds__(128) k_fold_const(const uint4* in, const uint32_t* in_rows,
                                                     uint32_t m, uint4* out, uint32_t* out_rows,
                                                     uint4* buckets, int last) {
-  using O = POps<F>;
+  using O = COps<F>;
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t groups = (m + FOLD_LEN - 1) / FOLD_LEN;
   if (g >= groups) return;
   const uint32_t lo = g * FOLD_LEN, hi = lo + FOLD_LEN < m ? lo + FOLD_LEN : m;
   uint32_t cur = FOLD_NONE;
   bool first = true;
-  const Proj<F> inf = O::inf();
-  Proj<F> acc = inf;
+  const typename O::P inf = O::inf();
+  typename O::P acc = inf;
   for (uint32_t i = lo; i < lo + FOLD_LEN; ++i) {
     const bool live = i < hi;
     const uint32_t j = live ? i : hi - 1;
@@ -397,9 +424,9 @@ __global__ void __launch_bounds__(128) k_fold_const(const uint4* in, const uint3
     O::store(to_bucket ? buckets : out, to_bucket ? cur : slot, acc);
     out_rows[slot] = cur;
     first = first && !finished;
-    acc = proj_sel(boundary, inf, acc);
+    acc = O::sel(boundary, inf, acc);
     cur = boundary ? r : cur;
-    const Proj<F> q = proj_sel(empty, inf, O::load(in, j));
+    const typename O::P q = O::sel(empty, inf, O::load(in, j));
     acc = O::add(acc, q);
   }
   const bool has_run = cur != FOLD_NONE;
@@ -874,7 +901,7 @@ inline void tail_reduce_ops(const MsmPlan& p, const AccumPtrs& a, uint32_t B, vo
 template <class F>
 inline void tail_reduce_impl(const MsmPlan& p, const AccumPtrs& a, uint32_t B, void* window_out,
                              hipStream_t s) {
-  if (p.constant) tail_reduce_ops<F, POps<F>>(p, a, B, window_out, s);
+  if (p.constant) tail_reduce_ops<F, COps<F>>(p, a, B, window_out, s);
   else tail_reduce_ops<F, LOps<F>>(p, a, B, window_out, s);
 }
 

@@ -1,8 +1,9 @@
-// Host check of the limb-form point arithmetic (G1L, G2L in snarkrs_amd/csrc/bn254.cuh)
+// Host check of the limb-form point arithmetic (G1L, G2L, P1L in snarkrs_amd/csrc/bn254.cuh)
 // against the word-form XYZZ formulas of the same header: chains of mixed additions with
 // random signs, repeated points (doublings) and opposite points (cancellations), general
 // additions of such sums (including p + p and p + (-p)), and every value's bound after every
-// step (G1 X < 9q, G2 X < 5q, the rest < 2q, limbs as the header states). Points are random
+// step (G1 X < 9q, G2 X < 5q, the rest < 2q, limbs as the header states); G1's complete
+// formulas (P1L) against proj_add_aff / proj_add, coordinate by coordinate. Points are random
 // field elements, not curve points: the formulas are compared as algebra, not the group law.
 // Prints "ok <steps>" and exits 0, or the first mismatches and exits 1.
 #include <cstdio>
@@ -129,6 +130,73 @@ static void sums(const char* name, const std::vector<std::pair<L, Xyzz<F>>>& pts
   }
 }
 
+// G1's complete formulas (P1L) against proj_add_aff / proj_add: the same algorithms, so every
+// coordinate agrees mod q, not only the point.
+static bool same_proj(const Proj<Fm>& a, const Proj<Fm>& b) {
+  return is_zero(sub(a.X, b.X)) && is_zero(sub(a.Y, b.Y)) && is_zero(sub(a.Z, b.Z));
+}
+
+static bool p1_bounds(const P1L& a) {
+  return below_kq(a.X, 2) && below_kq(a.Y, 2) && below_kq(a.Z, 2) &&
+         limbs_below(a.X, 1u << 29) && limbs_below(a.Y, 1u << 29) && limbs_below(a.Z, 1u << 29);
+}
+
+static P1L p1_add_aff(const P1L& p, const Aff<Fm>& a) {
+  uint32_t x[9], y[9];
+  limbs29_of(a.x.v, x);
+  limbs29_of(a.y.v, y);
+  return p1l_add_aff(p, x, y);
+}
+
+static void complete_g1(const std::vector<Aff<Fm>>& B) {
+  std::vector<std::pair<P1L, Proj<Fm>>> keep;
+  for (int ch = 0; ch < 400; ++ch) {
+    Proj<Fm> ref = proj_inf<Fm>();
+    P1L acc = p1l_inf();
+    const int len = 1 + (int)(rng() % 40);
+    int prev = -1;
+    bool prev_neg = false;
+    for (int s = 0; s < len; ++s) {
+      int i = (int)(rng() % B.size());
+      bool ng = rng() & 1;
+      const int mode = (int)(rng() % 10);
+      if (mode == 0 && prev >= 0) i = prev, ng = prev_neg;
+      if (mode == 1 && prev >= 0) i = prev, ng = !prev_neg;
+      const Aff<Fm> a = ng ? aff_neg(B[i]) : B[i];
+      ref = proj_add_aff(ref, a);
+      acc = p1_add_aff(acc, a);
+      ++steps;
+      if (!same_proj(p1l_out(acc), ref)) report("G1 complete add_aff", ch, s);
+      if (!p1_bounds(acc)) report("bounds (complete)", ch, s);
+      prev = i;
+      prev_neg = ng;
+    }
+    keep.push_back({acc, ref});
+  }
+  for (size_t i = 0; i + 1 < keep.size(); ++i) {
+    const auto& p = keep[i];
+    const auto& q = keep[i + 1];
+    for (int k = 0; k < 2; ++k) {
+      const auto& r = k ? p : q;
+      const P1L s = p1l_add(p.first, r.first);
+      ++steps;
+      if (!same_proj(p1l_out(s), proj_add(p.second, r.second))) report("G1 complete add", (int)i, k);
+      if (!p1_bounds(s)) report("bounds (complete add)", (int)i, k);
+    }
+  }
+  // Every coordinate at 2q - 1, the largest value the form holds.
+  Fm big;
+  for (int i = 0; i < 8; ++i) big.v[i] = fqm_2q(i);
+  big.v[0] -= 1;
+  const Proj<Fm> pb{big, big, big};
+  const P1L lb = p1l_from(pb);
+  steps += 2;
+  if (!same_proj(p1l_out(p1_add_aff(lb, Aff<Fm>{big, big})), proj_add_aff(pb, Aff<Fm>{big, big})))
+    report("G1 complete add_aff (2q - 1)", 0, 0);
+  if (!same_proj(p1l_out(p1l_add(lb, lb)), proj_add(pb, pb)))
+    report("G1 complete add (2q - 1)", 0, 0);
+}
+
 int main() {
   std::vector<Aff<Fm>> B1(24);
   for (auto& b : B1) rnd_below_2q(b.x), rnd_below_2q(b.y);
@@ -143,6 +211,7 @@ int main() {
     if (!is_zero(g1l_out(r).ZZ) && !is_zero(k1[i].second.ZZ)) report("G1 p + (-p)", (int)i, 0);
   }
   sums<Fm, G1L>("G1 add", k1, g1l_add, g1l_out, g1_bounds);
+  complete_g1(B1);
 
   std::vector<Aff<Fm2i>> B2(24);
   for (auto& b : B2) rnd_below_2q(b.x.c0), rnd_below_2q(b.x.c1), rnd_below_2q(b.y.c0),
