@@ -1508,19 +1508,20 @@ template <> struct CurveB3<Fm> {
   }
 };
 template <> struct CurveB3<Fm2i> {
-  static HD Fm2i times(const Fm2i& x) {
+  static HD Fm2i k() {  // components below q
     constexpr uint32_t C0[8] = {0x295cc5c1u, 0x582d62c9u, 0x8acc80b0u, 0x9042cb3du,
                                 0x9e1cb150u, 0xf6155910u, 0x1a987d08u, 0x1b2ff417u};
     constexpr uint32_t C1[8] = {0x01e5cc12u, 0xdeb7b4a1u, 0x9c61623au, 0x0b4079f4u,
                                 0xfd7d8f2bu, 0x4b0dc77fu, 0x24d1bcddu, 0x17dd1076u};
-    Fm2i k;
+    Fm2i r;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-      k.c0.v[i] = C0[i];
-      k.c1.v[i] = C1[i];
+      r.c0.v[i] = C0[i];
+      r.c1.v[i] = C1[i];
     }
-    return mul(x, k);
+    return r;
   }
+  static HD Fm2i times(const Fm2i& x) { return mul(x, k()); }
 };
 
 // Algorithm 7: P + Q for any two points (12 products and 2 by b3).
@@ -1725,6 +1726,215 @@ HD P1L p1l_add(const P1L& p, const P1L& q) {
   cross29(p.X, p.Z, q.X, q.Z, t0, t2, t5);  // X1 Z2 + Z1 X2
   SB();
   return p1l_finish(t0, t1, t2, t3, t4, t5);
+}
+
+// G2's complete formulas on limbs (P2L), the same steps over Fq2 with the bounds above:
+// components below 2q, the cross terms Karatsuba's (q, 7q), and each output component one wide
+// sum of four limb products. A component with two products of each sign stays within 18
+// products per column and sign (redc29); the all-positive second components (Y3: 64 q^2,
+// Z3: 140 q^2 in Algorithm 7) hold 36 products per column and reduce unsigned (redc29u).
+struct P2L {
+  F2L X, Y, Z;
+};
+
+// redc29 for a wide value whose columns are all sums of products (no subtraction): up to 54
+// limb products a column, so that with the reduction's nine they stay below 2^64, and the
+// carries shift unsigned. The value is below 2^261 q, the result below 2q.
+template <class M29>
+HD void redc29u(uint64_t t[17], uint32_t R[9]) {
+  constexpr uint32_t MASK = (1u << 29) - 1u;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const uint32_t m = ((uint32_t)t[i] * M29::N29) & MASK;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) t[i + j] += (uint64_t)m * M29::p29(j);
+    if (i < 8) t[i + 1] += t[i] >> 29;
+  }
+  uint64_t c = t[8] >> 29;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    const uint64_t x = (j < 8 ? t[9 + j] : 0) + c;
+    R[j] = (uint32_t)x & MASK;
+    c = x >> 29;
+  }
+}
+
+HD void f2l_b3(F2L& k) { f2l_of(CurveB3<Fm2i>::k(), k); }
+
+HD P2L p2l_inf() {
+  P2L r;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    r.X.c0[j] = r.X.c1[j] = r.Z.c0[j] = r.Z.c1[j] = r.Y.c1[j] = 0;
+    r.Y.c0[j] = one29(j);
+  }
+  return r;
+}
+
+HD void f2l_sel(bool c, const F2L& a, const F2L& b, F2L& r) {
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    r.c0[j] = c ? a.c0[j] : b.c0[j];
+    r.c1[j] = c ? a.c1[j] : b.c1[j];
+  }
+}
+
+HD P2L p2l_sel(bool c, const P2L& a, const P2L& b) {
+  P2L r;
+  f2l_sel(c, a.X, b.X, r.X);
+  f2l_sel(c, a.Y, b.Y, r.Y);
+  f2l_sel(c, a.Z, b.Z, r.Z);
+  return r;
+}
+
+HD P2L p2l_from(const Proj<Fm2i>& p) {
+  P2L r;
+  f2l_of(p.X, r.X);
+  f2l_of(p.Y, r.Y);
+  f2l_of(p.Z, r.Z);
+  return r;
+}
+
+HD void f2l_pack(const F2L& a, Fm2i& r) {
+  pack29(a.c0, r.c0.v);
+  pack29(a.c1, r.c1.v);
+}
+
+HD Proj<Fm2i> p2l_out(const P2L& p) {
+  Proj<Fm2i> r;
+  f2l_pack(p.X, r.X);
+  f2l_pack(p.Y, r.Y);
+  f2l_pack(p.Z, r.Z);
+  return r;
+}
+
+// cross29 over Fq2: a0 b1 + a1 b0 = (a0 + a1)(b0 + b1) - p0 - p1 (+ 5q per component).
+HD void f2l_cross(const F2L& a0, const F2L& a1, const F2L& b0, const F2L& b1, const F2L& p0,
+                  const F2L& p1, F2L& r) {
+  F2L u, v;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    u.c0[j] = a0.c0[j] + a1.c0[j];
+    u.c1[j] = a0.c1[j] + a1.c1[j];
+    v.c0[j] = b0.c0[j] + b1.c0[j];
+    v.c1[j] = b0.c1[j] + b1.c1[j];
+  }
+  norm29(u.c0);
+  norm29(u.c1);
+  norm29(v.c0);
+  norm29(v.c1);
+  f2l_mul(u, v, r);  // components below 4q: 2 16 q^2 < 2^261 q, below 2q
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    r.c0[j] = r.c0[j] + Kq5::k(j) - p0.c0[j] - p1.c0[j];
+    r.c1[j] = r.c1[j] + Kq5::k(j) - p0.c1[j] - p1.c1[j];
+  }
+  norm29(r.c0);
+  norm29(r.c1);
+}
+
+// r = a b + c d (wide, one reduction per component).
+HD void f2l_mul2(const F2L& a, const F2L& b, const F2L& c, const F2L& d, F2L& r) {
+  uint64_t t[17], u[17];
+  wide_zero(t);
+  wide_zero(u);
+  wide_mac(t, a.c0, b.c0);
+  wide_msub(t, a.c1, b.c1);
+  wide_mac(t, c.c0, d.c0);
+  wide_msub(t, c.c1, d.c1);
+  wide_mac(u, a.c0, b.c1);
+  wide_mac(u, a.c1, b.c0);
+  wide_mac(u, c.c0, d.c1);
+  wide_mac(u, c.c1, d.c0);
+  redc29<Fq29>(t, r.c0);
+  redc29u<Fq29>(u, r.c1);
+}
+
+HD P2L p2l_finish(F2L& t0, const F2L& t1, const F2L& t2, const F2L& t3, const F2L& t4,
+                  const F2L& t5) {
+  F2L k, b, s, d, b5;
+  f2l_b3(k);
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    t0.c0[j] *= 3u;
+    t0.c1[j] *= 3u;
+  }
+  norm29(t0.c0);
+  norm29(t0.c1);  // 3 t0 < 6q
+  f2l_mul(t2, k, b);  // b3 t2 < 2q
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    s.c0[j] = t1.c0[j] + b.c0[j];
+    s.c1[j] = t1.c1[j] + b.c1[j];
+  }
+  norm29(s.c0);
+  norm29(s.c1);               // s < 4q
+  sub29<Kq3>(t1.c0, b.c0, d.c0);
+  sub29<Kq3>(t1.c1, b.c1, d.c1);  // d < 5q
+  f2l_mul(t5, k, b5);         // b3 t5 < 2q
+  SB();
+  P2L r;
+  {  // X3 = t3 d - t4 b3 t5: |c0| < 49 q^2, |c1| < 70 q^2
+    uint64_t t[17], u[17];
+    wide_zero(t);
+    wide_zero(u);
+    wide_mac(t, t3.c0, d.c0);
+    wide_msub(t, t3.c1, d.c1);
+    wide_msub(t, t4.c0, b5.c0);
+    wide_mac(t, t4.c1, b5.c1);
+    wide_mac(u, t3.c0, d.c1);
+    wide_mac(u, t3.c1, d.c0);
+    wide_msub(u, t4.c0, b5.c1);
+    wide_msub(u, t4.c1, b5.c0);
+    redc29<Fq29>(t, r.X.c0);
+    redc29<Fq29>(u, r.X.c1);
+  }
+  SB();
+  f2l_mul2(s, d, b5, t0, r.Y);  // Y3 = s d + b3 t5 3 t0: |c0| < 32 q^2, c1 < 64 q^2
+  SB();
+  f2l_mul2(t4, s, t0, t3, r.Z);  // Z3 = t4 s + 3 t0 t3: |c0| < 70 q^2, c1 < 140 q^2
+  return r;
+}
+
+// Algorithm 8 over Fq2: p + (x2, y2), components below 2q (never the identity).
+HD P2L p2l_add_aff(const P2L& p, const F2L& x2, const F2L& y2) {
+  F2L t0, t1, t3, t4, t5;
+  f2l_mul(p.X, x2, t0);
+  f2l_mul(p.Y, y2, t1);
+  SB();
+  f2l_cross(p.X, p.Y, x2, y2, t0, t1, t3);  // X1 y2 + Y1 x2
+  SB();
+  f2l_mul(y2, p.Z, t4);
+  f2l_mul(x2, p.Z, t5);
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    t4.c0[j] += p.Y.c0[j];
+    t4.c1[j] += p.Y.c1[j];
+    t5.c0[j] += p.X.c0[j];
+    t5.c1[j] += p.X.c1[j];
+  }
+  norm29(t4.c0);
+  norm29(t4.c1);  // Y1 + y2 Z1 < 4q
+  norm29(t5.c0);
+  norm29(t5.c1);  // X1 + x2 Z1 < 4q
+  SB();
+  return p2l_finish(t0, t1, p.Z, t3, t4, t5);
+}
+
+// Algorithm 7 over Fq2.
+HD P2L p2l_add(const P2L& p, const P2L& q) {
+  F2L t0, t1, t2, t3, t4, t5;
+  f2l_mul(p.X, q.X, t0);
+  f2l_mul(p.Y, q.Y, t1);
+  f2l_mul(p.Z, q.Z, t2);
+  SB();
+  f2l_cross(p.X, p.Y, q.X, q.Y, t0, t1, t3);
+  SB();
+  f2l_cross(p.Y, p.Z, q.Y, q.Z, t1, t2, t4);
+  SB();
+  f2l_cross(p.X, p.Z, q.X, q.Z, t0, t2, t5);
+  SB();
+  return p2l_finish(t0, t1, t2, t3, t4, t5);
 }
 
 // (X : Y : Z) as XYZZ (X Z, Y Z^2, Z^2, Z^3): the identity (Z = 0) becomes XYZZ's (ZZ = 0).

@@ -140,10 +140,33 @@ struct POps {
   static __device__ __forceinline__ P add_aff(const P& p, const Aff<F>& a) { return proj_add_aff(p, a); }
 };
 
-// The constant-work kernels' arithmetic: POps, or for G1 the complete formulas on 29-bit limbs
-// (P1L), loaded from and stored to the same (X, Y, Z, 0) slots (words below 2q).
+// The constant-work kernels' arithmetic: the complete formulas on 29-bit limbs (P1L for G1, P2L
+// for G2; POps for any other field), loaded from and stored to the same (X, Y, Z, 0) slots
+// (words below 2q).
 template <class F>
 struct COps : POps<F> {};
+template <>
+struct COps<Fm2i> {
+  using P = P2L;
+  static __device__ __forceinline__ P inf() { return p2l_inf(); }
+  static __device__ __forceinline__ P add(const P& a, const P& b) { return p2l_add(a, b); }
+  static __device__ __forceinline__ P load(const uint4* base, size_t i) {
+    return p2l_from(POps<Fm2i>::load(base, i));
+  }
+  static __device__ __forceinline__ void store(uint4* base, size_t i, const P& p) {
+    POps<Fm2i>::store(base, i, p2l_out(p));
+  }
+  static __device__ __forceinline__ Xyzz<Fm2i> out(const P& p) {
+    return proj_to_xyzz(p2l_out(p));
+  }
+  static __device__ __forceinline__ P sel(bool c, const P& a, const P& b) { return p2l_sel(c, a, b); }
+  static __device__ __forceinline__ P add_aff(const P& p, const Aff<Fm2i>& a) {
+    F2L x, y;
+    f2l_of(a.x, x);
+    f2l_of(a.y, y);
+    return p2l_add_aff(p, x, y);
+  }
+};
 template <>
 struct COps<Fm> {
   using P = P1L;

@@ -1,9 +1,10 @@
-// Host check of the limb-form point arithmetic (G1L, G2L, P1L in snarkrs_amd/csrc/bn254.cuh)
+// Host check of the limb-form point arithmetic (G1L, G2L, P1L, P2L in snarkrs_amd/csrc/bn254.cuh)
 // against the word-form XYZZ formulas of the same header: chains of mixed additions with
 // random signs, repeated points (doublings) and opposite points (cancellations), general
 // additions of such sums (including p + p and p + (-p)), and every value's bound after every
-// step (G1 X < 9q, G2 X < 5q, the rest < 2q, limbs as the header states); G1's complete
-// formulas (P1L) against proj_add_aff / proj_add, coordinate by coordinate. Points are random
+// step (G1 X < 9q, G2 X < 5q, the rest < 2q, limbs as the header states); the complete
+// formulas on limbs (P1L, P2L) against proj_add_aff / proj_add, coordinate by coordinate,
+// including every coordinate at 2q - 1. Points are random
 // field elements, not curve points: the formulas are compared as algebra, not the group law.
 // Prints "ok <steps>" and exits 0, or the first mismatches and exits 1.
 #include <cstdio>
@@ -197,6 +198,73 @@ static void complete_g1(const std::vector<Aff<Fm>>& B) {
     report("G1 complete add (2q - 1)", 0, 0);
 }
 
+static bool same_proj(const Proj<Fm2i>& a, const Proj<Fm2i>& b) {
+  return is_zero(sub(a.X, b.X)) && is_zero(sub(a.Y, b.Y)) && is_zero(sub(a.Z, b.Z));
+}
+
+static bool p2_bounds(const P2L& a) {
+  for (const F2L* c : {&a.X, &a.Y, &a.Z})
+    for (const uint32_t* v : {c->c0, c->c1})
+      if (!below_kq(v, 2) || !limbs_below(v, 1u << 29)) return false;
+  return true;
+}
+
+static P2L p2_add_aff(const P2L& p, const Aff<Fm2i>& a) {
+  F2L x, y;
+  f2l_of(a.x, x);
+  f2l_of(a.y, y);
+  return p2l_add_aff(p, x, y);
+}
+
+static void complete_g2(const std::vector<Aff<Fm2i>>& B) {
+  std::vector<std::pair<P2L, Proj<Fm2i>>> keep;
+  for (int ch = 0; ch < 200; ++ch) {
+    Proj<Fm2i> ref = proj_inf<Fm2i>();
+    P2L acc = p2l_inf();
+    const int len = 1 + (int)(rng() % 30);
+    int prev = -1;
+    bool prev_neg = false;
+    for (int s = 0; s < len; ++s) {
+      int i = (int)(rng() % B.size());
+      bool ng = rng() & 1;
+      const int mode = (int)(rng() % 10);
+      if (mode == 0 && prev >= 0) i = prev, ng = prev_neg;
+      if (mode == 1 && prev >= 0) i = prev, ng = !prev_neg;
+      const Aff<Fm2i> a = ng ? aff_neg(B[i]) : B[i];
+      ref = proj_add_aff(ref, a);
+      acc = p2_add_aff(acc, a);
+      ++steps;
+      if (!same_proj(p2l_out(acc), ref)) report("G2 complete add_aff", ch, s);
+      if (!p2_bounds(acc)) report("bounds (complete G2)", ch, s);
+      prev = i;
+      prev_neg = ng;
+    }
+    keep.push_back({acc, ref});
+  }
+  for (size_t i = 0; i + 1 < keep.size(); ++i) {
+    const auto& p = keep[i];
+    const auto& q = keep[i + 1];
+    for (int k = 0; k < 2; ++k) {
+      const auto& r = k ? p : q;
+      const P2L s = p2l_add(p.first, r.first);
+      ++steps;
+      if (!same_proj(p2l_out(s), proj_add(p.second, r.second))) report("G2 complete add", (int)i, k);
+      if (!p2_bounds(s)) report("bounds (complete G2 add)", (int)i, k);
+    }
+  }
+  Fm big;
+  for (int i = 0; i < 8; ++i) big.v[i] = fqm_2q(i);
+  big.v[0] -= 1;
+  const Fm2i b2{big, big};
+  const Proj<Fm2i> pb{b2, b2, b2};
+  const P2L lb = p2l_from(pb);
+  steps += 2;
+  if (!same_proj(p2l_out(p2_add_aff(lb, Aff<Fm2i>{b2, b2})), proj_add_aff(pb, Aff<Fm2i>{b2, b2})))
+    report("G2 complete add_aff (2q - 1)", 0, 0);
+  if (!same_proj(p2l_out(p2l_add(lb, lb)), proj_add(pb, pb)))
+    report("G2 complete add (2q - 1)", 0, 0);
+}
+
 int main() {
   std::vector<Aff<Fm>> B1(24);
   for (auto& b : B1) rnd_below_2q(b.x), rnd_below_2q(b.y);
@@ -226,6 +294,7 @@ int main() {
     if (!is_zero(g2l_out(r).ZZ) && !is_zero(k2[i].second.ZZ)) report("G2 p + (-p)", (int)i, 0);
   }
   sums<Fm2i, G2L>("G2 add", k2, g2l_add, g2l_out, g2_bounds);
+  complete_g2(B2);
 
   if (bad) {
     std::printf("%ld mismatches in %ld steps\n", bad, steps);
