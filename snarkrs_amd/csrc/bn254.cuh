@@ -1578,27 +1578,38 @@ HD Proj<F> proj_add_aff(const Proj<F>& p, const Aff<F>& a) {
 
 // G1's complete formulas on 29-bit limbs (constant work): (X : Y : Z) with every coordinate
 // below 2q in exact limbs, as the products leave them; the same algorithms, values mod q and
-// outputs as proj_add / proj_add_aff. b3 = 9 is a product with 9 2^261 mod q, which leaves b3
-// times a value below 2q. Bounds, for inputs below 2q: t0, t1, t2 < 2q (3 t0 < 6q); the cross
-// terms (cross29) t3 and, in Algorithm 7, t4, t5 lie in (q, 7q); Algorithm 8's t4, t5 (a
-// product plus a coordinate) < 4q; s = t1 + b3 t2 < 4q, d = t1 - b3 t2 + 3q < 5q. The output
+// outputs as proj_add / proj_add_aff. b3 = 9 is nine limb multiplies and a subtraction of a
+// multiple of q (mul_b3_29), below 2q. Bounds, for inputs below 2q: t0, t1, t2 < 2q
+// (3 t0 < 6q); the cross terms (cross29) t3 and, in Algorithm 7, t4, t5 lie in (q, 7q);
+// Algorithm 8's t4, t5 (a product plus a coordinate) < 4q; s = t1 + b3 t2 < 4q,
+// d = t1 - b3 t2 + 3q < 5q. The output
 // sums stay below 70 q^2 (a reduction takes 169 q^2 = 2^261 q) and every factor is carried
 // to limbs below 2^29 + 8 first, so a column holds at most 18 such products.
 struct P1L {
   uint32_t X[9], Y[9], Z[9];
 };
 
-HD uint32_t nine29(int j) {  // 9 2^261 mod q
-  constexpr uint32_t K[9] = {0x1069329bu, 0x12f4a0b0u, 0x01fe70d2u, 0x0601df46u, 0x14b33a91u,
-                             0x019dc5bfu, 0x1f31e9c5u, 0x11c8b884u, 0x001b414au};
-  return K[j];
-}
-
-HD void mul_b3_29(const uint32_t a[9], uint32_t r[9]) {  // 9 a, below 2q for a < 169q
-  uint32_t k[9];
+// 9 a mod q for a below 7q (limbs below 2^29 + 8), in exact limbs below 2q: v = 9 a exactly
+// (below 63q < 2^261), less k q with k = floor(T / (qt + 1)), T and qt the top limbs of v and
+// q. Then k <= T 2^232 / q <= v / q, and v / q - k < (T + qt + 1) / (qt (qt + 1)) + 1, which
+// for T < 64 qt is below 1 + 2^-15.
+HD void mul_b3_29(const uint32_t a[9], uint32_t r[9]) {
+  constexpr uint32_t MASK = (1u << 29) - 1u;
+  uint64_t c = 0;
 #pragma unroll
-  for (int j = 0; j < 9; ++j) k[j] = nine29(j);
-  mont29_limbs<Fq29>(a, k, r);
+  for (int j = 0; j < 9; ++j) {
+    const uint64_t v = 9ull * a[j] + c;
+    r[j] = (uint32_t)v & MASK;
+    c = v >> 29;
+  }
+  const uint32_t k = r[8] / (Fq29::p29(8) + 1u);
+  int64_t d = 0;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    const int64_t x = (int64_t)r[j] - (int64_t)k * Fq29::p29(j) + d;
+    r[j] = (uint32_t)x & MASK;
+    d = x >> 29;
+  }
 }
 
 // a0 b1 + a1 b0 as (a0 + a1)(b0 + b1) - p0 - p1 + 5q, for a0, a1, b0, b1 below 2q and

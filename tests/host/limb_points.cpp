@@ -4,7 +4,7 @@
 // additions of such sums (including p + p and p + (-p)), and every value's bound after every
 // step (G1 X < 9q, G2 X < 5q, the rest < 2q, limbs as the header states); the complete
 // formulas on limbs (P1L, P2L) against proj_add_aff / proj_add, coordinate by coordinate,
-// including every coordinate at 2q - 1. Points are random
+// including every coordinate at 2q - 1, and G1's b3 step on its own. Points are random
 // field elements, not curve points: the formulas are compared as algebra, not the group law.
 // Prints "ok <steps>" and exits 0, or the first mismatches and exits 1.
 #include <cstdio>
@@ -149,6 +149,37 @@ static P1L p1_add_aff(const P1L& p, const Aff<Fm>& a) {
   return p1l_add_aff(p, x, y);
 }
 
+// G1's b3 = 9 (mul_b3_29) against the product with 9 2^261 mod q, for values below 7q: random
+// ones and those whose 9 a has a top limb next to a multiple of (q's top limb + 1), where the
+// quotient estimate changes.
+static void b3_g1() {
+  uint32_t nine[9];
+  limbs29_of(CurveB3<Fm>::times(f_one<Fm>()).v, nine);
+  const uint32_t qt1 = Fq29::p29(8) + 1u;
+  for (int it = 0; it < 200000; ++it) {
+    uint32_t a[9];
+    for (int j = 0; j < 8; ++j) a[j] = (uint32_t)rng() & 0x1fffffffu;
+    if (it & 1) {
+      // 9 a's top limb lands near m (qt + 1): a's top limb near m (qt + 1) / 9
+      const uint32_t m = 1 + (uint32_t)(rng() % 62);
+      const int64_t t = (int64_t)m * qt1 / 9 + (int64_t)(rng() % 5) - 2;
+      a[8] = (uint32_t)(t < 0 ? 0 : t);
+    } else {
+      a[8] = (uint32_t)(rng() % (7ull * Fq29::p29(8) + 7));
+    }
+    if (!below_kq(a, 7)) continue;
+    uint32_t r[9], ref[9];
+    mul_b3_29(a, r);
+    mont29_limbs<Fq29>(a, nine, ref);
+    Fm x, y;
+    pack29(r, x.v);
+    pack29(ref, y.v);
+    ++steps;
+    if (!below_kq(r, 2) || !limbs_below(r, 1u << 29) || !is_zero(sub(x, y)))
+      report("G1 b3", it, 0);
+  }
+}
+
 static void complete_g1(const std::vector<Aff<Fm>>& B) {
   std::vector<std::pair<P1L, Proj<Fm>>> keep;
   for (int ch = 0; ch < 400; ++ch) {
@@ -279,6 +310,7 @@ int main() {
     if (!is_zero(g1l_out(r).ZZ) && !is_zero(k1[i].second.ZZ)) report("G1 p + (-p)", (int)i, 0);
   }
   sums<Fm, G1L>("G1 add", k1, g1l_add, g1l_out, g1_bounds);
+  b3_g1();
   complete_g1(B1);
 
   std::vector<Aff<Fm2i>> B2(24);
