@@ -420,11 +420,11 @@ __global__ void __launch_bounds__(128) k_accumulate_const(const uint2* __restric
 // (exactly FOLD_LEN iterations; past m it repeats the last slot as empty), adding runs of equal
 // rows. A run that ends inside the group, other than its first, is complete (all of a bucket's
 // pieces are consecutive) and goes to its bucket; the group's first and last runs go to slots
-// 2g and 2g + 1 of the next level, or, on the last level (one group), to their buckets too.
+// 2g and 2g + 1 of the next level (k_scan_const finishes the last few hundred slots).
 template <class F>
 __global__ void __launch_bounds__(128) k_fold_const(const uint4* in, const uint32_t* in_rows,
                                                     uint32_t m, uint4* out, uint32_t* out_rows,
-                                                    uint4* buckets, int last) {
+                                                    uint4* buckets) {
   using O = COps<F>;
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t groups = (m + FOLD_LEN - 1) / FOLD_LEN;
@@ -441,7 +441,7 @@ __global__ void __launch_bounds__(128) k_fold_const(const uint4* in, const uint3
     const bool empty = r == FOLD_NONE;
     const bool boundary = !empty && r != cur;
     const bool finished = boundary && cur != FOLD_NONE;
-    const bool to_head = finished && first && !last;
+    const bool to_head = finished && first;
     const bool to_bucket = finished && !to_head;
     const size_t slot = 2 * (size_t)g + (to_head ? 0 : 1);
     O::store(to_bucket ? buckets : out, to_bucket ? cur : slot, acc);
@@ -454,10 +454,70 @@ __global__ void __launch_bounds__(128) k_fold_const(const uint4* in, const uint3
   }
   const bool has_run = cur != FOLD_NONE;
   const size_t slot = 2 * (size_t)g + (first ? 0 : 1);
-  const bool to_bucket = last && has_run;
-  O::store(to_bucket ? buckets : out, to_bucket ? cur : slot, acc);
+  O::store(out, slot, acc);
   out_rows[slot] = has_run ? cur : FOLD_NONE;
   out_rows[2 * (size_t)g + 1] = (has_run && !first) ? cur : FOLD_NONE;
+}
+
+// The fold's end, for at most BLOCK slots: one workgroup, a slot per thread. The non-empty
+// slots are packed to the front in order (their rows stay sorted, so each bucket's pieces are
+// adjacent), then a segmented inclusive scan in LDS runs one addition per level for every
+// thread: at distance d a thread adds the value d slots back where that slot has its row (the
+// slots between belong to the same run) and keeps its own otherwise. The last slot of a run
+// then holds the run's sum and stores it to its bucket. The level count depends on m only.
+// BLOCK: slots per scan, as many threads as the point form's registers allow in one workgroup.
+template <class F>
+struct ScanSlots {
+  static constexpr uint32_t N = sizeof(typename COps<F>::P) > 128 ? 256 : 512;
+};
+
+template <class F, uint32_t BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_scan_const(const uint4* in, const uint32_t* in_rows,
+                                                      uint32_t m, uint4* buckets) {
+  using O = COps<F>;
+  using P = typename O::P;
+  __shared__ P sh[BLOCK];
+  __shared__ uint32_t rows[BLOCK];
+  __shared__ uint32_t wave_count[BLOCK / 64];
+  const uint32_t i = threadIdx.x, lane = i & 63u, wave = i >> 6;
+  const bool live = i < m;
+  const uint32_t j = live ? i : m - 1;  // m >= 1
+  uint32_t r = live ? in_rows[j] : FOLD_NONE;
+  const bool has = r != FOLD_NONE;
+  P v = O::sel(has, O::load(in, j), O::inf());
+  // Pack: position = non-empty slots before this one.
+  const uint64_t ball = __ballot(has);
+  const uint32_t before = (uint32_t)__popcll(ball & ((1ull << lane) - 1ull));
+  if (lane == 0) wave_count[wave] = (uint32_t)__popcll(ball);
+  __syncthreads();
+  uint32_t pos = before, total = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < BLOCK / 64; ++w) {
+    pos += w < wave ? wave_count[w] : 0u;
+    total += wave_count[w];
+  }
+  if (has) {
+    sh[pos] = v;
+    rows[pos] = r;
+  }
+  if (i >= total) {  // positions no slot packs into
+    sh[i] = O::inf();
+    rows[i] = FOLD_NONE;
+  }
+  __syncthreads();
+  v = sh[i];
+  r = rows[i];
+  for (uint32_t d = 1; d < m; d <<= 1) {
+    __syncthreads();  // every read of the previous level is done
+    sh[i] = v;
+    __syncthreads();
+    const uint32_t k = i >= d ? i - d : i;
+    const bool take = i >= d && rows[k] == r;
+    const P sum = O::add(sh[k], v);
+    v = O::sel(take, sum, v);
+  }
+  const bool end = r != FOLD_NONE && (i + 1 == BLOCK || rows[i + 1] != r);
+  if (end) O::store(buckets, r, v);
 }
 
 // Every bucket starts as the identity (a bucket no entry names is never stored).
@@ -845,25 +905,29 @@ struct Tail {
 
 template <class F>
 inline void tail_fold_const(const AccumPtrs& a, hipStream_t s) {
-  // Levels until one group holds every slot: 2 nseg slots, then 2 per group of FOLD_LEN.
+  // Fold levels (2 nseg slots, then 2 per group of FOLD_LEN) until one workgroup's scan holds
+  // the rest.
+  constexpr uint32_t SCAN = ScanSlots<F>::N;
   size_t m = fold_slots0(a.nseg);
+  if (m == 0) return;
   const uint4* in = a.fold0;
   const uint32_t* in_rows = a.frows0;
   bool to1 = true;
-  while (m) {
+  while (m > SCAN) {
     const size_t groups = (m + FOLD_LEN - 1) / FOLD_LEN;
-    const int last = groups == 1;
     uint4* out = to1 ? a.fold1 : a.fold0;
     uint32_t* out_rows = to1 ? a.frows1 : a.frows0;
     hipLaunchKernelGGL(k_fold_const<F>, dim3((unsigned)((groups + 127) / 128)), dim3(128), 0, s,
-                       in, in_rows, (uint32_t)m, out, out_rows, a.buckets, last);
+                       in, in_rows, (uint32_t)m, out, out_rows, a.buckets);
     G16_LAUNCH_CHECK("msm fold (constant work)");
-    if (last) break;
     in = out;
     in_rows = out_rows;
     m = 2 * groups;
     to1 = !to1;
   }
+  hipLaunchKernelGGL((k_scan_const<F, SCAN>), dim3(1), dim3(SCAN), 0, s, in, in_rows,
+                     (uint32_t)m, a.buckets);
+  G16_LAUNCH_CHECK("msm fold scan (constant work)");
 }
 
 template <class F>
