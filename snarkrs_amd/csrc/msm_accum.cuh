@@ -366,8 +366,8 @@ __global__ void __launch_bounds__(128) k_accumulate(const uint2* __restrict__ en
 // ---------------------------------------------------------------------------------------
 static constexpr uint32_t FOLD_NONE = 0xFFFFFFFFu;
 // Slots a fold thread walks. A group emits at most two slots (its first and last run), so a
-// level shrinks the slot count only for FOLD_LEN > 2; 8 measured best (G1 MSM 2^20 4.04 ms,
-// G2 2^18 5.42 ms, against 4.74 / 7.58 at 32 and 4.03 / 5.62 at 4).
+// level shrinks the slot count only for FOLD_LEN > 2; 8 measured best (G1 MSM 2^20 3.08 ms,
+// G2 2^18 4.30 ms, against 3.13 / 4.38 at 4 and 3.19 / 4.59 at 16).
 static constexpr uint32_t FOLD_LEN = 8;
 static_assert(FOLD_LEN > 2, "each fold level must shrink the slot count");
 
