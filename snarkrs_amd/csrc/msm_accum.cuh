@@ -112,8 +112,9 @@ __device__ __forceinline__ void st_xyzz(uint4* base, size_t i, const Xyzz<F>& r)
 }
 
 // How the merge and reduction kernels see a point: XYZZ with the exceptional-case shortcuts
-// (XOps), or, for constant-work MSMs, (X : Y : Z) with the complete formulas (POps), stored in
-// an XYZZ-sized slot as (X, Y, Z, 0). `out` is the form the host reads (XYZZ).
+// (XOps), or, for constant-work MSMs, (X : Y : Z) with the complete formulas (POps in words;
+// COps below runs them on limbs), stored in an XYZZ-sized slot as (X, Y, Z, 0). `out` is the
+// form the host reads (XYZZ).
 template <class F>
 struct XOps {
   using P = Xyzz<F>;
