@@ -4,7 +4,9 @@ every window, so the entry count is n * windows whatever the scalars (all zero, 
 alternating, sparse -1, random), and every MSM and proof is bit-identical to the default
 path and to the oracle. The cases cover the complete formulas' exceptional inputs (every
 scalar 1: one bucket spanning every segment, folded over several levels; alternating: the
-identity and repeated points) and a batch of constant-work proofs (proved on slots)."""
+identity and repeated points), sizes from one point (two slots for the closing scan) to ones
+whose fold runs several levels before it, and a batch of constant-work proofs (proved on
+slots)."""
 import numpy as np
 import pytest
 
@@ -36,7 +38,7 @@ def _cases(n, seed):
 
 
 @pytest.mark.parametrize("group", [1, 2])
-@pytest.mark.parametrize("n", [129, 5000])
+@pytest.mark.parametrize("n", [1, 2, 129, 5000])
 def test_entries_do_not_depend_on_the_scalars(dev, orc, constant, group, n):
     rng = SplitMix64(800 + n + group)
     bases = dev.fixed_base(group, fr_array([rng.next_fr() for _ in range(n)]))
