@@ -1582,9 +1582,9 @@ HD Proj<F> proj_add_aff(const Proj<F>& p, const Aff<F>& a) {
 // multiple of q (mul_b3_29), below 2q. Bounds, for inputs below 2q: t0, t1, t2 < 2q
 // (3 t0 < 6q); the cross terms (cross29) t3 and, in Algorithm 7, t4, t5 lie in (q, 7q);
 // Algorithm 8's t4, t5 (a product plus a coordinate) < 4q; s = t1 + b3 t2 < 4q,
-// d = t1 - b3 t2 + 3q < 5q. The output
-// sums stay below 70 q^2 (a reduction takes 169 q^2 = 2^261 q) and every factor is carried
-// to limbs below 2^29 + 8 first, so a column holds at most 18 such products.
+// d = t1 - b3 t2 + 3q < 5q. The output sums stay below 70 q^2 (a reduction takes
+// 169 q^2 = 2^261 q) and every factor is carried to limbs below 2^29 + 8 first, so a column
+// holds at most 18 such products.
 struct P1L {
   uint32_t X[9], Y[9], Z[9];
 };
@@ -1740,7 +1740,8 @@ HD P1L p1l_add(const P1L& p, const P1L& q) {
 }
 
 // G2's complete formulas on limbs (P2L), the same steps over Fq2 with the bounds above:
-// components below 2q, the cross terms Karatsuba's (q, 7q), and each output component one wide
+// components below 2q, the cross terms Karatsuba's (q, 7q), b3 t2 and b3 t5 Fq2 products with
+// the constant (components below q; 2 7 1 < 169: below 2q), and each output component one wide
 // sum of four limb products. A component with two products of each sign stays within 18
 // products per column and sign (redc29); the all-positive second components (Y3: 64 q^2,
 // Z3: 140 q^2 in Algorithm 7) hold 36 products per column and reduce unsigned (redc29u).
