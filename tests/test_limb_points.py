@@ -1,9 +1,11 @@
 """The MSM kernels' limb-form point arithmetic (G1L, G2L: bucket sums, merges and reductions
-held in 29-bit limbs, bn254.cuh) against the word-form XYZZ formulas of the same header, on the
-host: mixed-addition chains with doublings and cancellations, general additions (p + q,
-p + p, p + (-p)), and every value's bound after every step. The word-form formulas are the
-ones the GPU MSM tests pin against the oracle; this pins the limb form's rare branches and
-bounds without a GPU. Built host-only with hipcc at -O0 (seconds)."""
+held in 29-bit limbs; P1L, P2L: the constant-work complete formulas, bn254.cuh) against the
+word-form formulas of the same header, on the host: mixed-addition chains with doublings and
+cancellations, general additions (p + q, p + p, p + (-p)), every value's bound after every
+step, the complete formulas coordinate by coordinate (every input at 2q - 1 included), and
+G1's b3 step on values below 7q. The word-form formulas are the ones the GPU MSM tests pin
+against the oracle; this pins the limb forms' rare branches and bounds without a GPU. Built
+host-only with hipcc at -O0 (seconds)."""
 import os
 import shutil
 import subprocess
