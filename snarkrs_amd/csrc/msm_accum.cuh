@@ -14,10 +14,14 @@ inline size_t accum_align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Entries per accumulation segment: 32, and 40 for a G1 MSM over per-window buckets, whose
 // merges are one or two additions per bucket, so fewer and longer segments pay there (the
-// 2^20 G1 MSM: 2.13 against 2.25-2.30 ms; G2 and the prove's merged MSMs were slower). The
-// length never depends on the batch: a wave's scratch is sized for its cap of proofs, and every
-// smaller wave of the circuit must cut its entries the same way to fit it.
+// 2^20 G1 MSM: 2.13 against 2.25-2.30 ms; G2 and the prove's merged MSMs were slower).
+// Constant-work MSMs over window tables (the prove's) take 64: their accumulation runs the
+// same number of rounds of waves at the prove's sizes and the fold starts from half the slots
+// (47.8-47.9 against 47.4-47.5 proofs/s). The length never depends on the batch: a wave's
+// scratch is sized for its cap of proofs, and every smaller wave of the circuit must cut its
+// entries the same way to fit it.
 inline uint32_t accum_seg(const MsmPlan& p, size_t point_bytes) {
+  if (p.constant && p.merged) return 64u;
   return !p.merged && point_bytes == 128 ? 40u : 32u;
 }
 
